@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark of the MPM timestep engine on one MI355X.
+
+Workload (default): 3D granular column collapse, 0.4 m cube, dx = 0.005 m, 8 particles per cell on a
+seeded jittered lattice (4,096,000 particles), quadratic B-spline, MUSL, APIC, Hencky + Drucker-Prager
+(30 deg, E = 1e6 Pa, nu = 0.3, rho = 1550 kg/m^3), SlipFree floor with Coulomb mu = 0.4, gravity 9.81,
+plain FP64 form. A step is one advanceStep() of all particles (+ the time bookkeeping of simulate()).
+
+Prints one JSON line: throughput in particle-updates/s (particles x steps / device-synchronised wall time of
+the timed steps), the roofline of the dominant kernel from per-kernel HIP-event durations over the timed
+region, the parity of the same configuration against the CPU oracle at a small size, and the oracle timed
+on the host cores on a bounded sample of the same workload (a reported baseline, not a target).
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "matter-tests_amd"))
+
+import numpy as np  # noqa: E402
+
+import mpm_abi as A  # noqa: E402
+import scenes as S  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published peak
+METRIC = "particle-updates/s 3D DP-APIC FP32 vs FP64 @stated err; served p50/p99 step ms"
+
+
+def algorithmic_bytes(kernel, cfg, n_particles, active_nodes, grid_nodes):
+    """Compulsory HBM bytes of one launch: particle state the kernel must read and write once, plus node
+    data it must read/write once. FP64 (8 B) throughout in the plain form."""
+    D = cfg.dim
+    f8 = 8
+    apic = cfg.flip_ratio < 0
+    flip = cfg.flip_ratio >= -1
+    tau = 6 if D == 3 else 3
+    mibf = bool(cfg.use_mibf)
+    if kernel == "k_p2g":
+        part = D + D + (D * D if apic else 0) + tau + (1 if mibf else 0)
+        node = 1 + D + D + (1 if mibf else 0)
+        return f8 * (part * n_particles + node * active_nodes)
+    if kernel == "k_g2p_musl":
+        part_r = D + (D if flip else 0)
+        part_w = D + (D * D if apic else 0) + D
+        node = D + (D if flip else 0) + D  # gather v (+flip), scatter mv2
+        return f8 * ((part_r + part_w) * n_particles + node * active_nodes)
+    if kernel in ("k_defgrad", "k_g2p_fused"):
+        part_r = D + D * D + 2 + (D if kernel == "k_g2p_fused" else 0)
+        part_w = D * D + tau + 3 + (D + D + (D * D if apic else 0) if kernel == "k_g2p_fused" else 0)
+        node = D + (D if (kernel == "k_g2p_fused" and flip) else 0)
+        return f8 * ((part_r + part_w) * n_particles + node * active_nodes)
+    if kernel == "k_grid_update":
+        return f8 * ((1 + D + D) + (1 + D + D)) * grid_nodes
+    if kernel == "k_zero_grid":
+        return f8 * (1 + D + D + (D if cfg.use_musl else 0)) * grid_nodes
+    if kernel == "k_musl_norm":
+        return f8 * (1 + D + D) * grid_nodes
+    return 0
+
+
+def parity_check(steps=10, n_axis=20):
+    """Same configuration at n_axis^3 particles: engine vs single-threaded oracle after `steps` steps."""
+    cfg, plates, objs, x = S.dp_collapse(n_per_axis=n_axis, musl=True)
+    out = {}
+    ge = A.Engine(A.product_library(), cfg, plates, objs)
+    oe = A.Engine(A.oracle_library(), cfg, plates, objs)
+    ge.set_particles(x)
+    oe.set_particles(x)
+    ge.step(steps)
+    oe.step(steps)
+    gp, op = ge.get_particles(), oe.get_particles()
+    for k in ("x", "v", "F"):
+        out[k] = float(np.max(np.abs(gp[k] - op[k])) / max(np.max(np.abs(op[k])), 1e-300))
+    gg, og = ge.get_grid(), oe.get_grid()
+    act = og["mass"] > 0
+    out["grid_mass"] = float(np.max(np.abs(gg["mass"][act] - og["mass"][act])) / np.max(og["mass"][act]))
+    out["grid_v"] = float(np.max(np.abs(gg["v"][act] - og["v"][act])) / max(np.max(np.abs(og["v"][act])), 1e-300))
+    out.update(particles=int(x.shape[0]), steps=steps, reference="CPU oracle, 1 thread")
+    return out
+
+
+def cpu_baseline(cfg, plates, objs, x, steps=3):
+    """The oracle (FP64 OpenMP restatement of the reference loops) on all host cores (at most 16), one
+    warm-up step then `steps` timed steps of the same 4,096,000-particle workload."""
+    lib = A.oracle_library()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ccfg = type(cfg).from_buffer_copy(cfg)
+    ccfg.n_threads = threads
+    e = A.Engine(lib, ccfg, plates, objs)
+    e.set_particles(x)
+    info = A.StepInfo()
+    lib.lib.oracle_time_steps(e.h, 1, info)
+    secs = lib.lib.oracle_time_steps(e.h, steps, info)
+    n = x.shape[0]
+    return {"value": n * steps / secs, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps (after 1 warm-up step) of the same {n}-particle scene, {secs:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n-axis", type=int, default=None, help="lattice points per axis (default 160: 4,096,000)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding "
+              "is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    cfg, plates, objs, x = S.dp_collapse(n_per_axis=args.n_axis, musl=True)
+    n = x.shape[0]
+    eng = A.Engine(A.product_library(), cfg, plates, objs)
+    eng.set_particles(x)
+    if args.warmup:
+        rc, _ = eng.step(args.warmup)
+        if rc:
+            raise A.MPMError(rc, "warm-up steps")
+    eng.set_profiling(True)
+    t0 = time.perf_counter()
+    rc, info = eng.step(args.steps)  # returns after the device finished (stream synchronised)
+    t1 = time.perf_counter()
+    if rc or info.steps_done != args.steps:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], f"timed steps ({info.steps_done}/{args.steps} done)")
+    kt = eng.kernel_times()
+    eng.set_profiling(False)
+    wall = t1 - t0
+    value = n * args.steps / wall
+    g = eng.get_grid()
+    active = int(np.count_nonzero(g["mass"] > 0))
+    grid_nodes = int(g["info"].grid_nodes)
+    kernels = {}
+    for name, (ms, launches) in kt.items():
+        avg_ms = ms / max(launches, 1)
+        b = algorithmic_bytes(name, cfg, n, active, grid_nodes)
+        kernels[name] = {"avg_ms": avg_ms, "launches": launches, "total_ms": ms,
+                         "algorithmic_bytes": b, "GBps": (b / (avg_ms * 1e-3) / 1e9) if avg_ms > 0 and b else None}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    d = kernels[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (d["GBps"] / HBM_PEAK_GBS) if d["GBps"] else None, "traffic": None,
+                "avg_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
+    result = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded jittered lattice, SplitMix64 seed 42)",
+        "config": {"workload": "3D granular column collapse, Drucker-Prager 30 deg, APIC, MUSL, plain FP64",
+                   "particles": n, "dx": cfg.dx, "ppc": 8, "spline": "quadratic", "boundary": "SlipFree floor mu=0.4",
+                   "grid_nodes_last_step": grid_nodes, "active_nodes_last_step": active, "precision": "fp64"},
+        "roofline": roofline,
+        "kernels": kernels,
+    }
+    if not args.no_parity:
+        result["error_vs_fp64_oracle"] = parity_check()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(cfg, plates, objs, x)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,255 @@
+/*
+ * mpm.h — C ABI of the MI355X MPM timestep engine (libmpm_hip.so).
+ *
+ * The reference (Matter, twmeehan/matter-tests) has no FFI: its seam is the C++ class
+ * `Simulation` (src/simulation/simulation.hpp:23-225), whose `simulate()` frame loop
+ * (src/simulation/simulation.cpp:168-201) calls `advanceStep()` (src/simulation/simulation.cpp:219-273).
+ * Each entry point below states the reference interface it replaces.
+ *
+ * Conventions
+ *   - Plain C types only. Host buffers are owned by the caller; device state is owned by the context.
+ *   - Every call returns 0 on success or a bitwise OR of MPM_ERR_* codes.
+ *     A nonzero return from mpm_step is the reference's `exit = 1` (src/simulation/simulation.hpp:28).
+ *   - One host thread per context. Work is issued on one HIP stream per context.
+ *   - Vectors are `dim` doubles per particle, interleaved (the memory image of std::vector<TV>);
+ *     matrices are dim*dim doubles per particle in column-major order (the memory image of
+ *     std::vector<TM> with Eigen's default storage), so a caller can pass `&particles.F[0](0,0)`.
+ */
+#ifndef MATTER_MPM_H
+#define MATTER_MPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+
+/* Enumerations. Integer values follow the declaration order of the reference enums
+ * (src/tools.hpp:53-57), so `static_cast<int32_t>(PlasticModel::DP)` is a valid value. */
+enum mpm_plate_type { MPM_PLATE_TOP = 0, MPM_PLATE_BOTTOM, MPM_PLATE_LEFT, MPM_PLATE_RIGHT,
+                      MPM_PLATE_FRONT, MPM_PLATE_BACK };
+enum mpm_elastic_model { MPM_ELASTIC_HENCKY = 0, MPM_ELASTIC_NEOHOOKEAN };
+enum mpm_plastic_model { MPM_PLASTIC_NONE = 0, MPM_PLASTIC_VM, MPM_PLASTIC_DP, MPM_PLASTIC_DPSOFT,
+                         MPM_PLASTIC_MCC, MPM_PLASTIC_VMVISC, MPM_PLASTIC_DPVISC, MPM_PLASTIC_MCCVISC,
+                         MPM_PLASTIC_DPMUI, MPM_PLASTIC_MCCMUI, MPM_PLASTIC_SNOW };
+enum mpm_hardening_law { MPM_HARD_NONE = 0, MPM_HARD_EXPO_EXPL, MPM_HARD_EXPO_IMPL, MPM_HARD_SINH_EXPL,
+                         MPM_HARD_SINH_IMPL };
+enum mpm_bc { MPM_BC_NOSLIP = 0, MPM_BC_SLIPSTICK, MPM_BC_SLIPFREE };
+
+/* Analytic boundary objects: the closed set of ObjectGeneral subclasses (src/objects/object_*.hpp). */
+enum mpm_object_kind {
+    MPM_OBJECT_BOX_ROTATED = 1, /* ObjectBoxRotated  src/objects/object_box.hpp:6-44    */
+    MPM_OBJECT_BUMP        = 2, /* ObjectBump        src/objects/object_bump.hpp:8-42   */
+    MPM_OBJECT_GROUND      = 3, /* ObjectGround      src/objects/object_ground.hpp:8-38 */
+    MPM_OBJECT_RAMP        = 4, /* ObjectRamp        src/objects/object_ramp.hpp:8-41   */
+    MPM_OBJECT_GATE        = 5, /* ObjectGate        src/objects/object_gate.hpp:8-43   */
+    MPM_OBJECT_SILO        = 6, /* ObjectSilo (3D)   src/objects/object_silo.hpp:8-56   */
+    MPM_OBJECT_CURVE       = 7  /* ObjectCurve       src/objects/object_curve.hpp:8-39  */
+};
+
+/* Arithmetic form of the step. */
+enum mpm_precision {
+    MPM_PRECISION_FP64  = 0, /* plain form: every quantity in binary64, the drop-in */
+    MPM_PRECISION_SPLIT = 1  /* transfers and grid in binary32, return mapping and reductions in binary64 */
+};
+
+/* Status / error bits. */
+#define MPM_OK                    0u
+#define MPM_ERR_INVALID_ARGUMENT  (1u << 0)  /* bad pointer, size or enum value                     */
+#define MPM_ERR_UNSUPPORTED       (1u << 1)  /* option outside this engine (pbc, Snow, linear spline) */
+#define MPM_ERR_HIP               (1u << 2)  /* a HIP runtime call failed                             */
+#define MPM_ERR_OUT_OF_MEMORY     (1u << 3)
+#define MPM_ERR_PERZYNA_NEGATIVE  (1u << 4)  /* plasticity.cpp:190-193,286-289,546-553 */
+#define MPM_ERR_NEWTON_MAXITER    (1u << 5)  /* plasticity.cpp:200-203,297-300,562-565 */
+#define MPM_ERR_NEWTON_SMALL_DERIV (1u << 6) /* plasticity.cpp:219-222,313-316,578-581 */
+#define MPM_ERR_MCC_NO_STEP       (1u << 7)  /* mcc_rma_implicit_*_onevar.hpp (limited search failed, sinh max iter) */
+#define MPM_ERR_GRID_BOUNDS       (1u << 8)  /* a stencil left the grid (undefined behaviour in the reference) */
+#define MPM_ERR_NONFINITE         (1u << 9)  /* NaN/Inf reached the particle state */
+#define MPM_ERR_CONSERVATION      (1u << 10) /* simulation.cpp:312-316,328-332 */
+#define MPM_ERR_NOT_READY         (1u << 11) /* step before particles were set */
+#define MPM_WARN_SUPERSONIC       (1u << 16) /* update_dt.cpp:14-17 (warning only; not an exit) */
+
+#define MPM_ERR_EXIT_MASK 0x0000FFFFu /* bits that map to the reference's `exit = 1` */
+
+/* Flags for mpm_step. */
+#define MPM_STEP_STOP_AT_FRAME_END  1u /* return after the step that completes a frame */
+#define MPM_STEP_CHECK_CONSERVATION 2u /* run checkMassConservation/checkMomentumConservation after P2G */
+
+/*
+ * Simulation parameters. One field per public member of the reference's Simulation
+ * (src/simulation/simulation.hpp:28-106), with the same names and meanings, plus the
+ * compile-time switches of src/tools.hpp:19-21 (dim, spline_degree) and the arithmetic form.
+ * Call mpm_config_defaults() first: it fills the reference's default member values.
+ */
+typedef struct mpm_config {
+    int32_t dim;            /* 2 or 3              (tools.hpp:20 THREEDIM)  */
+    int32_t spline_degree;  /* 2 (quadratic) or 3 (cubic)  (tools.hpp:21)   */
+    int32_t precision;      /* enum mpm_precision                           */
+    int32_t n_threads;      /* OpenMP threads of the reference; ignored by the GPU engine */
+
+    int32_t elastic_model;  /* enum mpm_elastic_model */
+    int32_t plastic_model;  /* enum mpm_plastic_model */
+    int32_t hardening_law;  /* enum mpm_hardening_law */
+    int32_t use_pradhana;
+    int32_t use_mises_q;
+    int32_t use_mibf;
+    int32_t use_musl;
+    int32_t gravity_special;
+    int32_t pbc;                       /* unsupported: must be 0 */
+    int32_t change_particle_positions; /* position_update.cpp:38-43 */
+    uint32_t end_frame;
+    uint32_t reserved0;
+
+    double grid_reference_point[3];
+    double gravity[3];
+    double fps, cfl, cfl_elastic, flip_ratio, rho, gravity_time;
+    double Lx, Ly, Lz;
+    double particle_mass, particle_volume, dx;
+    double E, nu, stress_tolerance, xi;
+    double q_max, q_min, p_min;           /* von Mises        */
+    double M, q_cohesion;                 /* Drucker-Prager   */
+    double perzyna_exp, perzyna_visc;     /* Perzyna          */
+    double beta, p0;                      /* Modified Cam-Clay */
+    double rho_s, grain_diameter, I_ref, mu_1, mu_2; /* mu(I) */
+} mpm_config;
+
+/* Axis-aligned plate: ObjectPlate (src/objects/object_plate.hpp:13-28 3D, :83-96 2D). */
+typedef struct mpm_plate {
+    double pos_object;
+    int32_t plate_type;   /* enum mpm_plate_type */
+    int32_t bc;           /* enum mpm_bc         */
+    double friction;
+    double pos_lower, pos_upper;
+    double v_object[3];   /* vx_object, vy_object, vz_object */
+    double vmin_factor, load_factor;
+} mpm_plate;
+
+/* Analytic object (tagged union over the ObjectGeneral subclasses). */
+typedef struct mpm_object {
+    int32_t kind;          /* enum mpm_object_kind */
+    int32_t bc;            /* enum mpm_bc          */
+    double friction;
+    double param;          /* GROUND: y_ground; GATE: height; SILO: cut; otherwise unused */
+    double center[3];      /* BOX_ROTATED */
+    double half_extents[3];/* BOX_ROTATED */
+    double rotation[9];    /* BOX_ROTATED, dim x dim column-major (first dim*dim entries used) */
+} mpm_object;
+
+/* Host views of particle state (Particles, src/data_structures.hpp:8-47). Any pointer may be NULL
+ * to skip that field. Sizes: x, v: n*dim; F, Bmat: n*dim*dim; scalars: n. */
+typedef struct mpm_particle_arrays {
+    double* x;
+    double* v;
+    double* F;
+    double* Bmat;
+    double* eps_pl_dev;
+    double* eps_pl_vol;
+    double* eps_pl_vol_pradhana;
+    double* delta_gamma;
+    double* viscosity;
+    double* muI;
+} mpm_particle_arrays;
+
+/* Result of mpm_step: the state simulate() keeps between steps (simulation.cpp:175-200). */
+typedef struct mpm_step_info {
+    double dt;              /* dt of the last step taken                          */
+    double time;            /* simulation time after the last step                */
+    uint64_t step;          /* current_time_step after the last step              */
+    uint32_t frame;         /* frame counter after the last step                  */
+    uint32_t plastic_count; /* "Proj: n / Np" of the last step (deformation_update.cpp:50-51) */
+    uint32_t steps_done;    /* steps executed by this call                        */
+    uint32_t status;        /* MPM_ERR_* / MPM_WARN_* bits raised by this call    */
+    int32_t frame_ended;    /* the last step completed a frame                    */
+    int32_t finished;       /* time reached end_frame/fps (simulation.cpp:191)    */
+} mpm_step_info;
+
+/* Geometry of the dense node grid of the last step (remesh.cpp:282-301). */
+typedef struct mpm_grid_info {
+    uint32_t Nx, Ny, Nz;    /* Nz = 1 in 2D */
+    uint32_t grid_nodes;
+    double xc, yc, zc;      /* origin = first node coordinate per axis */
+} mpm_grid_info;
+
+typedef struct mpm_context mpm_context;
+
+/* Fills the reference's default member values (simulation.hpp:28-106). Replaces the
+ * default member initialisers of class Simulation. */
+void mpm_config_defaults(mpm_config* cfg);
+
+/* Creates a context on the current HIP device and derives the constants that simulate()
+ * computes before its loop (simulation.cpp:94-145: apicDinverse, lambda, mu, K, wave speed,
+ * dt_max, q/d/e/f/rma prefactors, fac_Q). Replaces `Simulation sim; sim.<fields> = ...;`
+ * plus the set-up part of Simulation::simulate(). */
+int mpm_create(const mpm_config* cfg, mpm_context** out);
+
+/* Releases all device memory of the context. Replaces Simulation::~Simulation (simulation.hpp:26). */
+void mpm_destroy(mpm_context* ctx);
+
+/* Appends a plate. Replaces `plates.push_back(std::make_unique<ObjectPlate>(...))`
+ * (simulation.hpp:109, object_plate.hpp:13-28). Plates are applied in insertion order. */
+int mpm_add_plate(mpm_context* ctx, const mpm_plate* plate);
+
+/* Appends an analytic object. Replaces `objects.push_back(std::make_unique<Object...>(...))`
+ * (simulation.hpp:110, object_general.hpp:8-23). Objects are applied in insertion order, before plates. */
+int mpm_add_object(mpm_context* ctx, const mpm_object* object);
+
+/* Uploads particle state (Np = n). Fields passed as NULL take the values of the reference's
+ * Particles constructor (data_structures.hpp:10-27: zeros, F = I). Replaces writing
+ * `sim.Np`, `sim.particles` before simulate(). If use_mibf, muI is then filled as simulate()
+ * does (simulation.cpp:138-145). Resets time, frame and step to zero. */
+int mpm_set_particles(mpm_context* ctx, uint32_t n, const mpm_particle_arrays* in);
+
+/* Advances up to nsteps steps. Each step is one iteration of simulate()'s loop
+ * (simulation.cpp:175-200): advanceStep() (simulation.cpp:219-273), then time += dt,
+ * current_time_step++, and the frame bookkeeping. Stops early at the end of the simulation
+ * (final_time), after an exit-class error (the failing step completes, time does not
+ * advance, as simulation.cpp:175-177), or, with MPM_STEP_STOP_AT_FRAME_END, after the step
+ * that completes a frame. Returns the OR of MPM_ERR_* bits raised (warnings excluded). */
+int mpm_step(mpm_context* ctx, uint32_t nsteps, uint32_t flags, mpm_step_info* info);
+
+/* Downloads particle state in the order of mpm_set_particles. Replaces reading
+ * `sim.particles` (tests/tests.cpp:49,141-143,359). */
+int mpm_get_particles(mpm_context* ctx, mpm_particle_arrays* out);
+
+/* Number of particles of the context. Replaces reading `sim.Np`. */
+uint32_t mpm_num_particles(const mpm_context* ctx);
+
+/* Downloads the dense grid of the last step in the reference's row-major node order
+ * ind(i,j,k) = (i*Ny + j)*Nz + k (simulation.hpp:197-203): node coordinates
+ * (x[Nx], y[Ny], z[Nz]), mass[grid_nodes], v and flip [grid_nodes*dim], friction[grid_nodes]
+ * (use_mibf only). Any output pointer may be NULL. Call first with all NULL to get the sizes. */
+int mpm_get_grid(mpm_context* ctx, mpm_grid_info* info, double* x, double* y, double* z,
+                 double* mass, double* v, double* flip, double* friction);
+
+/* Downloads plate state after moveObjects (simulation.cpp:277-281): pos_object, pos_lower,
+ * pos_upper and v_object per plate, in insertion order. */
+int mpm_get_plates(mpm_context* ctx, mpm_plate* plates, uint32_t capacity, uint32_t* count);
+
+/* Accumulated device time of one kernel of the step pipeline. */
+typedef struct mpm_kernel_timing {
+    char name[32];       /* kernel name, e.g. "k_p2g" */
+    double total_ms;     /* sum of HIP-event durations of its launches */
+    uint64_t launches;   /* launches that were timed */
+} mpm_kernel_timing;
+
+/* Enables (1) or disables (0) per-kernel HIP-event timing of mpm_step and resets the accumulators.
+ * Replaces the phase timers of advanceStep (`timer t_p2g ...`, simulation.cpp:240-267, accumulated in
+ * runtime_p2g/euler/g2p/defgrad, simulation.hpp:154-158) with one timer per kernel on the context's stream. */
+int mpm_set_profiling(mpm_context* ctx, int enable);
+
+/* Copies the accumulated timings (one entry per kernel that ran) into out[0..capacity). */
+int mpm_get_kernel_times(mpm_context* ctx, mpm_kernel_timing* out, uint32_t capacity, uint32_t* count);
+
+/* Human-readable name of the lowest set status bit. */
+const char* mpm_status_string(uint32_t status);
+
+/* MPM_ABI_VERSION of the library. */
+int mpm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MATTER_MPM_H */

@@ -1,0 +1,531 @@
+// mpm_engine.hip — context management and the C ABI of include/mpm.h for the gfx950 MPM engine.
+//
+// A context owns one HIP stream, the particle state (structure of arrays in HBM), a dense node grid sized
+// with headroom for a batch of steps, and a device-resident StepState. mpm_step enqueues batches of
+// steps without host synchronisation; the device stops itself (StepState::stop) at errors, frame ends,
+// the final time or when the next grid would not fit, and the host re-allocates and resumes.
+#include "mpm_kernels.hpp"
+#include <vector>
+#include <cstring>
+#include <cmath>
+#include <algorithm>
+#include <new>
+
+using namespace mpm;
+
+struct mpm_context {
+    mpm_config cfg;
+    Params P;
+    std::vector<DevPlate> plates;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0;
+    bool ready = false;
+    // particles
+    double* pbuf = nullptr;
+    size_t pbuf_n = 0;
+    PartPtrs pp{};
+    double* xbuf[2][3] = {{nullptr}};
+    // grid
+    double* gbuf = nullptr;
+    double* cbuf = nullptr;
+    uint64_t cap_nodes = 0;
+    uint32_t cap_axis = 0;
+    GridPtrs gp{};
+    // step state
+    StepState* S = nullptr;   // device
+    StepState* Sh = nullptr;  // pinned host mirror
+    uint32_t est_n[3] = {1, 1, 1};
+    // per-kernel HIP-event timing (mpm_set_profiling)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, size_t>> ev_pending;  // (kernel id, index of start event)
+    double kt_ms[16] = {0};
+    uint64_t kt_n[16] = {0};
+};
+
+namespace {
+const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p_fused", "k_g2p_musl",
+                                    "k_musl_norm", "k_defgrad", "k_finish"};
+enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P_FUSED, KS_G2P_MUSL, KS_MUSL_NORM, KS_DEFGRAD, KS_FINISH, KS_COUNT };
+
+// Records an event pair around one launch when profiling; returns the launch's slot.
+struct LaunchTimer {
+    mpm_context* c; int id; size_t slot = 0; bool on;
+    LaunchTimer(mpm_context* c_, int id_) : c(c_), id(id_), on(c_->profiling) {
+        if (!on) return;
+        slot = c->ev_pending.size() * 2;
+        while (c->ev_pool.size() < slot + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { on = false; return; } c->ev_pool.push_back(e); }
+        (void)hipEventRecord(c->ev_pool[slot], c->stream);
+    }
+    ~LaunchTimer() {
+        if (!on) return;
+        (void)hipEventRecord(c->ev_pool[slot + 1], c->stream);
+        c->ev_pending.push_back({id, slot});
+    }
+};
+
+void collect_timings(mpm_context* c) {
+    for (auto& pr : c->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[pr.second], c->ev_pool[pr.second + 1]) == hipSuccess) {
+            c->kt_ms[pr.first] += ms; c->kt_n[pr.first] += 1;
+        }
+    }
+    c->ev_pending.clear();
+}
+}  // namespace
+
+namespace {
+
+constexpr uint32_t kBatch = 32;
+
+#define HIP_TRY(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return MPM_ERR_HIP; } while (0)
+
+int valid_enum(int v, int lo, int hi) { return v >= lo && v <= hi; }
+
+void derive_params(const mpm_config& c, Params& P) {
+    std::memset(&P, 0, sizeof(P));
+    P.dim = c.dim; P.spline_degree = c.spline_degree; P.elastic_model = c.elastic_model; P.plastic_model = c.plastic_model;
+    P.hardening_law = c.hardening_law; P.use_pradhana = c.use_pradhana; P.use_mises_q = c.use_mises_q; P.use_mibf = c.use_mibf;
+    P.use_musl = c.use_musl; P.gravity_special = c.gravity_special; P.change_particle_positions = c.change_particle_positions;
+    P.end_frame = c.end_frame;
+    // simulate() set-up, simulation.cpp:94-145
+    P.dx = c.dx;
+    P.apicDinverse = (c.spline_degree == 3 ? 3.0 : 4.0) / (c.dx * c.dx);
+    P.lambda = c.nu * c.E / ((1.0 + c.nu) * (1.0 - 2.0 * c.nu));
+    P.mu = c.E / (2.0 * (1.0 + c.nu));
+    P.K = P.lambda + 2.0 * P.mu / c.dim;  // calculateBulkModulus, simulation.cpp:288-290
+    P.wave_speed = std::sqrt(c.E / c.rho);
+    P.dt_max = c.cfl_elastic * c.dx / P.wave_speed;
+    P.frame_dt = 1.0 / c.fps;
+    P.one_over_dx = 1.0 / c.dx;
+    if (c.use_mises_q) { P.q_prefac = std::sqrt(3.0) / std::sqrt(2.0); P.d_prefac = std::sqrt(2.0) / std::sqrt(3.0); }
+    else { P.q_prefac = 1.0 / std::sqrt(2.0); P.d_prefac = std::sqrt(2.0); }
+    P.e_mu_prefac = 2 * P.q_prefac * P.mu;
+    P.f_mu_prefac = 2 * P.q_prefac / P.d_prefac * P.mu;
+    P.rma_prefac = 2 * P.q_prefac * P.q_prefac;
+    P.fac_Q = c.I_ref / (c.grain_diameter * std::sqrt(c.rho_s));
+    P.final_time = c.end_frame * P.frame_dt;
+    P.particle_mass = c.particle_mass; P.particle_volume = c.particle_volume;
+    P.cfl = c.cfl; P.flip_ratio = c.flip_ratio; P.gravity_time = c.gravity_time;
+    for (int d = 0; d < 3; ++d) { P.gravity_final[d] = d < c.dim ? c.gravity[d] : 0.0; P.grid_reference_point[d] = c.grid_reference_point[d]; }
+    P.E = c.E; P.nu = c.nu; P.stress_tolerance = c.stress_tolerance; P.xi = c.xi;
+    P.q_max = c.q_max; P.q_min = c.q_min; P.p_min = c.p_min; P.M = c.M; P.q_cohesion = c.q_cohesion;
+    P.perzyna_exp = c.perzyna_exp; P.perzyna_visc = c.perzyna_visc; P.beta = c.beta; P.p0 = c.p0;
+    P.rho_s = c.rho_s; P.grain_diameter = c.grain_diameter; P.I_ref = c.I_ref; P.mu_1 = c.mu_1; P.mu_2 = c.mu_2;
+}
+
+uint32_t particle_fields(int D) { return 3 * D + D * D * 2 + (D == 3 ? 6 : 3) + 6; }  // x, xn, v, B, F, tau, scalars
+
+int alloc_particles(mpm_context* c, uint32_t n) {
+    const int D = c->cfg.dim;
+    size_t need = (size_t)particle_fields(D) * n;
+    if (c->pbuf && c->pbuf_n >= need) return 0;
+    if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
+    if (hipMalloc(&c->pbuf, need * sizeof(double)) != hipSuccess) { c->pbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+    c->pbuf_n = need;
+    return 0;
+}
+
+void layout_particles(mpm_context* c, uint32_t n) {
+    const int D = c->cfg.dim;
+    double* q = c->pbuf;
+    auto take = [&]() { double* r = q; q += n; return r; };
+    PartPtrs pp{};
+    for (int d = 0; d < D; ++d) c->xbuf[0][d] = take();
+    for (int d = 0; d < D; ++d) c->xbuf[1][d] = take();
+    for (int d = 0; d < D; ++d) pp.v[d] = take();
+    for (int i = 0; i < D * D; ++i) pp.B[i] = take();
+    for (int i = 0; i < D * D; ++i) pp.F[i] = take();
+    for (int i = 0; i < (D == 3 ? 6 : 3); ++i) pp.tau[i] = take();
+    pp.eps_dev = take(); pp.eps_vol = take(); pp.eps_prad = take(); pp.dgamma = take(); pp.visc = take(); pp.muI = take();
+    pp.n = n;
+    c->pp = pp;
+}
+
+PartPtrs with_x(const mpm_context* c, uint32_t xsel) {
+    PartPtrs pp = c->pp;
+    for (int d = 0; d < 3; ++d) { pp.x[d] = c->xbuf[xsel & 1][d]; pp.xn[d] = c->xbuf[(xsel + 1) & 1][d]; }
+    return pp;
+}
+
+int ensure_grid(mpm_context* c, uint32_t batch) {
+    const int D = c->cfg.dim;
+    uint32_t want_axis = 0; uint64_t want_nodes = 1;
+    for (int d = 0; d < D; ++d) {
+        uint32_t a = c->est_n[d] + 2 * (batch + 4);
+        want_axis = std::max(want_axis, a);
+        want_nodes *= a;
+    }
+    if (want_axis <= c->cap_axis && want_nodes <= c->cap_nodes) return 0;
+    if (want_nodes > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
+    want_axis = std::max(want_axis, c->cap_axis);
+    want_nodes = std::max(want_nodes, c->cap_nodes);
+    if (c->gbuf) { (void)hipFree(c->gbuf); c->gbuf = nullptr; }
+    if (c->cbuf) { (void)hipFree(c->cbuf); c->cbuf = nullptr; }
+    const int nf = 1 + 3 * D + 1 + 3 * D;  // mass, mv, f, fric, v, flip, mv2
+    if (hipMalloc(&c->gbuf, (size_t)nf * want_nodes * sizeof(double)) != hipSuccess) { c->gbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
+    if (hipMalloc(&c->cbuf, (size_t)3 * want_axis * sizeof(double)) != hipSuccess) { c->cbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
+    double* q = c->gbuf;
+    auto take = [&]() { double* r = q; q += want_nodes; return r; };
+    GridPtrs g{};
+    g.mass = take();
+    for (int d = 0; d < D; ++d) g.mv[d] = take();
+    for (int d = 0; d < D; ++d) g.f[d] = take();
+    g.fric = take();
+    for (int d = 0; d < D; ++d) g.v[d] = take();
+    for (int d = 0; d < D; ++d) g.flip[d] = take();
+    for (int d = 0; d < D; ++d) g.mv2[d] = take();
+    for (int d = 0; d < 3; ++d) g.coord[d] = c->cbuf + (size_t)d * want_axis;
+    c->gp = g;
+    c->cap_axis = want_axis; c->cap_nodes = want_nodes;
+    return 0;
+}
+
+__global__ void k_control(StepState* S, uint32_t stop_at_frame_end, uint32_t check_cons, uint32_t cap_axis, uint32_t cap_nodes) {
+    if (threadIdx.x != 0) return;
+    S->stop = 0; S->need_realloc = 0; S->steps_done = 0; S->err = 0; S->warn = 0; S->frame_ended = 0;
+    S->stop_at_frame_end = stop_at_frame_end; S->check_conservation = check_cons;
+    S->cap_axis = cap_axis; S->cap_nodes = cap_nodes;
+}
+
+template <int D, int SPL>
+void enqueue_step(mpm_context* c, uint32_t xsel) {
+    hipStream_t s = c->stream;
+    const Params& P = c->P;
+    PartPtrs pp = with_x(c, xsel);
+    const uint32_t pblocks = (c->n + kBlock - 1) / kBlock;
+    const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 4096);
+    { LaunchTimer t(c, KS_SETUP); k_setup<D><<<1, 64, 0, s>>>(P, c->S, c->gp); }
+    { LaunchTimer t(c, KS_ZERO); k_zero_grid<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp); }
+    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, pp, c->gp); }
+    { LaunchTimer t(c, KS_GRID); k_grid_update<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp); }
+    if (P.use_musl) {
+        { LaunchTimer t(c, KS_G2P_MUSL); k_g2p_musl<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, pp, c->gp); }
+        { LaunchTimer t(c, KS_MUSL_NORM); k_musl_norm<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp); }
+        { LaunchTimer t(c, KS_DEFGRAD); k_defgrad<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, pp, c->gp); }
+    } else {
+        LaunchTimer t(c, KS_G2P_FUSED); k_g2p_fused<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, pp, c->gp);
+    }
+    { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, c->n); }
+}
+
+void enqueue(mpm_context* c, uint32_t xsel) {
+    const int D = c->cfg.dim, SPL = c->cfg.spline_degree;
+    if (D == 3 && SPL == 2) enqueue_step<3, 2>(c, xsel);
+    else if (D == 3 && SPL == 3) enqueue_step<3, 3>(c, xsel);
+    else if (D == 2 && SPL == 2) enqueue_step<2, 2>(c, xsel);
+    else enqueue_step<2, 3>(c, xsel);
+}
+
+int sync_state(mpm_context* c) {
+    HIP_TRY(hipMemcpyAsync(c->Sh, c->S, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mpm_config_defaults(mpm_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->dim = 3; c->spline_degree = 2; c->precision = MPM_PRECISION_FP64; c->n_threads = 1;
+    c->elastic_model = MPM_ELASTIC_HENCKY; c->plastic_model = MPM_PLASTIC_NONE; c->hardening_law = MPM_HARD_EXPO_IMPL;
+    c->use_pradhana = 1; c->end_frame = 1;
+    for (int i = 0; i < 3; ++i) c->grid_reference_point[i] = 2e10;
+    c->fps = 1; c->cfl = 0.5; c->cfl_elastic = 0.5; c->flip_ratio = -0.95; c->rho = 1000;
+    c->Lx = c->Ly = c->Lz = 1;
+    c->E = 1e6; c->nu = 0.3; c->stress_tolerance = 1e-5;
+    c->q_max = 100; c->q_min = 100; c->p_min = -1e20; c->M = 1;
+    c->perzyna_exp = 1; c->p0 = 1000;
+    c->rho_s = 2500; c->grain_diameter = 1e-3; c->I_ref = 0.279;
+    c->mu_1 = std::tan(20.9 * M_PI / 180.0); c->mu_2 = std::tan(32.8 * M_PI / 180.0);
+}
+
+int mpm_create(const mpm_config* cfg, mpm_context** out) {
+    if (!cfg || !out) return MPM_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    const mpm_config& c = *cfg;
+    if (c.dim != 2 && c.dim != 3) return MPM_ERR_INVALID_ARGUMENT;
+    if (c.spline_degree != 2 && c.spline_degree != 3) return MPM_ERR_UNSUPPORTED;
+    if (c.precision != MPM_PRECISION_FP64) return MPM_ERR_UNSUPPORTED;
+    if (c.pbc) return MPM_ERR_UNSUPPORTED;
+    if (!valid_enum(c.elastic_model, 0, 1) || !valid_enum(c.plastic_model, 0, 10) || !valid_enum(c.hardening_law, 0, 4))
+        return MPM_ERR_INVALID_ARGUMENT;
+    if (c.plastic_model == MPM_PLASTIC_SNOW) return MPM_ERR_UNSUPPORTED;
+    if (c.elastic_model != MPM_ELASTIC_HENCKY && c.plastic_model != MPM_PLASTIC_NONE) return MPM_ERR_UNSUPPORTED;  // simulation.cpp:80-84
+    if (!(c.dx > 0) || !(c.fps > 0) || !(c.rho > 0)) return MPM_ERR_INVALID_ARGUMENT;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPM_ERR_HIP;
+    mpm_context* ctx = new (std::nothrow) mpm_context();
+    if (!ctx) return MPM_ERR_OUT_OF_MEMORY;
+    ctx->cfg = c;
+    derive_params(c, ctx->P);
+    (void)hipGetDevice(&ctx->device);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MPM_ERR_HIP; }
+    if (hipMalloc(&ctx->S, sizeof(StepState)) != hipSuccess) { (void)hipStreamDestroy(ctx->stream); delete ctx; return MPM_ERR_OUT_OF_MEMORY; }
+    if (hipHostMalloc(&ctx->Sh, sizeof(StepState), hipHostMallocDefault) != hipSuccess) { (void)hipFree(ctx->S); (void)hipStreamDestroy(ctx->stream); delete ctx; return MPM_ERR_OUT_OF_MEMORY; }
+    std::memset(ctx->Sh, 0, sizeof(StepState));
+    *out = ctx;
+    return 0;
+}
+
+void mpm_destroy(mpm_context* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->pbuf) (void)hipFree(c->pbuf);
+    if (c->gbuf) (void)hipFree(c->gbuf);
+    if (c->cbuf) (void)hipFree(c->cbuf);
+    if (c->S) (void)hipFree(c->S);
+    if (c->Sh) (void)hipHostFree(c->Sh);
+    for (hipEvent_t ev : c->ev_pool) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int mpm_add_plate(mpm_context* c, const mpm_plate* p) {
+    if (!c || !p) return MPM_ERR_INVALID_ARGUMENT;
+    if ((int)c->plates.size() >= kMaxPlates) return MPM_ERR_UNSUPPORTED;
+    if (!valid_enum(p->plate_type, 0, 5) || !valid_enum(p->bc, 0, 2)) return MPM_ERR_INVALID_ARGUMENT;
+    DevPlate d{};
+    d.pos_object = p->pos_object; d.pos_lower = p->pos_lower; d.pos_upper = p->pos_upper;
+    for (int i = 0; i < 3; ++i) { d.v_object[i] = (i < c->cfg.dim) ? p->v_object[i] : 0.0; d.v_object_original[i] = d.v_object[i]; }
+    d.friction = p->friction; d.vmin_factor = p->vmin_factor; d.load_factor = p->load_factor;
+    d.plate_type = p->plate_type; d.bc = p->bc;
+    c->plates.push_back(d);
+    c->P.n_plates = (int)c->plates.size();
+    if (c->ready) {  // particles already resident: append the plate to the device state too
+        const size_t k = c->plates.size() - 1;
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->Sh->plates[k] = d;
+        HIP_TRY(hipMemcpy(&c->S->plates[k], &d, sizeof(DevPlate), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int mpm_add_object(mpm_context* c, const mpm_object* o) {
+    if (!c || !o) return MPM_ERR_INVALID_ARGUMENT;
+    if (c->P.n_objects >= kMaxObjects) return MPM_ERR_UNSUPPORTED;
+    if (!valid_enum(o->kind, MPM_OBJECT_BOX_ROTATED, MPM_OBJECT_CURVE) || !valid_enum(o->bc, 0, 2)) return MPM_ERR_INVALID_ARGUMENT;
+    if (o->kind == MPM_OBJECT_SILO && c->cfg.dim != 3) return MPM_ERR_UNSUPPORTED;
+    DevObject& d = c->P.objects[c->P.n_objects];
+    std::memset(&d, 0, sizeof(d));
+    d.kind = o->kind; d.bc = o->bc; d.friction = o->friction; d.param = o->param;
+    for (int i = 0; i < 3; ++i) { d.center[i] = o->center[i]; d.half_extents[i] = o->half_extents[i]; }
+    const int D = c->cfg.dim;
+    // the rotation arrives as dim x dim column-major
+    for (int i = 0; i < D * D; ++i) d.rot[i] = o->rotation[i];
+    c->P.n_objects++;
+    return 0;
+}
+
+int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in) {
+    if (!c || !in || !in->x || n == 0) return MPM_ERR_INVALID_ARGUMENT;
+    const int D = c->cfg.dim;
+    if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int rc = alloc_particles(c, n);
+    if (rc) return rc;
+    c->n = n;
+    layout_particles(c, n);
+    // structure-of-arrays staging: one host vector per device column
+    std::vector<double> col(n);
+    auto up = [&](double* dst) { return hipMemcpy(dst, col.data(), sizeof(double) * n, hipMemcpyHostToDevice); };
+    for (int d = 0; d < D; ++d) {
+        for (uint32_t p = 0; p < n; ++p) col[p] = in->x[(size_t)p * D + d];
+        HIP_TRY(up(c->xbuf[0][d]));
+        for (uint32_t p = 0; p < n; ++p) col[p] = in->v ? in->v[(size_t)p * D + d] : 0.0;
+        HIP_TRY(up(c->pp.v[d]));
+    }
+    for (int i = 0; i < D * D; ++i) {
+        const bool diag = (i % (D + 1)) == 0;
+        for (uint32_t p = 0; p < n; ++p) col[p] = in->F ? in->F[(size_t)p * D * D + i] : (diag ? 1.0 : 0.0);
+        HIP_TRY(up(c->pp.F[i]));
+        for (uint32_t p = 0; p < n; ++p) col[p] = in->Bmat ? in->Bmat[(size_t)p * D * D + i] : 0.0;
+        HIP_TRY(up(c->pp.B[i]));
+    }
+    const double* sc[6] = {in->eps_pl_dev, in->eps_pl_vol, in->eps_pl_vol_pradhana, in->delta_gamma, in->viscosity, in->muI};
+    double* dst[6] = {c->pp.eps_dev, c->pp.eps_vol, c->pp.eps_prad, c->pp.dgamma, c->pp.visc, c->pp.muI};
+    for (int k = 0; k < 6; ++k) {
+        for (uint32_t p = 0; p < n; ++p) col[p] = sc[k] ? sc[k][p] : 0.0;
+        if (k == 5 && c->cfg.use_mibf) {  // simulate() fills muI when use_mibf (simulation.cpp:138-145)
+            const double fill = (c->cfg.plastic_model == MPM_PLASTIC_DPMUI || c->cfg.plastic_model == MPM_PLASTIC_MCCMUI) ? c->cfg.mu_1 : c->cfg.M;
+            std::fill(col.begin(), col.end(), fill);
+        }
+        HIP_TRY(up(dst[k]));
+    }
+    // initial grid estimate from the particle bounds
+    for (int d = 0; d < D; ++d) {
+        double mn = in->x[d], mx = in->x[d];
+        for (uint32_t p = 1; p < n; ++p) { double v = in->x[(size_t)p * D + d]; mn = std::min(mn, v); mx = std::max(mx, v); }
+        if (c->cfg.grid_reference_point[0] < 1e10) { mn = std::min(mn, c->cfg.grid_reference_point[d]); mx = std::max(mx, c->cfg.grid_reference_point[d]); }
+        double span = (mx - mn) / c->cfg.dx;
+        if (!(span < 4.0e6)) return MPM_ERR_OUT_OF_MEMORY;  // non-finite or absurd extent
+        c->est_n[d] = (uint32_t)span + 8;
+    }
+    for (int d = D; d < 3; ++d) c->est_n[d] = 1;
+    // step state
+    StepState& S = *c->Sh;
+    std::memset(&S, 0, sizeof(S));
+    for (int d = 0; d < 3; ++d) { S.gravity[d] = c->P.gravity_final[d]; S.xmin_key[d] = 0x7FFFFFFFFFFFFFFFLL; S.xmax_key[d] = (-0x7FFFFFFFFFFFFFFFLL - 1); }
+    for (size_t k = 0; k < c->plates.size(); ++k) S.plates[k] = c->plates[k];
+    S.n[0] = S.n[1] = S.n[2] = 1;
+    HIP_TRY(hipMemcpy(c->S, &S, sizeof(StepState), hipMemcpyHostToDevice));
+    const uint32_t pblocks = (n + kBlock - 1) / kBlock;
+    PartPtrs pp = with_x(c, 0);
+    if (D == 3) k_prepare<3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, pp);
+    else k_prepare<2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, pp);
+    HIP_TRY(hipGetLastError());
+    rc = sync_state(c);
+    if (rc) return rc;
+    c->ready = true;
+    return 0;
+}
+
+int mpm_step(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* info) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    if (info) std::memset(info, 0, sizeof(*info));
+    if (!c->ready) return MPM_ERR_NOT_READY;
+    if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
+    uint32_t remaining = nsteps, done = 0, status = 0, reallocs = 0;
+    const int D = c->cfg.dim;
+    while (remaining > 0) {
+        uint32_t batch = std::min(remaining, kBatch);
+        int rc = ensure_grid(c, batch);
+        if (rc) { status |= rc; break; }
+        uint32_t xsel = c->Sh->xsel;
+        k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u,
+                                           (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u, c->cap_axis, (uint32_t)c->cap_nodes);
+        for (uint32_t j = 0; j < batch; ++j) enqueue(c, c->P.use_musl ? (xsel + j) : 0u);
+        if (hipGetLastError() != hipSuccess) { status |= MPM_ERR_HIP; break; }
+        rc = sync_state(c);
+        if (rc) { status |= rc; break; }
+        if (c->profiling) collect_timings(c);
+        const StepState& S = *c->Sh;
+        done += S.steps_done;
+        status |= S.err | S.warn;
+        remaining -= std::min(remaining, S.steps_done);
+        for (int d = 0; d < D; ++d) c->est_n[d] = S.n[d];
+        if (S.need_realloc) {  // next ensure_grid grows the capacity from S.n
+            if (++reallocs > 8) { status |= MPM_ERR_OUT_OF_MEMORY; break; }
+            continue;
+        }
+        reallocs = 0;
+        if (S.stop) break;
+        if (S.steps_done == 0) break;
+    }
+    const StepState& S = *c->Sh;
+    if (info) {
+        info->dt = S.dt; info->time = S.time; info->step = S.step; info->frame = S.frame;
+        info->plastic_count = S.plastic_count; info->steps_done = done; info->status = status;
+        info->frame_ended = (int32_t)S.frame_ended; info->finished = (int32_t)S.finished;
+    }
+    return (int)(status & MPM_ERR_EXIT_MASK);
+}
+
+uint32_t mpm_num_particles(const mpm_context* c) { return c ? c->n : 0; }
+
+int mpm_get_particles(mpm_context* c, mpm_particle_arrays* out) {
+    if (!c || !out) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c->ready) return MPM_ERR_NOT_READY;
+    const int D = c->cfg.dim;
+    const uint32_t n = c->n;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    PartPtrs pp = with_x(c, c->Sh->xsel);
+    std::vector<double> col(n);
+    auto down = [&](const double* src) { return hipMemcpy(col.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost); };
+    for (int d = 0; d < D; ++d) {
+        if (out->x) { HIP_TRY(down(pp.x[d])); for (uint32_t p = 0; p < n; ++p) out->x[(size_t)p * D + d] = col[p]; }
+        if (out->v) { HIP_TRY(down(pp.v[d])); for (uint32_t p = 0; p < n; ++p) out->v[(size_t)p * D + d] = col[p]; }
+    }
+    for (int i = 0; i < D * D; ++i) {
+        if (out->F) { HIP_TRY(down(pp.F[i])); for (uint32_t p = 0; p < n; ++p) out->F[(size_t)p * D * D + i] = col[p]; }
+        if (out->Bmat) { HIP_TRY(down(pp.B[i])); for (uint32_t p = 0; p < n; ++p) out->Bmat[(size_t)p * D * D + i] = col[p]; }
+    }
+    double* dsts[6] = {out->eps_pl_dev, out->eps_pl_vol, out->eps_pl_vol_pradhana, out->delta_gamma, out->viscosity, out->muI};
+    const double* srcs[6] = {pp.eps_dev, pp.eps_vol, pp.eps_prad, pp.dgamma, pp.visc, pp.muI};
+    for (int k = 0; k < 6; ++k) if (dsts[k]) HIP_TRY(hipMemcpy(dsts[k], srcs[k], sizeof(double) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mpm_get_grid(mpm_context* c, mpm_grid_info* info, double* x, double* y, double* z, double* mass, double* v,
+                 double* flip, double* friction) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c->ready) return MPM_ERR_NOT_READY;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const StepState& S = *c->Sh;
+    const int D = c->cfg.dim;
+    const uint32_t nn = S.grid_nodes;
+    if (info) {
+        info->Nx = S.n[0]; info->Ny = S.n[1]; info->Nz = D == 3 ? S.n[2] : 1; info->grid_nodes = nn;
+        info->xc = S.origin[0]; info->yc = S.origin[1]; info->zc = D == 3 ? S.origin[2] : 0.0;
+    }
+    if (nn == 0 || !c->gbuf) return 0;
+    double* axes[3] = {x, y, z};
+    for (int d = 0; d < D; ++d) if (axes[d]) HIP_TRY(hipMemcpy(axes[d], c->gp.coord[d], sizeof(double) * S.n[d], hipMemcpyDeviceToHost));
+    if (mass) HIP_TRY(hipMemcpy(mass, c->gp.mass, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    if (friction && c->cfg.use_mibf) HIP_TRY(hipMemcpy(friction, c->gp.fric, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    std::vector<double> col(nn);
+    for (int d = 0; d < D; ++d) {
+        if (v) { HIP_TRY(hipMemcpy(col.data(), c->gp.v[d], sizeof(double) * nn, hipMemcpyDeviceToHost)); for (uint32_t l = 0; l < nn; ++l) v[(size_t)l * D + d] = col[l]; }
+        if (flip) { HIP_TRY(hipMemcpy(col.data(), c->gp.flip[d], sizeof(double) * nn, hipMemcpyDeviceToHost)); for (uint32_t l = 0; l < nn; ++l) flip[(size_t)l * D + d] = col[l]; }
+    }
+    return 0;
+}
+
+int mpm_get_plates(mpm_context* c, mpm_plate* out, uint32_t cap, uint32_t* count) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    if (count) *count = (uint32_t)c->plates.size();
+    const bool live = c->ready;
+    for (uint32_t i = 0; i < c->plates.size() && i < cap && out; ++i) {
+        const DevPlate& p = live ? c->Sh->plates[i] : c->plates[i];
+        out[i].pos_object = p.pos_object; out[i].plate_type = p.plate_type; out[i].bc = p.bc; out[i].friction = p.friction;
+        out[i].pos_lower = p.pos_lower; out[i].pos_upper = p.pos_upper;
+        for (int d = 0; d < 3; ++d) out[i].v_object[d] = p.v_object[d];
+        out[i].vmin_factor = p.vmin_factor; out[i].load_factor = p.load_factor;
+    }
+    return 0;
+}
+
+int mpm_set_profiling(mpm_context* c, int enable) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->ev_pending.clear();
+    for (int k = 0; k < KS_COUNT; ++k) { c->kt_ms[k] = 0; c->kt_n[k] = 0; }
+    c->profiling = enable != 0;
+    return 0;
+}
+
+int mpm_get_kernel_times(mpm_context* c, mpm_kernel_timing* out, uint32_t cap, uint32_t* count) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    uint32_t n = 0;
+    for (int k = 0; k < KS_COUNT; ++k) {
+        if (!c->kt_n[k]) continue;
+        if (out && n < cap) {
+            std::memset(out[n].name, 0, sizeof(out[n].name));
+            std::strncpy(out[n].name, kKernelNames[k], sizeof(out[n].name) - 1);
+            out[n].total_ms = c->kt_ms[k]; out[n].launches = c->kt_n[k];
+        }
+        ++n;
+    }
+    if (count) *count = n;
+    return 0;
+}
+
+const char* mpm_status_string(uint32_t s) {
+    if (s == 0) return "ok";
+    static const char* names[] = {"invalid argument", "unsupported option", "HIP runtime error", "out of memory",
+                                  "negative Perzyna delta_gamma", "Newton iteration limit", "Newton derivative too small",
+                                  "MCC return map found no admissible step", "stencil outside the grid", "non-finite state",
+                                  "conservation check failed", "particles not set"};
+    for (int b = 0; b < 12; ++b) if (s & (1u << b)) return names[b];
+    if (s & MPM_WARN_SUPERSONIC) return "particle faster than the elastic wave speed";
+    return "unknown status";
+}
+
+int mpm_abi_version(void) { return MPM_ABI_VERSION; }
+
+}  // extern "C"

@@ -1,0 +1,780 @@
+// mpm_math.hpp — per-particle and per-node arithmetic of the MPM step for gfx950 kernels.
+//
+// Everything here is FP64 and branch-for-branch equivalent to the reference's formulas (cited per
+// function); the organisation is the GPU's: the return map works on principal Hencky strains only and
+// reports whether F must be recomposed, errors become bits instead of console messages, and boundary
+// objects are a closed tagged union instead of virtual classes. MPM_HD functions also compile for the
+// host so they can be exercised without a GPU.
+#pragma once
+#include <stdint.h>
+#include <math.h>
+#include "mpm_types.hpp"
+#include "../../include/mpm.h"
+
+#if defined(__HIPCC__)
+#define MPM_HD __host__ __device__ inline
+#else
+#define MPM_HD inline
+#endif
+
+namespace mpm {
+
+// ------------------------------------------------------------------------------------------------
+// B-spline kernels (src/tools.hpp:110-175): N(u) and dN/du for the quadratic and cubic B-spline.
+// ------------------------------------------------------------------------------------------------
+MPM_HD int sgn_d(double v) { return (0.0 < v) - (v < 0.0); }
+
+template <int SPL> struct BSpline;
+template <> struct BSpline<2> {
+    static constexpr double apic_factor = 4.0;  // apicDinverse = 4/dx^2 (simulation.cpp:98)
+    MPM_HD static double N(double u) {
+        double a = fabs(u);
+        if (a < 0.5) return 0.75 - a * a;
+        if (a < 1.5) return 0.5 * (1.5 - a) * (1.5 - a);
+        return 0.0;
+    }
+    MPM_HD static double dN(double u) {
+        double a = fabs(u);
+        if (a < 0.5) return -2 * u;
+        if (a < 1.5) return u - 1.5 * sgn_d(u);
+        return 0.0;
+    }
+};
+template <> struct BSpline<3> {
+    static constexpr double apic_factor = 3.0;  // apicDinverse = 3/dx^2 (simulation.cpp:95)
+    MPM_HD static double N(double u) {
+        double a = fabs(u);
+        if (a < 1.0) return 0.5 * a * a * a - a * a + 0.6666666666666666666666666666666666666666666666666;
+        if (a < 2.0) return 0.1666666666666666666666666666666666666666666 * (2.0 - a) * (2.0 - a) * (2.0 - a);
+        return 0.0;
+    }
+    MPM_HD static double dN(double u) {
+        double a = fabs(u);
+        if (a < 1.0) return u * (1.5 * a - 2.0);
+        if (a < 2.0) return -0.5 * sgn_d(u) * (2.0 - a) * (2.0 - a);
+        return 0.0;
+    }
+};
+
+// ------------------------------------------------------------------------------------------------
+// Small dense matrices, column-major m[r + c*D].
+// ------------------------------------------------------------------------------------------------
+template <int D> struct M { double m[D * D]; MPM_HD double& operator()(int r, int c) { return m[r + c * D]; } MPM_HD double operator()(int r, int c) const { return m[r + c * D]; } };
+template <int D> struct V { double v[D]; MPM_HD double& operator[](int i) { return v[i]; } MPM_HD double operator[](int i) const { return v[i]; } };
+
+template <int D> MPM_HD void set_identity(M<D>& a) { for (int i = 0; i < D * D; ++i) a.m[i] = 0.0; for (int i = 0; i < D; ++i) a(i, i) = 1.0; }
+// (A*B)(r,c) = sum_k A(r,k) B(k,c), summed in k order.
+template <int D> MPM_HD M<D> mul(const M<D>& A, const M<D>& B) {
+    M<D> R;
+    for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { double s = A(r, 0) * B(0, c); for (int k = 1; k < D; ++k) s += A(r, k) * B(k, c); R(r, c) = s; }
+    return R;
+}
+// U * diag(d) * V^T evaluated as (U*diag(d)) * V^T.
+template <int D> MPM_HD M<D> udvt(const M<D>& U, const V<D>& d, const M<D>& W) {
+    M<D> UD; for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) UD(r, c) = U(r, c) * d[c];
+    M<D> R;
+    for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { double s = UD(r, 0) * W(c, 0); for (int k = 1; k < D; ++k) s += UD(r, k) * W(c, k); R(r, c) = s; }
+    return R;
+}
+MPM_HD double det(const M<2>& a) { return a(0, 0) * a(1, 1) - a(1, 0) * a(0, 1); }
+MPM_HD double det(const M<3>& a) {
+    return a(0, 0) * (a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1)) - a(1, 0) * (a(0, 1) * a(2, 2) - a(0, 2) * a(2, 1))
+         + a(2, 0) * (a(0, 1) * a(1, 2) - a(0, 2) * a(1, 1));
+}
+
+// ------------------------------------------------------------------------------------------------
+// Two-sided Jacobi SVD, A = U diag(s) W^T with s >= 0 sorted descending (the convention of the
+// reference's Eigen::JacobiSVD, simulation.hpp:232 and plasticity.cpp:19). Each sweep visits the
+// pairs (1,0),(2,0),(2,1); a pair is first made symmetric by a plane rotation on the left, then
+// diagonalised by a symmetric Jacobi rotation applied on both sides. Iterates until every off-diagonal
+// entry is below 2*eps times the largest diagonal entry (at most 24 sweeps).
+// ------------------------------------------------------------------------------------------------
+struct PlaneRot { double c, s; };
+
+MPM_HD PlaneRot sym_jacobi(double x, double y, double z) {  // annihilates y in [[x y][y z]]
+    PlaneRot r{1.0, 0.0};
+    double ay = fabs(y);
+    if (2.0 * ay < 2.2250738585072014e-308) return r;
+    double tau = (x - z) / (2.0 * ay);
+    double w = sqrt(tau * tau + 1.0);
+    double t = (tau > 0) ? 1.0 / (tau + w) : 1.0 / (tau - w);
+    double n = 1.0 / sqrt(t * t + 1.0);
+    double st = t > 0 ? 1.0 : -1.0;
+    r.s = -st * (y / ay) * fabs(t) * n;
+    r.c = n;
+    return r;
+}
+
+template <int D> MPM_HD void svd(const M<D>& A, M<D>& U, V<D>& s, M<D>& W) {
+    double scale = 0.0;
+    for (int i = 0; i < D * D; ++i) scale = fmax(scale, fabs(A.m[i]));
+    if (scale == 0.0) scale = 1.0;
+    double inv = 1.0 / scale;
+    M<D> X; for (int i = 0; i < D * D; ++i) X.m[i] = A.m[i] * inv;
+    set_identity(U); set_identity(W);
+    double dmax = 0.0; for (int i = 0; i < D; ++i) dmax = fmax(dmax, fabs(X(i, i)));
+    const double tol = 2.0 * 2.220446049250313e-16;
+    for (int sweep = 0; sweep < 24; ++sweep) {
+        bool rotated = false;
+        for (int p = 1; p < D; ++p) for (int q = 0; q < p; ++q) {
+            double thr = fmax(2.2250738585072014e-308, tol * dmax);
+            if (!(fabs(X(p, q)) > thr || fabs(X(q, p)) > thr)) continue;
+            rotated = true;
+            // symmetrising rotation r1 acting on rows (p,q)
+            double a = X(p, p), b = X(p, q), c = X(q, p), d = X(q, q);
+            double tsum = a + d, diff = c - b;
+            PlaneRot r1{1.0, 0.0};
+            if (fabs(diff) >= 2.2250738585072014e-308) { double u = tsum / diff; double h = sqrt(1.0 + u * u); r1.s = 1.0 / h; r1.c = u / h; }
+            double s00 = r1.c * a + r1.s * c, s01 = r1.c * b + r1.s * d, s11 = -r1.s * b + r1.c * d;
+            PlaneRot jr = sym_jacobi(s00, s01, s11);
+            // left rotation jl = r1 * jr^T
+            PlaneRot jl{r1.c * jr.c + r1.s * jr.s, -r1.c * jr.s + r1.s * jr.c};
+            for (int k = 0; k < D; ++k) {  // rows p,q of X <- jl applied on the left
+                double xp = X(p, k), xq = X(q, k);
+                X(p, k) = jl.c * xp + jl.s * xq; X(q, k) = -jl.s * xp + jl.c * xq;
+            }
+            for (int k = 0; k < D; ++k) {  // U <- U * jl^T  (columns p,q)
+                double up = U(k, p), uq = U(k, q);
+                U(k, p) = jl.c * up + jl.s * uq; U(k, q) = -jl.s * up + jl.c * uq;
+            }
+            for (int k = 0; k < D; ++k) {  // X <- X * jr ; W <- W * jr  (columns p,q)
+                double xp = X(k, p), xq = X(k, q);
+                X(k, p) = jr.c * xp - jr.s * xq; X(k, q) = jr.s * xp + jr.c * xq;
+                double wp = W(k, p), wq = W(k, q);
+                W(k, p) = jr.c * wp - jr.s * wq; W(k, q) = jr.s * wp + jr.c * wq;
+            }
+            dmax = fmax(dmax, fmax(fabs(X(p, p)), fabs(X(q, q))));
+        }
+        if (!rotated) break;
+    }
+    for (int i = 0; i < D; ++i) {
+        double a = X(i, i);
+        s[i] = fabs(a) * scale;
+        if (a < 0) for (int k = 0; k < D; ++k) U(k, i) = -U(k, i);
+    }
+    for (int i = 0; i < D - 1; ++i) {  // selection sort, descending
+        int pos = i; double mx = s[i];
+        for (int k = i + 1; k < D; ++k) if (s[k] > mx) { mx = s[k]; pos = k; }
+        if (pos != i) {
+            double t = s[i]; s[i] = s[pos]; s[pos] = t;
+            for (int k = 0; k < D; ++k) { t = U(k, i); U(k, i) = U(k, pos); U(k, pos) = t; t = W(k, i); W(k, i) = W(k, pos); W(k, pos) = t; }
+        }
+    }
+}
+
+// Symmetric Kirchhoff stress tau = P(F) F^T stored as packed upper triangle:
+// 3D (xx, yy, zz, xy, xz, yz); 2D (xx, yy, xy).
+template <int D> struct SymPack { static constexpr int n = D == 3 ? 6 : 3; };
+
+// Hencky: tau = U diag(2 mu ln s + lambda tr(ln s)) U^T, the product HenckyPiola(F) * F^T of
+// simulation.hpp:231-238 (P = U diag((2 mu ln s + lambda tr ln s)/s) W^T, F^T = W diag(s) U^T).
+template <int D> MPM_HD void hencky_tau(const M<D>& U, const V<D>& lnS, double mu, double lambda, double* tau) {
+    double tr = lnS[0]; for (int i = 1; i < D; ++i) tr += lnS[i];
+    V<D> d; for (int i = 0; i < D; ++i) d[i] = 2.0 * mu * lnS[i] + lambda * tr;
+    auto T = [&](int r, int c) { double s = U(r, 0) * d[0] * U(c, 0); for (int k = 1; k < D; ++k) s += U(r, k) * d[k] * U(c, k); return s; };
+    if (D == 3) { tau[0] = T(0, 0); tau[1] = T(1, 1); tau[2] = T(2, 2); tau[3] = T(0, 1); tau[4] = T(0, 2); tau[5] = T(1, 2); }
+    else { tau[0] = T(0, 0); tau[1] = T(1, 1); tau[2] = T(0, 1); }
+}
+// Neo-Hookean: tau = mu (F F^T - I) + lambda ln(J) I  (NeoHookeanPiola * F^T, simulation.hpp:227-229).
+template <int D> MPM_HD void neohookean_tau(const M<D>& F, double mu, double lambda, double* tau) {
+    double lj = lambda * log(det(F));
+    auto B = [&](int r, int c) { double s = F(r, 0) * F(c, 0); for (int k = 1; k < D; ++k) s += F(r, k) * F(c, k); return s; };
+    if (D == 3) {
+        tau[0] = mu * (B(0, 0) - 1.0) + lj; tau[1] = mu * (B(1, 1) - 1.0) + lj; tau[2] = mu * (B(2, 2) - 1.0) + lj;
+        tau[3] = mu * B(0, 1); tau[4] = mu * B(0, 2); tau[5] = mu * B(1, 2);
+    } else {
+        tau[0] = mu * (B(0, 0) - 1.0) + lj; tau[1] = mu * (B(1, 1) - 1.0) + lj; tau[2] = mu * B(0, 1);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Return mapping on principal Hencky strains (plasticity.cpp:11-665).
+// ------------------------------------------------------------------------------------------------
+struct PlasticState { double eps_dev, eps_vol, eps_prad, dgamma, visc, muI; };
+
+struct RMResult {
+    bool recompose;   // F must be rebuilt as U exp(h) W^T
+    bool plastic;     // counts towards plastic_count (may be counted twice, as the reference does)
+    int plastic_incr; // number of plastic_count++ executed
+    uint32_t err;
+};
+
+// MCCRMAExplicit, mcc_rma_explicit.hpp:8-96
+MPM_HD bool mcc_explicit(double& p, double& q, double Mm, double p0, double beta, double mu, double K, double rma) {
+    double y = Mm * Mm * (p - p0) * (p + beta * p0) + (q * q);
+    if (!(y > 0)) return false;
+    double dg = 0.0, pt = p, qt = q;
+    const double dkdp = 2 * Mm * Mm, dldq = 2;
+    const int max_iter = 40;
+    for (int iter = 0; iter < max_iter; ++iter) {
+        y = Mm * Mm * (p - p0) * (p + beta * p0) + (q * q);
+        double k = Mm * Mm * (beta * p0 + 2 * p - p0);
+        double l = 2 * q;
+        double r1 = pt - p - K * dg * k;
+        double r2 = qt - q - rma * mu * dg * l;
+        if (iter > 4 && fabs(y) < 1e-3 && fabs(r1) < 1e-3 && fabs(r2) < 1e-3) break;
+        if (iter == max_iter - 1) { if (!(p0 > 0.1)) { p = 1e-15; q = 1e-15; } break; }
+        double J11 = -1 - K * dg * dkdp, J13 = -K * k;
+        double J22 = -1 - rma * mu * dg * dldq, J23 = -rma * mu * l;
+        double J31 = k, J32 = l;
+        double dt_ = J11 * (-J32 * J23) + J13 * (-J31 * J22);
+        if (fabs(dt_) < 1e-6) {
+            p -= 0.001 * r1; q -= 0.001 * r2; dg -= 0.001 * y / (p0 * p0);
+        } else {
+            p -= (-J32 * J23 * r1 + J32 * J13 * r2 - J22 * J13 * y) / dt_;
+            q -= (J31 * J23 * r1 - J31 * J13 * r2 - J11 * J23 * y) / dt_;
+            dg -= (-J31 * J22 * r1 - J11 * J32 * r2 + J11 * J22 * y) / dt_;
+        }
+        if (q < 1e-15) q = 1e-15;
+    }
+    p = fmax(p, -beta * p0);
+    p = fmin(p, p0);
+    q = Mm * sqrt((p0 - p) * (beta * p0 + p));
+    return true;
+}
+
+// Consolidation pressure laws of the implicit MCC maps: exponential p00*exp(-xi*epv)
+// (mcc_rma_implicit_exponential_onevar.hpp:136,181) and sinh K*sinh(-xi*epv + asinh(p00/K)) floored at
+// p0_min (mcc_rma_implicit_sinh_onevar.hpp:37,78-79).
+// Admissible scaled Newton step: limited_search_exponential.hpp:8-71 / limited_search_sinh.hpp:8-72.
+template <bool SINH>
+MPM_HD double mcc_limited_search(double epv, double epv_t, double delta, double xi, double K, double p_t, double p00,
+                                 double p0_min, double beta, int& right) {
+    double scale = 1.0; int tries = 0; const int max_tries = 32; int f = 0; double test = epv;
+    while (tries < max_tries) {
+        test = epv - scale * delta;
+        double pp0 = SINH ? fmax(p0_min, K * sinh(-xi * test + asinh(p00 / K))) : p00 * exp(-xi * test);
+        double pp = K * (test - epv_t) + p_t;
+        double val = 2. * pp + (beta - 1.) * pp0;
+        double mid = (pp0 - beta * pp0) * 0.5;
+        if (fabs(pp - mid) <= 1.e-5) break;
+        if ((right == 1 && (pp < mid || p_t < mid)) || (right == 0 && (pp > mid || p_t > mid))) { test = epv; right = -1; break; }
+        if (sgn_d(epv_t - test) == sgn_d(val) || sgn_d(epv_t - test) == 0) break;
+        scale *= 0.5;
+        tries++;
+        if (tries != max_tries) continue;
+        tries = 0; scale = 1.; delta *= -1.; f++;
+        if (f > 1) return INFINITY;
+    }
+    return test;
+}
+
+// MCCRMAImplicitExponentialOnevar (mcc_rma_implicit_exponential_onevar.hpp:9-166) and
+// MCCRMAImplicitSinhOnevar (mcc_rma_implicit_sinh_onevar.hpp:9-171): 1-variable Newton on epv with
+// values normalised by the trial consolidation pressure.
+template <bool SINH>
+MPM_HD bool mcc_implicit(double& p, double& q, uint32_t& err, double Mm, double p00, double beta, double mu, double K,
+                         double xi, double epv) {
+    double p0 = SINH ? fmax(0.01, K * sinh(-xi * epv + asinh(p00 / K))) : p00 * exp(-xi * epv);
+    const double Msq = Mm * Mm, damp = 1.0;
+    double y = Msq * (p * p + (beta - 1.) * p0 * p - beta * p0 * p0) + damp * q * q;
+    if (y <= 0) return false;
+    const double p0_t = p0;
+    const double sf = 1. / p0_t;
+    double p0_min = SINH ? 0.01 * sf : 0.0;
+    p00 *= sf; p0 = 1.; p *= sf; q *= sf; mu *= sf; K *= sf;
+    const double p_t = p, q_t = q, epv_t = epv;
+    int right = (p_t > 0.5 * (p0 - beta * p0)) ? 1 : 0;
+    const int max_iter = SINH ? 40 : 30;
+    for (int iter = 0; iter < max_iter; ++iter) {
+        double mid = (p0 - beta * p0) * 0.5;
+        if (fabs(p - mid) < 1e-4) {
+            q = Mm * sqrt((p0 - p) * (beta * p0 + p) / damp);
+            p *= p0_t; q *= p0_t;
+            return true;
+        }
+        p = (epv - epv_t) * K + p_t;
+        double p0h = 0.0;
+        if (SINH) { p0h = K * sinh(-xi * epv + asinh(p00 / K)); p0 = fmax(p0_min, p0h); }
+        else p0 = p00 * exp(-xi * epv);
+        double denom = 2. * K * Msq * p + Msq * (beta - 1.) * K * p0;
+        double dg = (p_t - p) / denom;
+        double qd = 1 + 2. * mu * damp * dg;
+        q = q_t / qd;
+        y = Msq * (p - p0) * (p + beta * p0) + damp * q * q;
+        if (iter > 5 && fabs(y) < 1e-4) break;
+        if (SINH && iter == max_iter - 1 && p0 > p0_min) err |= MPM_ERR_MCC_NO_STEP;
+        double dp = K, dp0;
+        if (SINH) {
+            double hh = -xi * K * cosh(-xi * epv + asinh(p00 / K));
+            dp0 = (p0h > p0_min) ? 0.5 * (hh + hh - p0_min) : 0.5 * (hh + p0_min - hh);
+        } else dp0 = -xi * p0;
+        double ddg = (-dp / denom) - (p_t - p) * (2. * K * Msq * dp + Msq * (beta - 1.) * K * dp0) / (denom * denom);
+        double dq = -q_t * 2. * mu * damp * ddg / (qd * qd);
+        double dy = Msq * (dp - dp0) * (p + beta * p0) + Msq * (p - p0) * (beta * dp0 + dp) + 2. * damp * q * dq;
+        epv = mcc_limited_search<SINH>(epv, epv_t, y / dy, xi, K, p_t, p00, p0_min, beta, right);
+        if (epv == INFINITY) { err |= MPM_ERR_MCC_NO_STEP; break; }
+    }
+    p = fmax(p, -beta * p0);
+    p = fmin(p, p0);
+    q = Mm * sqrt((p0 - p) * (beta * p0 + p) / damp);
+    p *= p0_t; q *= p0_t;
+    return true;
+}
+
+// Perzyna Newton iteration on delta_gamma (VMVisc plasticity.cpp:197-229 with hardening of q_yield;
+// DPVisc :293-323 and MCCVisc :558-588 without).
+MPM_HD double perzyna_newton(const Params& P, double dt, double q_trial, double q_yield, double eps_dev, bool vm_hard, uint32_t& err) {
+    double dg = 0.01 * (q_trial - q_yield) / P.f_mu_prefac;
+    const int max_iter = 60;
+    for (int iter = 0; iter < max_iter; ++iter) {
+        if (iter == max_iter - 1) err |= MPM_ERR_NEWTON_MAXITER;
+        double tm = P.perzyna_visc * dg + dt;
+        double tmp = dt / tm;
+        double tmp1 = pow(tmp, P.perzyna_exp);
+        if (vm_hard) q_yield = P.q_min + (P.q_max - P.q_min) * exp(-P.xi * (eps_dev + (1.0 / P.d_prefac) * dg));
+        double res = (q_trial - P.f_mu_prefac * dg) * tmp1 - q_yield;
+        if (fabs(res) < 1e-2) break;
+        double rd = -P.f_mu_prefac * tmp1 + (q_trial - P.f_mu_prefac * dg) * P.perzyna_exp * pow(tmp, P.perzyna_exp - 1) * (-P.perzyna_visc * dt) / (tm * tm);
+        if (vm_hard) rd -= -P.xi / P.d_prefac * (P.q_max - P.q_min) * exp(-P.xi * (eps_dev + (1.0 / P.d_prefac) * dg));
+        if (fabs(rd) < 1e-14) err |= MPM_ERR_NEWTON_SMALL_DERIV;
+        dg -= res / rd;
+        if (dg < 0) dg = 1e-10;
+    }
+    return dg;
+}
+
+// plasticity(), plasticity.cpp:17-621. `h` holds the clamped principal Hencky strains
+// ln(max(|s|,1e-4)) of F_trial on entry and the projected strains on exit when recompose is set.
+// For VMVisc's capped branch the result is F = U W^T, i.e. h = 0.
+template <int D>
+MPM_HD RMResult return_map(const Params& P, double dt, V<D>& h, PlasticState& st) {
+    RMResult r{false, false, 0, 0u};
+    const double dim = D;
+    double tr = h[0]; for (int i = 1; i < D; ++i) tr += h[i];
+    V<D> dev; for (int i = 0; i < D; ++i) dev[i] = h[i] - (tr / dim) * 1.0;
+    double nrm2 = dev[0] * dev[0]; for (int i = 1; i < D; ++i) nrm2 += dev[i] * dev[i];
+    const double dn = sqrt(nrm2);
+    if (dn > 0) for (int i = 0; i < D; ++i) dev[i] /= dn;
+    const double K = P.K, inv_d = 1.0 / P.d_prefac;
+    auto set_iso = [&](double pp) { double e = -pp / (K * dim); for (int i = 0; i < D; ++i) h[i] = e * 1.0; r.recompose = true; };
+    auto shift_dev = [&](double amount) { for (int i = 0; i < D; ++i) h[i] -= amount * dev[i]; r.recompose = true; };
+
+    switch (P.plastic_model) {
+    case MPM_PLASTIC_VM: {  // :30-49
+        double dg = dn - P.q_max / P.e_mu_prefac;
+        if (dg > 0) {
+            r.plastic_incr++;
+            st.dgamma = P.d_prefac * dg / dt;
+            shift_dev(dg);
+            st.eps_dev += dg;
+        }
+    } break;
+    case MPM_PLASTIC_DP: {  // :51-85
+        double p_trial = -K * tr;
+        double q_yield = P.M * p_trial + P.q_cohesion;
+        if (q_yield < 1e-10) {
+            r.plastic_incr++;
+            double p_proj = -P.q_cohesion / P.M;
+            set_iso(p_proj);
+            double dg = P.d_prefac * dn;
+            st.dgamma = dg / dt;
+            st.eps_dev += inv_d * dg;
+            st.eps_vol += (p_proj - p_trial) / K;
+        } else {
+            double dg = P.d_prefac * (dn - q_yield / P.e_mu_prefac);
+            if (dg > 0) {
+                r.plastic_incr++;
+                st.dgamma = dg / dt;
+                shift_dev(inv_d * dg);
+                st.eps_dev += inv_d * dg;
+            }
+        }
+    } break;
+    case MPM_PLASTIC_DPSOFT: {  // :87-162
+        double p_trial = -K * tr, q_trial = P.e_mu_prefac * dn;
+        double p_tip_orig = -P.q_cohesion / P.M;
+        double p_tip = p_tip_orig * exp(-P.xi * st.eps_dev);
+        double p_shift = P.use_pradhana ? -K * st.eps_prad : 0.0;
+        double q_yield = P.M * (p_trial + p_shift) + (-p_tip * P.M);
+        double ps = p_trial + p_shift;
+        if (ps <= p_tip) {
+            r.plastic_incr++;
+            double p_proj = p_tip_orig * exp(-P.xi * st.eps_dev);
+            set_iso(p_proj);
+            double dg = P.d_prefac * dn;
+            st.dgamma = dg / dt;
+            st.eps_dev += inv_d * dg;
+            double inst = (p_proj - p_trial) / K;
+            st.eps_vol += inst;
+            if (P.use_pradhana) st.eps_prad += inst;
+        }
+        if (ps > p_tip && q_trial <= q_yield) {
+            if (P.use_pradhana) st.eps_prad = 0;
+            st.dgamma = 0;
+        }
+        if (ps > p_tip && q_trial > q_yield) {
+            r.plastic_incr++;
+            double tmp_dev = st.eps_dev + (dn - q_yield / P.e_mu_prefac);
+            double p_proj = p_tip_orig * exp(-P.xi * tmp_dev);
+            if (ps < p_proj) {
+                double dg = P.d_prefac * dn;
+                st.dgamma = dg / dt;
+                st.eps_dev += inv_d * dg;
+                double inst = (p_proj - p_trial) / K;
+                st.eps_vol += inst;
+                if (P.use_pradhana) st.eps_prad += inst;
+                set_iso(p_proj);
+            } else {
+                double q_yield_new = P.M * ps + (-p_proj * P.M);
+                double dg = P.d_prefac * (dn - q_yield_new / P.e_mu_prefac);
+                shift_dev(inv_d * dg);
+                st.eps_dev += inv_d * dg;
+                st.dgamma = dg / dt;
+                if (P.use_pradhana) st.eps_prad = 0;
+            }
+        }
+    } break;
+    case MPM_PLASTIC_VMVISC: {  // :164-238
+        double q_trial = P.e_mu_prefac * dn;
+        double q_yield = P.q_min + (P.q_max - P.q_min) * exp(-P.xi * st.eps_dev);
+        double p_trial = -K * tr;
+        if (p_trial < P.p_min * exp(-P.xi * st.eps_vol)) {
+            double dg = q_trial / P.f_mu_prefac;
+            double inst = -p_trial / K;
+            for (int i = 0; i < D; ++i) h[i] = 0.0;
+            r.recompose = true;
+            st.eps_vol += inst;
+            st.eps_dev += inv_d * dg;
+            st.dgamma = dg / dt;
+        } else if (q_trial > q_yield) {
+            r.plastic_incr++;
+            double dg;
+            if (P.perzyna_exp == 1 && P.xi == 0) {
+                dg = (q_trial - q_yield) / (P.f_mu_prefac + q_yield * P.perzyna_visc / dt);
+                if (dg < 0) r.err |= MPM_ERR_PERZYNA_NEGATIVE;
+            } else {
+                dg = perzyna_newton(P, dt, q_trial, q_yield, st.eps_dev, true, r.err);
+            }
+            shift_dev(inv_d * dg);
+            st.eps_dev += inv_d * dg;
+            st.dgamma = dg / dt;
+        }
+    } break;
+    case MPM_PLASTIC_DPVISC: {  // :240-339
+        double p_trial = -K * tr, q_trial = P.e_mu_prefac * dn;
+        double p_tip = -P.q_cohesion / P.M;
+        double p_shift = P.use_pradhana ? -K * st.eps_prad : 0.0;
+        double ps = p_trial + p_shift;
+        double q_yield = P.M * ps + P.q_cohesion;
+        if (P.use_mibf) st.muI = P.M;
+        if (ps < p_tip) {
+            double dg = P.d_prefac * dn;
+            double p_proj = p_tip;
+            double inst = (p_proj - p_trial) / K;
+            r.plastic_incr++;
+            set_iso(p_proj);
+            st.dgamma = dg / dt;
+            st.eps_dev += inv_d * dg;
+            st.eps_vol += inst;
+            if (P.use_pradhana) st.eps_prad += inst;
+        } else {
+            if (P.use_pradhana) st.eps_prad = 0;
+            st.dgamma = 0;
+        }
+        if (ps > p_tip && q_trial > q_yield) {
+            r.plastic_incr++;
+            double dg;
+            if (P.perzyna_exp == 1) {
+                dg = (q_trial - q_yield) / (P.f_mu_prefac + q_yield * P.perzyna_visc / dt);
+                if (dg < 0) r.err |= MPM_ERR_PERZYNA_NEGATIVE;
+            } else {
+                dg = perzyna_newton(P, dt, q_trial, q_yield, st.eps_dev, false, r.err);
+            }
+            if (P.use_mibf) {
+                if (fabs(ps) < 1e-10) st.muI = 1e10;
+                else st.muI = ((q_trial - P.f_mu_prefac * dg) - P.q_cohesion) / ps;
+            }
+            shift_dev(inv_d * dg);
+            st.eps_dev += inv_d * dg;
+            st.dgamma = dg / dt;
+        }
+    } break;
+    case MPM_PLASTIC_DPMUI: {  // :342-403
+        double p_trial = -K * tr, q_trial = P.e_mu_prefac * dn;
+        double p_tip = -P.q_cohesion / P.mu_1;
+        double p_shift = P.use_pradhana ? -K * st.eps_prad : 0.0;
+        double ps = p_trial + p_shift;
+        st.muI = P.mu_1;
+        st.visc = 0;
+        if (ps < p_tip) {
+            double dg = P.d_prefac * dn;
+            double inst = (p_tip - p_trial) / K;
+            r.plastic_incr++;
+            set_iso(p_tip);
+            st.dgamma = dg;  // not divided by dt (plasticity.cpp:364)
+            st.eps_dev += inv_d * dg;
+            st.eps_vol += inst;
+            st.eps_prad += inst;
+        } else {
+            st.eps_prad = 0;
+        }
+        double q_yield = P.mu_1 * ps + P.q_cohesion;
+        if (ps > p_tip && q_trial > (q_yield + P.stress_tolerance)) {
+            r.plastic_incr++;
+            double p_special = ps - p_tip + P.stress_tolerance;
+            double fa = P.f_mu_prefac * dt;
+            double fb = p_trial * (P.mu_2 - P.mu_1) + P.f_mu_prefac * dt * P.fac_Q * sqrt(p_special) - (q_trial - q_yield);
+            double fc = -(q_trial - q_yield) * P.fac_Q * sqrt(p_special);
+            double dg = (-fb + sqrt(fb * fb - 4 * fa * fc)) / (2 * fa);
+            double mu_i = P.mu_1 + (P.mu_2 - P.mu_1) / (P.fac_Q * sqrt(p_special) / dg + 1.0);
+            st.muI = mu_i;
+            st.visc = (mu_i - P.mu_1) * p_special / dg;
+            dg *= dt;
+            shift_dev(inv_d * dg);
+            st.eps_dev += inv_d * dg;
+            st.dgamma = dg / dt;
+        }
+    } break;
+    case MPM_PLASTIC_MCCMUI:
+    case MPM_PLASTIC_MCC:
+    case MPM_PLASTIC_MCCVISC: {  // MCCMui :405-484, MCC/MCCVisc :487-621
+        const bool mui = P.plastic_model == MPM_PLASTIC_MCCMUI;
+        const double Mm = mui ? P.mu_1 : P.M;
+        double p_stress = -K * tr, q_stress = P.e_mu_prefac * dn;
+        const double q_trial = q_stress;
+        if (P.plastic_model == MPM_PLASTIC_MCCVISC && P.use_mibf) st.muI = P.M;
+        bool rma = false;
+        double p_c = 0;
+        switch (P.hardening_law) {
+        case MPM_HARD_NONE:
+            if (mui) r.err |= MPM_ERR_INVALID_ARGUMENT;
+            else rma = mcc_explicit(p_stress, q_stress, Mm, P.p0, P.beta, P.mu, K, P.rma_prefac);
+            break;
+        case MPM_HARD_EXPO_EXPL:
+            p_c = fmax(P.stress_tolerance, P.p0 * exp(-P.xi * st.eps_vol));
+            rma = mcc_explicit(p_stress, q_stress, Mm, p_c, P.beta, P.mu, K, P.rma_prefac);
+            break;
+        case MPM_HARD_SINH_EXPL:
+            p_c = fmax(P.stress_tolerance, K * sinh(-P.xi * st.eps_vol + asinh(P.p0 / K)));
+            rma = mcc_explicit(p_stress, q_stress, Mm, p_c, P.beta, P.mu, K, P.rma_prefac);
+            break;
+        case MPM_HARD_EXPO_IMPL:
+            rma = mcc_implicit<false>(p_stress, q_stress, r.err, Mm, P.p0, P.beta, P.mu, K, P.xi, st.eps_vol);
+            break;
+        case MPM_HARD_SINH_IMPL:
+            rma = mcc_implicit<true>(p_stress, q_stress, r.err, Mm, P.p0, P.beta, P.mu, K, P.xi, st.eps_vol);
+            break;
+        default: r.err |= MPM_ERR_INVALID_ARGUMENT; break;
+        }
+        if (mui) { st.muI = P.mu_1; st.visc = 0; }
+        if (rma) {
+            r.plastic_incr++;
+            double ep = p_stress / (K * dim);
+            double inst = tr + dim * ep;
+            st.eps_vol += inst;
+            if (mui) {
+                if (q_trial > (q_stress + P.stress_tolerance)) {
+                    if (P.hardening_law == MPM_HARD_EXPO_IMPL) p_c = fmax(P.p0 * exp(-P.xi * st.eps_vol), p_stress + P.stress_tolerance);
+                    else if (P.hardening_law == MPM_HARD_SINH_IMPL) p_c = fmax(K * sinh(-P.xi * st.eps_vol + asinh(P.p0 / K)), p_stress + P.stress_tolerance);
+                    double p_special = p_stress + P.beta * p_c + P.stress_tolerance;
+                    double fa = P.f_mu_prefac * dt;
+                    double fb = sqrt((p_c - p_stress) * p_special) * (P.mu_2 - P.mu_1) + P.f_mu_prefac * dt * P.fac_Q * sqrt(p_special) - (q_trial - q_stress);
+                    double fc = -(q_trial - q_stress) * P.fac_Q * sqrt(p_special);
+                    double gds = (-fb + sqrt(fb * fb - 4 * fa * fc)) / (2 * fa);
+                    q_stress = fmax(q_stress, q_trial - P.f_mu_prefac * dt * gds);
+                    double mu_i = P.mu_1 + (P.mu_2 - P.mu_1) / (P.fac_Q * sqrt(p_special) / gds + 1.0);
+                    st.muI = mu_i;
+                    st.visc = (mu_i - P.mu_1) * sqrt((p_c - p_stress) * p_special) / gds;
+                    double dg = (q_trial - q_stress) / P.f_mu_prefac;
+                    st.eps_dev += inv_d * dg;
+                    st.dgamma = dg / dt;
+                }
+            } else {
+                if (P.plastic_model == MPM_PLASTIC_MCCVISC) {
+                    double q_yield = q_stress, dg;
+                    if (P.perzyna_exp == 1) {
+                        dg = (q_trial - q_yield) / (P.f_mu_prefac + q_yield * P.perzyna_visc / dt);
+                        if (dg < 0) { if (dg > -1e-14) dg = 0; else r.err |= MPM_ERR_PERZYNA_NEGATIVE; }
+                    } else {
+                        dg = perzyna_newton(P, dt, q_trial, q_yield, st.eps_dev, false, r.err);
+                    }
+                    q_stress = fmax(q_yield, q_trial - P.f_mu_prefac * dg);
+                    if (P.use_mibf) {
+                        if (P.hardening_law == MPM_HARD_EXPO_IMPL) p_c = fmax(0.0, P.p0 * exp(-P.xi * st.eps_vol));
+                        else if (P.hardening_law == MPM_HARD_SINH_IMPL) p_c = fmax(0.0, K * sinh(-P.xi * st.eps_vol + asinh(P.p0 / K)));
+                        if ((p_stress < -P.beta * p_c + P.stress_tolerance) || (p_stress > p_c - P.stress_tolerance)) st.muI = P.M;
+                        else st.muI = q_stress / sqrt((p_stress + P.beta * p_c) * (p_c - p_stress));
+                    }
+                }
+                double dg = (q_trial - q_stress) / P.f_mu_prefac;
+                st.eps_dev += inv_d * dg;
+                st.dgamma = dg / dt;
+            }
+            for (int i = 0; i < D; ++i) h[i] = (q_stress / P.e_mu_prefac) * dev[i] - ep * 1.0;
+            r.recompose = true;
+        }
+    } break;
+    default: break;  // NoPlasticity
+    }
+    r.plastic = r.plastic_incr > 0;
+    return r;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Boundary conditions on one grid node (boundaryCollision, boundary_collision.cpp:5-394).
+// ------------------------------------------------------------------------------------------------
+template <int D> MPM_HD V<D> normalized_v(V<D> n) {
+    double z = n[0] * n[0]; for (int i = 1; i < D; ++i) z += n[i] * n[i];
+    if (z > 0) { double s = sqrt(z); for (int i = 0; i < D; ++i) n[i] = n[i] / s; }
+    return n;
+}
+
+// inside()/normal() of the analytic objects (src/objects/object_*.hpp).
+template <int D> MPM_HD bool object_inside(const DevObject& o, const V<D>& X) {
+    const double x = X[0], y = X[1];
+    switch (o.kind) {
+    case MPM_OBJECT_BOX_ROTATED: {  // object_box.hpp:18-24
+        for (int i = 0; i < D; ++i) {
+            double l = o.rot[0 + i * D] * (X[0] - o.center[0]);
+            for (int k = 1; k < D; ++k) l += o.rot[k + i * D] * (X[k] - o.center[k]);
+            if (fabs(l) > o.half_extents[i]) return false;
+        }
+        return true;
+    }
+    case MPM_OBJECT_BUMP: return y < 0.0475 / cosh(25 * (x - 0.43));          // object_bump.hpp:15-27
+    case MPM_OBJECT_GROUND: return !(y > o.param);                              // object_ground.hpp:16-25
+    case MPM_OBJECT_RAMP: return y < 0.1 * tanh(-100 * x);                      // object_ramp.hpp:15-26
+    case MPM_OBJECT_GATE: return y > o.param + 100 * x * x;                     // object_gate.hpp:16-28
+    case MPM_OBJECT_SILO: {                                                     // object_silo.hpp:16-35
+        double z = X[D - 1];
+        if (y < o.param) return false;
+        double rs = tanh(y) + 1;
+        return !(x * x + z * z < rs * rs);
+    }
+    case MPM_OBJECT_CURVE: return !(y > x * x);                                 // object_curve.hpp:15-25
+    default: return false;
+    }
+}
+template <int D> MPM_HD V<D> object_normal(const DevObject& o, const V<D>& X) {
+    V<D> n; for (int i = 0; i < D; ++i) n[i] = 0.0;
+    const double x = X[0];
+    switch (o.kind) {
+    case MPM_OBJECT_BOX_ROTATED: {  // object_box.hpp:26-43
+        double loc[3] = {0, 0, 0};
+        for (int i = 0; i < D; ++i) { double l = o.rot[0 + i * D] * (X[0] - o.center[0]); for (int k = 1; k < D; ++k) l += o.rot[k + i * D] * (X[k] - o.center[k]); loc[i] = l; }
+        int ax = 0; double md = 0.0;
+        for (int i = 0; i < D; ++i) { double dd = fabs(o.half_extents[i] - fabs(loc[i])); if (dd > md) { md = dd; ax = i; } }
+        double sgn = (loc[ax] > 0) ? 1.0 : -1.0;
+        double ln[3] = {0, 0, 0}; ln[ax] = sgn;
+        for (int r = 0; r < D; ++r) { double s = o.rot[r + 0 * D] * ln[0]; for (int k = 1; k < D; ++k) s += o.rot[r + k * D] * ln[k]; n[r] = s; }
+        return n;
+    }
+    case MPM_OBJECT_BUMP: { double arg = 25 * (x - 0.43); double b = -25 * 0.0475 * tanh(arg) / cosh(arg); n[0] = -b; n[1] = 1; return normalized_v(n); }
+    case MPM_OBJECT_GROUND: n[1] = 1; return n;
+    case MPM_OBJECT_RAMP: { double t = cosh(-100 * x); double b = 0.1 * (-100) / (t * t); n[0] = -b; n[1] = 1; return normalized_v(n); }
+    case MPM_OBJECT_GATE: { double b = 2.0 * 100 * x; n[0] = -b; n[1] = 1; for (int i = 0; i < D; ++i) n[i] *= -1; return normalized_v(n); }
+    case MPM_OBJECT_SILO: { double z = X[D - 1], y = X[1]; double th = atan2(z, x); double sy = 1.0 / cosh(y); n[0] = -cos(th); n[1] = sy * sy; n[D - 1] = -sin(th); return normalized_v(n); }
+    case MPM_OBJECT_CURVE: n[0] = -2 * x; n[1] = 1; return normalized_v(n);
+    default: return n;
+    }
+}
+
+// ObjectPlate::inside, object_plate.hpp:30-78 (3D) / :98-131 (2D)
+template <int D> MPM_HD bool plate_inside(const DevPlate& p, const V<D>& X) {
+    switch (p.plate_type) {
+    case MPM_PLATE_LEFT: return X[1] < p.pos_upper && X[1] > p.pos_lower && (X[0] - p.pos_object) <= 0;
+    case MPM_PLATE_RIGHT: return X[1] < p.pos_upper && X[1] > p.pos_lower && (p.pos_object - X[0]) <= 0;
+    case MPM_PLATE_BOTTOM: return X[0] < p.pos_upper && X[0] > p.pos_lower && (X[1] - p.pos_object) <= 0;
+    case MPM_PLATE_TOP: return X[0] < p.pos_upper && X[0] > p.pos_lower && (p.pos_object - X[1]) <= 0;
+    case MPM_PLATE_BACK: return D == 3 && X[D - 1] - p.pos_object <= 0;
+    case MPM_PLATE_FRONT: return D == 3 && p.pos_object - X[D - 1] <= 0;
+    default: return false;
+    }
+}
+
+// 3D Coulomb reduction of the two tangential components; the second uses the updated first
+// (boundary_collision.cpp:106-119 and its five siblings).
+MPM_HD void plate_tangent3(double& a, double& b, double n_comp, double fr) {
+    double vt = sqrt(a * a + b * b);
+    double fn = fr * fabs(n_comp);
+    if (fr == 0) return;
+    if (vt <= fn) { a = 0; b = 0; return; }
+    a = sgn_d(a) * (vt - fn) / sqrt(1 + b * b / (a * a));
+    b = sgn_d(b) * (vt - fn) / sqrt(1 + a * a / (b * b));
+}
+// 2D Coulomb reduction with the strict stick test (boundary_collision.cpp:286-297).
+MPM_HD void plate_tangent2(double& a, double n_comp, double fr) {
+    if (fr == 0) return;
+    if (fabs(a) < fr * fabs(n_comp)) { a = 0; return; }
+    if (a > 0) a -= fr * fabs(n_comp); else a += fr * fabs(n_comp);
+}
+
+// boundaryCollision(index, Xi, vi): objects first (chained), then plates (chained).
+template <int D>
+MPM_HD void boundary_collision(const Params& P, const DevPlate* plates, double dt, double node_friction, V<D> X, V<D>& vi) {
+    V<D> vo = vi;
+    for (int i = 0; i < D; ++i) X[i] += dt * vi[i];
+    for (int k = 0; k < P.n_objects; ++k) {
+        const DevObject& o = P.objects[k];
+        if (!object_inside<D>(o, X)) continue;
+        V<D> vr = vo;
+        if (o.bc == MPM_BC_NOSLIP) { for (int i = 0; i < D; ++i) vr[i] = 0.0; }
+        else {
+            V<D> n = object_normal<D>(o, X);
+            double dn = vr[0] * n[0]; for (int i = 1; i < D; ++i) dn += vr[i] * n[i];
+            double fr = P.use_mibf ? node_friction : o.friction;
+            if (o.bc == MPM_BC_SLIPFREE) {
+                if (dn < 0) {
+                    V<D> vt; for (int i = 0; i < D; ++i) vt[i] = vr[i] - dn * n[i];
+                    if (fr > 0) {
+                        double z = vt[0] * vt[0]; for (int i = 1; i < D; ++i) z += vt[i] * vt[i];
+                        if ((-dn) * fr < sqrt(z)) { V<D> u = normalized_v(vt); for (int i = 0; i < D; ++i) vr[i] = vt[i] + u[i] * dn * fr; }
+                        else for (int i = 0; i < D; ++i) vr[i] = 0.0;
+                    } else vr = vt;
+                }
+            } else {  // SlipStick
+                for (int i = 0; i < D; ++i) vr[i] = vr[i] - dn * n[i];
+                if (dn < 0 && fr > 0) {
+                    double z = vr[0] * vr[0]; for (int i = 1; i < D; ++i) z += vr[i] * vr[i];
+                    if ((-dn) * fr < sqrt(z)) { V<D> u = normalized_v(vr); for (int i = 0; i < D; ++i) vr[i] = vr[i] + u[i] * dn * fr; }
+                    else for (int i = 0; i < D; ++i) vr[i] = 0.0;
+                }
+            }
+        }
+        vi = vr;
+        vo = vi;
+    }
+    for (int k = 0; k < P.n_plates; ++k) {
+        const DevPlate& pl = plates[k];
+        if (!plate_inside<D>(pl, X)) continue;
+        double v[3] = {0, 0, 0};
+        for (int i = 0; i < D; ++i) v[i] = vo[i] - pl.v_object[i];
+        if (pl.bc == MPM_BC_NOSLIP) { v[0] = v[1] = v[2] = 0; }
+        else {
+            const double fr = P.use_mibf ? node_friction : pl.friction;
+            const bool sf = pl.bc == MPM_BC_SLIPFREE;
+            const int t = pl.plate_type;
+            const bool tb = t == MPM_PLATE_TOP || t == MPM_PLATE_BOTTOM;
+            const bool lr = t == MPM_PLATE_LEFT || t == MPM_PLATE_RIGHT;
+            const bool fbk = t == MPM_PLATE_FRONT || t == MPM_PLATE_BACK;
+            if (tb && (!sf || (t == MPM_PLATE_TOP && v[1] > 0) || (t == MPM_PLATE_BOTTOM && v[1] < 0))) {
+                if (D == 3) plate_tangent3(v[0], v[2], v[1], fr); else plate_tangent2(v[0], v[1], fr);
+                v[1] = 0;
+            } else if (lr && (!sf || (t == MPM_PLATE_LEFT && v[0] < 0) || (t == MPM_PLATE_RIGHT && v[0] > 0))) {
+                if (D == 3) plate_tangent3(v[1], v[2], v[0], fr); else plate_tangent2(v[1], v[0], fr);
+                v[0] = 0;
+            } else if (D == 3 && fbk && (!sf || (t == MPM_PLATE_FRONT && v[2] > 0) || (t == MPM_PLATE_BACK && v[2] < 0))) {
+                plate_tangent3(v[0], v[1], v[2], fr);
+                v[2] = 0;
+            }
+        }
+        for (int i = 0; i < D; ++i) vi[i] = v[i] + pl.v_object[i];
+        vo = vi;
+    }
+}
+
+// ObjectPlate::move, object_plate.hpp:136-162
+template <int D> MPM_HD void plate_move(DevPlate& p, double dt, double frame_dt, double time) {
+    double factor = (time < p.load_factor * frame_dt) ? (1.0 / p.vmin_factor) : 1.0;
+    for (int i = 0; i < D; ++i) p.v_object[i] = p.v_object_original[i] * factor;
+    if (p.plate_type == MPM_PLATE_LEFT || p.plate_type == MPM_PLATE_RIGHT) {
+        p.pos_object += dt * p.v_object[0]; p.pos_upper += dt * p.v_object[1]; p.pos_lower += dt * p.v_object[1];
+    } else if (p.plate_type == MPM_PLATE_BOTTOM || p.plate_type == MPM_PLATE_TOP) {
+        p.pos_object += dt * p.v_object[1]; p.pos_upper += dt * p.v_object[0]; p.pos_lower += dt * p.v_object[0];
+    } else if (D == 3 && (p.plate_type == MPM_PLATE_BACK || p.plate_type == MPM_PLATE_FRONT)) {
+        p.pos_object += dt * p.v_object[2];
+    }
+}
+
+}  // namespace mpm

@@ -1,0 +1,73 @@
+// mpm_types.hpp — plain-old-data records shared by the host side of the engine (context, C ABI) and the
+// gfx950 kernels. No HIP types appear here, so the header compiles in both passes.
+#pragma once
+#include <stdint.h>
+
+namespace mpm {
+
+constexpr int kMaxPlates = 16;
+constexpr int kMaxObjects = 16;
+
+// Plate state (ObjectPlate, src/objects/object_plate.hpp:164-188). Mutable: moveObjects() advances it.
+struct DevPlate {
+    double pos_object, pos_lower, pos_upper;
+    double v_object[3], v_object_original[3];
+    double friction, vmin_factor, load_factor;
+    int32_t plate_type, bc;
+};
+
+// Analytic object (ObjectGeneral subclasses, src/objects/object_*.hpp). Immutable.
+struct DevObject {
+    int32_t kind, bc;
+    double friction, param;
+    double center[3], half_extents[3];
+    double rot[9];  // dim x dim column-major
+};
+
+// Constants of a context: the simulate() set-up values (src/simulation/simulation.cpp:94-145) and the
+// material/boundary parameters (src/simulation/simulation.hpp:28-110). Passed to kernels by value.
+struct Params {
+    int32_t dim, spline_degree, elastic_model, plastic_model, hardening_law;
+    int32_t use_pradhana, use_mises_q, use_mibf, use_musl, gravity_special, change_particle_positions;
+    int32_t n_objects, n_plates;
+    uint32_t end_frame;
+    double dx, one_over_dx, apicDinverse, particle_mass, particle_volume;
+    double cfl, dt_max, frame_dt, final_time, wave_speed, flip_ratio, gravity_time;
+    double gravity_final[3], grid_reference_point[3];
+    double E, nu, mu, lambda, K, fac_Q, stress_tolerance, xi;
+    double q_prefac, d_prefac, e_mu_prefac, f_mu_prefac, rma_prefac;
+    double q_max, q_min, p_min, M, q_cohesion, perzyna_exp, perzyna_visc, beta, p0;
+    double rho_s, grain_diameter, I_ref, mu_1, mu_2;
+    DevObject objects[kMaxObjects];
+};
+
+// Device-resident loop state of simulate() plus the per-step grid geometry and reductions.
+// Written by the single-workgroup set-up/finish kernels, read by every other kernel.
+struct StepState {
+    // loop state (simulation.hpp:151-153, simulation.cpp:154-156)
+    double time, dt;
+    uint64_t step;
+    uint32_t frame, pad0;
+    double gravity[3];
+    // grid geometry of the current step (remesh.cpp:282-301)
+    double origin[3];
+    uint32_t n[3], grid_nodes;
+    // remeshFixedInit bookkeeping (remesh.cpp:106-131)
+    double min_init[3], max_init[3], low_init[3], high_init[3];
+    // reductions over the particles of the previous step's end (update_dt.cpp:7-12, remesh.cpp:186-223)
+    unsigned long long vmax2_bits;   // max |v|^2 as the bit pattern of a non-negative double
+    long long xmin_key[3], xmax_key[3]; // order-preserving int64 keys of min/max coordinate
+    uint32_t plastic_count, err, warn;
+    uint32_t stop;          // nonzero: remaining kernels of this batch are no-ops
+    uint32_t need_realloc;  // the grid of the next step exceeds capacity
+    uint32_t steps_done, frame_ended, finished;
+    uint32_t stop_at_frame_end, check_conservation;
+    uint32_t cap_axis, cap_nodes;
+    uint32_t xsel;          // MUSL: which position buffer holds the current x (toggles per executed step)
+    uint32_t pad1;
+    // conservation sums (simulation.cpp:292-333), accumulated after P2G when requested
+    double cons_grid_mass, cons_grid_mom[3], cons_part_mom[3];
+    DevPlate plates[kMaxPlates];
+};
+
+}  // namespace mpm
