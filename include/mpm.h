@@ -227,6 +227,14 @@ int mpm_get_grid(mpm_context* ctx, mpm_grid_info* info, double* x, double* y, do
  * pos_upper and v_object per plate, in insertion order. */
 int mpm_get_plates(mpm_context* ctx, mpm_plate* plates, uint32_t capacity, uint32_t* count);
 
+/* Cell binning of the current particle positions in the grid frame of the last step, in the order of
+ * mpm_set_particles: cell_base[n*dim] = floor((x - origin) / dx) - 1 per axis, the first node of the
+ * 4-node window of p2g.cpp:21-24 / g2p.cpp:29-32 (before the P2G clamp at 0); block_key[n] = the
+ * engine's sort key, (bx*nby + by)*nbz + bz with b = max(cell_base, 0) / 4 per axis (3D), or
+ * bx*nby + by with blocks of 8 nodes (2D). Either pointer may be NULL. The reference has no binning;
+ * the cell index replaces the i_base/j_base/k_base expressions of the lines cited above. */
+int mpm_get_binning(mpm_context* ctx, int32_t* cell_base, uint32_t* block_key);
+
 /* Accumulated device time of one kernel of the step pipeline. */
 typedef struct mpm_kernel_timing {
     char name[32];       /* kernel name, e.g. "k_p2g" */

@@ -36,15 +36,30 @@ def build_engine():
     return ENGINE_SO
 
 
+REFTESTS_SRC = os.path.join(REPO, "tests", "cpp", "reference_tests.cpp")
+REFTESTS_BIN = os.path.join(REPO, "tests", "cpp", "reference_tests")
+
+
+def build_reference_tests():
+    """The reference's physics tests against the C++ host API (links libmpm_hip.so)."""
+    tmp = REFTESTS_BIN + ".tmp"
+    run(["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unused-function", REFTESTS_SRC, "-I", os.path.join(REPO, "include"),
+         "-L", HERE, "-lmpm_hip", "-Wl,-rpath,$ORIGIN/../../matter-tests_amd", "-o", tmp])
+    os.replace(tmp, REFTESTS_BIN)
+    return REFTESTS_BIN
+
+
 def build_oracle():
     run(["make", "-s", "-C", os.path.join(REPO, "oracle")])
     return os.path.join(REPO, "oracle", "liboracle.so")
 
 
 def main(argv):
-    what = argv[1:] or ["engine", "oracle"]
+    what = argv[1:] or ["engine", "oracle", "reftests"]
     if "engine" in what:
         build_engine()
+    if "reftests" in what:
+        build_reference_tests()
     if "oracle" in what:
         build_oracle()
 

@@ -21,11 +21,15 @@ struct mpm_context {
     hipStream_t stream = nullptr;
     uint32_t n = 0;
     bool ready = false;
-    // particles
+    // particles: two layouts (double buffer) and the counting-sort arrays
     double* pbuf = nullptr;
     size_t pbuf_n = 0;
-    PartPtrs pp{};
-    double* xbuf[2][3] = {{nullptr}};
+    uint32_t* ibuf = nullptr;   // ids (2 layouts), keys, permutation
+    size_t ibuf_n = 0;
+    PartState ps[2]{};
+    SortBufs sb{};
+    uint32_t* bbuf = nullptr;   // block histogram, offsets, cursors
+    uint32_t cap_blocks = 0;
     // grid
     double* gbuf = nullptr;
     double* cbuf = nullptr;
@@ -46,8 +50,9 @@ struct mpm_context {
 
 namespace {
 const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p_fused", "k_g2p_musl",
-                                    "k_musl_norm", "k_defgrad", "k_finish"};
-enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P_FUSED, KS_G2P_MUSL, KS_MUSL_NORM, KS_DEFGRAD, KS_FINISH, KS_COUNT };
+                                    "k_musl_norm", "k_defgrad", "k_finish", "k_bin_count", "k_bin_scan", "k_bin_scatter"};
+enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P_FUSED, KS_G2P_MUSL, KS_MUSL_NORM, KS_DEFGRAD, KS_FINISH,
+                KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_COUNT };
 
 // Records an event pair around one launch when profiling; returns the launch's slot.
 struct LaunchTimer {
@@ -116,15 +121,22 @@ void derive_params(const mpm_config& c, Params& P) {
     P.rho_s = c.rho_s; P.grain_diameter = c.grain_diameter; P.I_ref = c.I_ref; P.mu_1 = c.mu_1; P.mu_2 = c.mu_2;
 }
 
-uint32_t particle_fields(int D) { return 3 * D + D * D * 2 + (D == 3 ? 6 : 3) + 6; }  // x, xn, v, B, F, tau, scalars
+uint32_t particle_fields(int D) { return D + D + D * D * 2 + (D == 3 ? 6 : 3) + 6; }  // x, v, B, F, tau, scalars
 
 int alloc_particles(mpm_context* c, uint32_t n) {
     const int D = c->cfg.dim;
-    size_t need = (size_t)particle_fields(D) * n;
-    if (c->pbuf && c->pbuf_n >= need) return 0;
-    if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
-    if (hipMalloc(&c->pbuf, need * sizeof(double)) != hipSuccess) { c->pbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
-    c->pbuf_n = need;
+    size_t need = (size_t)2 * particle_fields(D) * n;
+    size_t ineed = (size_t)4 * n;  // id x2, key, perm
+    if (!(c->pbuf && c->pbuf_n >= need)) {
+        if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
+        if (hipMalloc(&c->pbuf, need * sizeof(double)) != hipSuccess) { c->pbuf = nullptr; c->pbuf_n = 0; return MPM_ERR_OUT_OF_MEMORY; }
+        c->pbuf_n = need;
+    }
+    if (!(c->ibuf && c->ibuf_n >= ineed)) {
+        if (c->ibuf) { (void)hipFree(c->ibuf); c->ibuf = nullptr; }
+        if (hipMalloc(&c->ibuf, ineed * sizeof(uint32_t)) != hipSuccess) { c->ibuf = nullptr; c->ibuf_n = 0; return MPM_ERR_OUT_OF_MEMORY; }
+        c->ibuf_n = ineed;
+    }
     return 0;
 }
 
@@ -132,22 +144,19 @@ void layout_particles(mpm_context* c, uint32_t n) {
     const int D = c->cfg.dim;
     double* q = c->pbuf;
     auto take = [&]() { double* r = q; q += n; return r; };
-    PartPtrs pp{};
-    for (int d = 0; d < D; ++d) c->xbuf[0][d] = take();
-    for (int d = 0; d < D; ++d) c->xbuf[1][d] = take();
-    for (int d = 0; d < D; ++d) pp.v[d] = take();
-    for (int i = 0; i < D * D; ++i) pp.B[i] = take();
-    for (int i = 0; i < D * D; ++i) pp.F[i] = take();
-    for (int i = 0; i < (D == 3 ? 6 : 3); ++i) pp.tau[i] = take();
-    pp.eps_dev = take(); pp.eps_vol = take(); pp.eps_prad = take(); pp.dgamma = take(); pp.visc = take(); pp.muI = take();
-    pp.n = n;
-    c->pp = pp;
-}
-
-PartPtrs with_x(const mpm_context* c, uint32_t xsel) {
-    PartPtrs pp = c->pp;
-    for (int d = 0; d < 3; ++d) { pp.x[d] = c->xbuf[xsel & 1][d]; pp.xn[d] = c->xbuf[(xsel + 1) & 1][d]; }
-    return pp;
+    for (int s = 0; s < 2; ++s) {
+        PartState ps{};
+        for (int d = 0; d < D; ++d) ps.x[d] = take();
+        for (int d = 0; d < D; ++d) ps.v[d] = take();
+        for (int i = 0; i < D * D; ++i) ps.B[i] = take();
+        for (int i = 0; i < D * D; ++i) ps.F[i] = take();
+        for (int i = 0; i < (D == 3 ? 6 : 3); ++i) ps.tau[i] = take();
+        ps.eps_dev = take(); ps.eps_vol = take(); ps.eps_prad = take(); ps.dgamma = take(); ps.visc = take(); ps.muI = take();
+        ps.id = c->ibuf + (size_t)s * n;
+        c->ps[s] = ps;
+    }
+    c->sb.key = c->ibuf + (size_t)2 * n;
+    c->sb.perm = c->ibuf + (size_t)3 * n;
 }
 
 int ensure_grid(mpm_context* c, uint32_t batch) {
@@ -157,6 +166,15 @@ int ensure_grid(mpm_context* c, uint32_t batch) {
         uint32_t a = c->est_n[d] + 2 * (batch + 4);
         want_axis = std::max(want_axis, a);
         want_nodes *= a;
+    }
+    uint64_t want_blocks = 1;
+    const uint32_t be = D == 3 ? BlockEdge<3>::v : BlockEdge<2>::v;
+    for (int d = 0; d < D; ++d) want_blocks *= (c->est_n[d] + 2 * (batch + 4) + be - 1) / be;
+    if (want_blocks > c->cap_blocks) {
+        if (c->bbuf) { (void)hipFree(c->bbuf); c->bbuf = nullptr; }
+        if (hipMalloc(&c->bbuf, (size_t)3 * want_blocks * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; c->cap_blocks = 0; return MPM_ERR_OUT_OF_MEMORY; }
+        c->cap_blocks = (uint32_t)want_blocks;
+        c->sb.count = c->bbuf; c->sb.offset = c->bbuf + want_blocks; c->sb.cursor = c->bbuf + 2 * want_blocks;
     }
     if (want_axis <= c->cap_axis && want_nodes <= c->cap_nodes) return 0;
     if (want_nodes > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
@@ -183,32 +201,38 @@ int ensure_grid(mpm_context* c, uint32_t batch) {
     return 0;
 }
 
-__global__ void k_control(StepState* S, uint32_t stop_at_frame_end, uint32_t check_cons, uint32_t cap_axis, uint32_t cap_nodes) {
+__global__ void k_control(StepState* S, uint32_t stop_at_frame_end, uint32_t check_cons, uint32_t cap_axis, uint32_t cap_nodes,
+                          uint32_t cap_blocks) {
     if (threadIdx.x != 0) return;
     S->stop = 0; S->need_realloc = 0; S->steps_done = 0; S->err = 0; S->warn = 0; S->frame_ended = 0;
     S->stop_at_frame_end = stop_at_frame_end; S->check_conservation = check_cons;
-    S->cap_axis = cap_axis; S->cap_nodes = cap_nodes;
+    S->cap_axis = cap_axis; S->cap_nodes = cap_nodes; S->cap_blocks = cap_blocks;
 }
 
 template <int D, int SPL>
 void enqueue_step(mpm_context* c, uint32_t xsel) {
     hipStream_t s = c->stream;
     const Params& P = c->P;
-    PartPtrs pp = with_x(c, xsel);
-    const uint32_t pblocks = (c->n + kBlock - 1) / kBlock;
+    const PartState& src = c->ps[xsel & 1];
+    const PartState& dst = c->ps[(xsel + 1) & 1];
+    const uint32_t n = c->n;
+    const uint32_t pblocks = (n + kBlock - 1) / kBlock;
     const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 4096);
     { LaunchTimer t(c, KS_SETUP); k_setup<D><<<1, 64, 0, s>>>(P, c->S, c->gp); }
-    { LaunchTimer t(c, KS_ZERO); k_zero_grid<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp); }
-    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, pp, c->gp); }
+    { LaunchTimer t(c, KS_ZERO); k_zero_grid<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp, c->sb); }
+    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, n); }
+    { LaunchTimer t(c, KS_BIN_SCAN); k_bin_scan<<<1, 1024, 0, s>>>(c->S, c->sb); }
+    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<D><<<pblocks, kBlock, 0, s>>>(c->S, src, dst, c->sb, n); }
+    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, c->gp, n); }
     { LaunchTimer t(c, KS_GRID); k_grid_update<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp); }
     if (P.use_musl) {
-        { LaunchTimer t(c, KS_G2P_MUSL); k_g2p_musl<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, pp, c->gp); }
+        { LaunchTimer t(c, KS_G2P_MUSL); k_g2p_musl<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, n); }
         { LaunchTimer t(c, KS_MUSL_NORM); k_musl_norm<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp); }
-        { LaunchTimer t(c, KS_DEFGRAD); k_defgrad<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, pp, c->gp); }
+        { LaunchTimer t(c, KS_DEFGRAD); k_defgrad<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, n); }
     } else {
-        LaunchTimer t(c, KS_G2P_FUSED); k_g2p_fused<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, pp, c->gp);
+        LaunchTimer t(c, KS_G2P_FUSED); k_g2p_fused<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, n);
     }
-    { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, c->n); }
+    { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
 }
 
 void enqueue(mpm_context* c, uint32_t xsel) {
@@ -277,6 +301,8 @@ void mpm_destroy(mpm_context* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->pbuf) (void)hipFree(c->pbuf);
+    if (c->ibuf) (void)hipFree(c->ibuf);
+    if (c->bbuf) (void)hipFree(c->bbuf);
     if (c->gbuf) (void)hipFree(c->gbuf);
     if (c->cbuf) (void)hipFree(c->cbuf);
     if (c->S) (void)hipFree(c->S);
@@ -334,21 +360,22 @@ int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in)
     // structure-of-arrays staging: one host vector per device column
     std::vector<double> col(n);
     auto up = [&](double* dst) { return hipMemcpy(dst, col.data(), sizeof(double) * n, hipMemcpyHostToDevice); };
+    const PartState& L0 = c->ps[0];
     for (int d = 0; d < D; ++d) {
         for (uint32_t p = 0; p < n; ++p) col[p] = in->x[(size_t)p * D + d];
-        HIP_TRY(up(c->xbuf[0][d]));
+        HIP_TRY(up(L0.x[d]));
         for (uint32_t p = 0; p < n; ++p) col[p] = in->v ? in->v[(size_t)p * D + d] : 0.0;
-        HIP_TRY(up(c->pp.v[d]));
+        HIP_TRY(up(L0.v[d]));
     }
     for (int i = 0; i < D * D; ++i) {
         const bool diag = (i % (D + 1)) == 0;
         for (uint32_t p = 0; p < n; ++p) col[p] = in->F ? in->F[(size_t)p * D * D + i] : (diag ? 1.0 : 0.0);
-        HIP_TRY(up(c->pp.F[i]));
+        HIP_TRY(up(L0.F[i]));
         for (uint32_t p = 0; p < n; ++p) col[p] = in->Bmat ? in->Bmat[(size_t)p * D * D + i] : 0.0;
-        HIP_TRY(up(c->pp.B[i]));
+        HIP_TRY(up(L0.B[i]));
     }
     const double* sc[6] = {in->eps_pl_dev, in->eps_pl_vol, in->eps_pl_vol_pradhana, in->delta_gamma, in->viscosity, in->muI};
-    double* dst[6] = {c->pp.eps_dev, c->pp.eps_vol, c->pp.eps_prad, c->pp.dgamma, c->pp.visc, c->pp.muI};
+    double* dst[6] = {L0.eps_dev, L0.eps_vol, L0.eps_prad, L0.dgamma, L0.visc, L0.muI};
     for (int k = 0; k < 6; ++k) {
         for (uint32_t p = 0; p < n; ++p) col[p] = sc[k] ? sc[k][p] : 0.0;
         if (k == 5 && c->cfg.use_mibf) {  // simulate() fills muI when use_mibf (simulation.cpp:138-145)
@@ -372,12 +399,13 @@ int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in)
     std::memset(&S, 0, sizeof(S));
     for (int d = 0; d < 3; ++d) { S.gravity[d] = c->P.gravity_final[d]; S.xmin_key[d] = 0x7FFFFFFFFFFFFFFFLL; S.xmax_key[d] = (-0x7FFFFFFFFFFFFFFFLL - 1); }
     for (size_t k = 0; k < c->plates.size(); ++k) S.plates[k] = c->plates[k];
+    for (int s = 0; s < kRedSlots; ++s)
+        for (int d = 0; d < 3; ++d) { S.red[s].mn[d] = 0x7FFFFFFFFFFFFFFFLL; S.red[s].mx[d] = (-0x7FFFFFFFFFFFFFFFLL - 1); }
     S.n[0] = S.n[1] = S.n[2] = 1;
     HIP_TRY(hipMemcpy(c->S, &S, sizeof(StepState), hipMemcpyHostToDevice));
     const uint32_t pblocks = (n + kBlock - 1) / kBlock;
-    PartPtrs pp = with_x(c, 0);
-    if (D == 3) k_prepare<3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, pp);
-    else k_prepare<2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, pp);
+    if (D == 3) k_prepare<3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, c->ps[0], n);
+    else k_prepare<2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, c->ps[0], n);
     HIP_TRY(hipGetLastError());
     rc = sync_state(c);
     if (rc) return rc;
@@ -398,8 +426,9 @@ int mpm_step(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* inf
         if (rc) { status |= rc; break; }
         uint32_t xsel = c->Sh->xsel;
         k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u,
-                                           (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u, c->cap_axis, (uint32_t)c->cap_nodes);
-        for (uint32_t j = 0; j < batch; ++j) enqueue(c, c->P.use_musl ? (xsel + j) : 0u);
+                                           (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u, c->cap_axis, (uint32_t)c->cap_nodes,
+                                           c->cap_blocks);
+        for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j);
         if (hipGetLastError() != hipSuccess) { status |= MPM_ERR_HIP; break; }
         rc = sync_state(c);
         if (rc) { status |= rc; break; }
@@ -434,20 +463,27 @@ int mpm_get_particles(mpm_context* c, mpm_particle_arrays* out) {
     const int D = c->cfg.dim;
     const uint32_t n = c->n;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    PartPtrs pp = with_x(c, c->Sh->xsel);
+    const PartState& L = c->ps[c->Sh->xsel & 1];
+    std::vector<uint32_t> id(n);
+    HIP_TRY(hipMemcpy(id.data(), L.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    for (uint32_t p = 0; p < n; ++p) if (id[p] >= n) return MPM_ERR_HIP;
     std::vector<double> col(n);
     auto down = [&](const double* src) { return hipMemcpy(col.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost); };
     for (int d = 0; d < D; ++d) {
-        if (out->x) { HIP_TRY(down(pp.x[d])); for (uint32_t p = 0; p < n; ++p) out->x[(size_t)p * D + d] = col[p]; }
-        if (out->v) { HIP_TRY(down(pp.v[d])); for (uint32_t p = 0; p < n; ++p) out->v[(size_t)p * D + d] = col[p]; }
+        if (out->x) { HIP_TRY(down(L.x[d])); for (uint32_t p = 0; p < n; ++p) out->x[(size_t)id[p] * D + d] = col[p]; }
+        if (out->v) { HIP_TRY(down(L.v[d])); for (uint32_t p = 0; p < n; ++p) out->v[(size_t)id[p] * D + d] = col[p]; }
     }
     for (int i = 0; i < D * D; ++i) {
-        if (out->F) { HIP_TRY(down(pp.F[i])); for (uint32_t p = 0; p < n; ++p) out->F[(size_t)p * D * D + i] = col[p]; }
-        if (out->Bmat) { HIP_TRY(down(pp.B[i])); for (uint32_t p = 0; p < n; ++p) out->Bmat[(size_t)p * D * D + i] = col[p]; }
+        if (out->F) { HIP_TRY(down(L.F[i])); for (uint32_t p = 0; p < n; ++p) out->F[(size_t)id[p] * D * D + i] = col[p]; }
+        if (out->Bmat) { HIP_TRY(down(L.B[i])); for (uint32_t p = 0; p < n; ++p) out->Bmat[(size_t)id[p] * D * D + i] = col[p]; }
     }
     double* dsts[6] = {out->eps_pl_dev, out->eps_pl_vol, out->eps_pl_vol_pradhana, out->delta_gamma, out->viscosity, out->muI};
-    const double* srcs[6] = {pp.eps_dev, pp.eps_vol, pp.eps_prad, pp.dgamma, pp.visc, pp.muI};
-    for (int k = 0; k < 6; ++k) if (dsts[k]) HIP_TRY(hipMemcpy(dsts[k], srcs[k], sizeof(double) * n, hipMemcpyDeviceToHost));
+    const double* srcs[6] = {L.eps_dev, L.eps_vol, L.eps_prad, L.dgamma, L.visc, L.muI};
+    for (int k = 0; k < 6; ++k) {
+        if (!dsts[k]) continue;
+        HIP_TRY(down(srcs[k]));
+        for (uint32_t p = 0; p < n; ++p) dsts[k][id[p]] = col[p];
+    }
     return 0;
 }
 
@@ -486,6 +522,35 @@ int mpm_get_plates(mpm_context* c, mpm_plate* out, uint32_t cap, uint32_t* count
         out[i].pos_lower = p.pos_lower; out[i].pos_upper = p.pos_upper;
         for (int d = 0; d < 3; ++d) out[i].v_object[d] = p.v_object[d];
         out[i].vmin_factor = p.vmin_factor; out[i].load_factor = p.load_factor;
+    }
+    return 0;
+}
+
+int mpm_get_binning(mpm_context* c, int32_t* cell_base, uint32_t* block_key) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c->ready || c->Sh->grid_nodes == 0) return MPM_ERR_NOT_READY;
+    const int D = c->cfg.dim;
+    const uint32_t n = c->n;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const PartState& L = c->ps[c->Sh->xsel & 1];
+    int32_t* dbase = nullptr; uint32_t* dkey = nullptr;
+    HIP_TRY(hipMalloc(&dbase, sizeof(int32_t) * (size_t)n * D));
+    if (hipMalloc(&dkey, sizeof(uint32_t) * n) != hipSuccess) { (void)hipFree(dbase); return MPM_ERR_OUT_OF_MEMORY; }
+    const uint32_t pblocks = (n + kBlock - 1) / kBlock;
+    if (D == 3) k_binning<3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, n, dbase, dkey);
+    else k_binning<2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, n, dbase, dkey);
+    std::vector<int32_t> b((size_t)n * D); std::vector<uint32_t> kk(n), id(n);
+    int rc = 0;
+    if (hipStreamSynchronize(c->stream) != hipSuccess ||
+        hipMemcpy(b.data(), dbase, sizeof(int32_t) * b.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(kk.data(), dkey, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(id.data(), L.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess) rc = MPM_ERR_HIP;
+    (void)hipFree(dbase); (void)hipFree(dkey);
+    if (rc) return rc;
+    for (uint32_t p = 0; p < n; ++p) {
+        if (id[p] >= n) return MPM_ERR_HIP;
+        if (cell_base) for (int d = 0; d < D; ++d) cell_base[(size_t)id[p] * D + d] = b[(size_t)p * D + d];
+        if (block_key) block_key[id[p]] = kk[p];
     }
     return 0;
 }

@@ -105,60 +105,85 @@ MPM_HD PlaneRot sym_jacobi(double x, double y, double z) {  // annihilates y in 
     return r;
 }
 
+// One two-sided Jacobi step on the pair (p, q); compile-time indices keep X, U, W in registers.
+template <int D, int p, int q>
+MPM_HD void jacobi_pair(M<D>& X, M<D>& U, M<D>& W, double& dmax, bool& rotated) {
+    const double thr = fmax(2.2250738585072014e-308, 2.0 * 2.220446049250313e-16 * dmax);
+    if (!(fabs(X(p, q)) > thr || fabs(X(q, p)) > thr)) return;
+    rotated = true;
+    // symmetrising rotation r1 acting on rows (p,q)
+    const double a = X(p, p), b = X(p, q), c = X(q, p), d = X(q, q);
+    const double tsum = a + d, diff = c - b;
+    PlaneRot r1{1.0, 0.0};
+    if (fabs(diff) >= 2.2250738585072014e-308) { const double u = tsum / diff; const double h = sqrt(1.0 + u * u); r1.s = 1.0 / h; r1.c = u / h; }
+    const double s00 = r1.c * a + r1.s * c, s01 = r1.c * b + r1.s * d, s11 = -r1.s * b + r1.c * d;
+    const PlaneRot jr = sym_jacobi(s00, s01, s11);
+    const PlaneRot jl{r1.c * jr.c + r1.s * jr.s, -r1.c * jr.s + r1.s * jr.c};  // jl = r1 * jr^T
+#pragma unroll
+    for (int k = 0; k < D; ++k) {  // rows p,q of X <- jl applied on the left; U <- U * jl^T
+        const double xp = X(p, k), xq = X(q, k);
+        X(p, k) = jl.c * xp + jl.s * xq; X(q, k) = -jl.s * xp + jl.c * xq;
+        const double up = U(k, p), uq = U(k, q);
+        U(k, p) = jl.c * up + jl.s * uq; U(k, q) = -jl.s * up + jl.c * uq;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {  // X <- X * jr ; W <- W * jr  (columns p,q)
+        const double xp = X(k, p), xq = X(k, q);
+        X(k, p) = jr.c * xp - jr.s * xq; X(k, q) = jr.s * xp + jr.c * xq;
+        const double wp = W(k, p), wq = W(k, q);
+        W(k, p) = jr.c * wp - jr.s * wq; W(k, q) = jr.s * wp + jr.c * wq;
+    }
+    dmax = fmax(dmax, fmax(fabs(X(p, p)), fabs(X(q, q))));
+}
+
+// Orders singular values i < j descending, swapping the columns of U and W along (branch-free selects).
+template <int D, int i, int j>
+MPM_HD void sv_order(V<D>& s, M<D>& U, M<D>& W) {
+    const bool sw = s[j] > s[i];
+    const double a = s[i], b = s[j];
+    s[i] = sw ? b : a; s[j] = sw ? a : b;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double ui = U(k, i), uj = U(k, j), wi = W(k, i), wj = W(k, j);
+        U(k, i) = sw ? uj : ui; U(k, j) = sw ? ui : uj;
+        W(k, i) = sw ? wj : wi; W(k, j) = sw ? wi : wj;
+    }
+}
+
 template <int D> MPM_HD void svd(const M<D>& A, M<D>& U, V<D>& s, M<D>& W) {
     double scale = 0.0;
+#pragma unroll
     for (int i = 0; i < D * D; ++i) scale = fmax(scale, fabs(A.m[i]));
     if (scale == 0.0) scale = 1.0;
-    double inv = 1.0 / scale;
-    M<D> X; for (int i = 0; i < D * D; ++i) X.m[i] = A.m[i] * inv;
+    const double inv = 1.0 / scale;
+    M<D> X;
+#pragma unroll
+    for (int i = 0; i < D * D; ++i) X.m[i] = A.m[i] * inv;
     set_identity(U); set_identity(W);
-    double dmax = 0.0; for (int i = 0; i < D; ++i) dmax = fmax(dmax, fabs(X(i, i)));
-    const double tol = 2.0 * 2.220446049250313e-16;
-    for (int sweep = 0; sweep < 24; ++sweep) {
+    double dmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) dmax = fmax(dmax, fabs(X(i, i)));
+    for (int sweep = 0; sweep < 24; ++sweep) {  // sweeps visit the pairs (1,0), (2,0), (2,1)
         bool rotated = false;
-        for (int p = 1; p < D; ++p) for (int q = 0; q < p; ++q) {
-            double thr = fmax(2.2250738585072014e-308, tol * dmax);
-            if (!(fabs(X(p, q)) > thr || fabs(X(q, p)) > thr)) continue;
-            rotated = true;
-            // symmetrising rotation r1 acting on rows (p,q)
-            double a = X(p, p), b = X(p, q), c = X(q, p), d = X(q, q);
-            double tsum = a + d, diff = c - b;
-            PlaneRot r1{1.0, 0.0};
-            if (fabs(diff) >= 2.2250738585072014e-308) { double u = tsum / diff; double h = sqrt(1.0 + u * u); r1.s = 1.0 / h; r1.c = u / h; }
-            double s00 = r1.c * a + r1.s * c, s01 = r1.c * b + r1.s * d, s11 = -r1.s * b + r1.c * d;
-            PlaneRot jr = sym_jacobi(s00, s01, s11);
-            // left rotation jl = r1 * jr^T
-            PlaneRot jl{r1.c * jr.c + r1.s * jr.s, -r1.c * jr.s + r1.s * jr.c};
-            for (int k = 0; k < D; ++k) {  // rows p,q of X <- jl applied on the left
-                double xp = X(p, k), xq = X(q, k);
-                X(p, k) = jl.c * xp + jl.s * xq; X(q, k) = -jl.s * xp + jl.c * xq;
-            }
-            for (int k = 0; k < D; ++k) {  // U <- U * jl^T  (columns p,q)
-                double up = U(k, p), uq = U(k, q);
-                U(k, p) = jl.c * up + jl.s * uq; U(k, q) = -jl.s * up + jl.c * uq;
-            }
-            for (int k = 0; k < D; ++k) {  // X <- X * jr ; W <- W * jr  (columns p,q)
-                double xp = X(k, p), xq = X(k, q);
-                X(k, p) = jr.c * xp - jr.s * xq; X(k, q) = jr.s * xp + jr.c * xq;
-                double wp = W(k, p), wq = W(k, q);
-                W(k, p) = jr.c * wp - jr.s * wq; W(k, q) = jr.s * wp + jr.c * wq;
-            }
-            dmax = fmax(dmax, fmax(fabs(X(p, p)), fabs(X(q, q))));
+        jacobi_pair<D, 1, 0>(X, U, W, dmax, rotated);
+        if constexpr (D == 3) {
+            jacobi_pair<D, 2, 0>(X, U, W, dmax, rotated);
+            jacobi_pair<D, 2, 1>(X, U, W, dmax, rotated);
         }
         if (!rotated) break;
     }
+#pragma unroll
     for (int i = 0; i < D; ++i) {
-        double a = X(i, i);
+        const double a = X(i, i);
         s[i] = fabs(a) * scale;
-        if (a < 0) for (int k = 0; k < D; ++k) U(k, i) = -U(k, i);
+        const double sg = a < 0 ? -1.0 : 1.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) U(k, i) = sg * U(k, i);
     }
-    for (int i = 0; i < D - 1; ++i) {  // selection sort, descending
-        int pos = i; double mx = s[i];
-        for (int k = i + 1; k < D; ++k) if (s[k] > mx) { mx = s[k]; pos = k; }
-        if (pos != i) {
-            double t = s[i]; s[i] = s[pos]; s[pos] = t;
-            for (int k = 0; k < D; ++k) { t = U(k, i); U(k, i) = U(k, pos); U(k, pos) = t; t = W(k, i); W(k, i) = W(k, pos); W(k, pos) = t; }
-        }
+    sv_order<D, 0, 1>(s, U, W);  // descending order: compare-swap network
+    if constexpr (D == 3) {
+        sv_order<D, 1, 2>(s, U, W);
+        sv_order<D, 0, 1>(s, U, W);
     }
 }
 

@@ -41,6 +41,16 @@ struct Params {
     DevObject objects[kMaxObjects];
 };
 
+// Partial reductions of the particle kernels. Each workgroup folds its result into slot
+// (blockIdx % kRedSlots) so that no single address takes one atomic per wavefront of the launch;
+// k_setup and k_finish fold the slots into StepState and reset them.
+constexpr int kRedSlots = 128;
+struct RedSlot {
+    unsigned long long v2;  // max |v|^2 bits
+    long long mn[3], mx[3]; // order-preserving keys of the coordinate bounds
+    uint32_t plastic, pad;
+};
+
 // Device-resident loop state of simulate() plus the per-step grid geometry and reductions.
 // Written by the single-workgroup set-up/finish kernels, read by every other kernel.
 struct StepState {
@@ -63,11 +73,14 @@ struct StepState {
     uint32_t steps_done, frame_ended, finished;
     uint32_t stop_at_frame_end, check_conservation;
     uint32_t cap_axis, cap_nodes;
-    uint32_t xsel;          // MUSL: which position buffer holds the current x (toggles per executed step)
-    uint32_t pad1;
+    uint32_t xsel;          // which particle-state buffer is current (toggles per executed step)
+    uint32_t nblocks;       // blocks of the counting sort (BlockEdge^dim nodes each)
+    uint32_t nb[3];         // blocks per axis
+    uint32_t cap_blocks;
     // conservation sums (simulation.cpp:292-333), accumulated after P2G when requested
     double cons_grid_mass, cons_grid_mom[3], cons_part_mom[3];
     DevPlate plates[kMaxPlates];
+    RedSlot red[kRedSlots];
 };
 
 }  // namespace mpm

@@ -179,6 +179,7 @@ class Library:
             L.mpm_status_string.restype = C.c_char_p
             L.mpm_config_defaults.argtypes = [C.POINTER(Config)]
             L.mpm_set_profiling.argtypes = [vp, C.c_int]
+            L.mpm_get_binning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]
             L.mpm_get_kernel_times.argtypes = [vp, C.POINTER(KernelTiming), C.c_uint32, C.POINTER(C.c_uint32)]
         else:
             L.oracle_create.argtypes = [C.POINTER(Config)]
@@ -328,6 +329,14 @@ class Engine:
                                           _dptr(g["flip"]), _dptr(g["friction"])), "get_grid")
         g["info"] = info
         return g
+
+    def get_binning(self):
+        """(cell_base[n, dim] int32, block_key[n] uint32) of the current positions in the last step's grid."""
+        base = np.zeros((self.n, self.dim), dtype=np.int32)
+        key = np.zeros(self.n, dtype=np.uint32)
+        self._check(self.L.lib.mpm_get_binning(self.h, base.ctypes.data_as(C.POINTER(C.c_int32)),
+                                               key.ctypes.data_as(C.POINTER(C.c_uint32))), "get_binning")
+        return base, key
 
     def set_profiling(self, enable=True):
         self._check(self.L.lib.mpm_set_profiling(self.h, 1 if enable else 0), "set_profiling")

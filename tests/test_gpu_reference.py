@@ -1,0 +1,47 @@
+"""The reference's own physics tests (tests/tests.cpp:13-736), run through the C++ host API on the GPU engine.
+
+tests/cpp/reference_tests.cpp restates each gtest case against matter-tests_amd/host/matter.hpp (which drives
+libmpm_hip.so through the C ABI) with the reference's parameters and tolerances; this module runs one case
+per process so that a failure names its case. The binary is built by __graft_entry__.build().
+"""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "reference_tests")
+
+CASES = [
+    ("BoundaryTest.AnalyticSlipFree", 120),
+    ("BoundaryTest.AnalyticSlipStick", 120),
+    ("CoulombFrictionTest.PlateSlipFree", 120),
+    ("CoulombFrictionTest.PlateSlipStick", 120),
+    ("CoulombFrictionTest.GeneralSlipFree", 120),
+    ("CoulombFrictionTest.GeneralSlipStick", 120),
+    ("BoundaryTest.MIBF", 300),
+    ("ElasticityTest.BulkModulus", 60),
+    ("EnergyTest.Rotation", 120),
+    ("CollapseTest.DruckerPragerOne", 600),
+    ("CollapseTest.DruckerPragerTwo", 600),
+]
+
+
+def test_reference_tests_binary_lists_every_case():
+    """CPU check: the binary exists and registers exactly the cases run below (no GPU call is made)."""
+    if not os.path.exists(BIN):
+        pytest.skip("tests/cpp/reference_tests is not built; run __graft_entry__.build()")
+    src = open(os.path.join(HERE, "cpp", "reference_tests.cpp")).read()
+    for name, _ in CASES:
+        suite, case = name.split(".")
+        assert f"MATTER_TEST({suite}, {case})" in src, name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,limit", [pytest.param(n, t, marks=pytest.mark.timeout(t + 30), id=n) for n, t in CASES])
+def test_reference_case(name, limit):
+    assert os.path.exists(BIN), "tests/cpp/reference_tests is not built; run __graft_entry__.build()"
+    print(f"running {name}", flush=True)
+    r = subprocess.run([BIN, name], capture_output=True, text=True, timeout=limit)
+    print(r.stdout[-4000:], r.stderr[-2000:])
+    assert r.returncode == 0, f"{name} failed (exit {r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-1000:]}"
