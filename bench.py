@@ -29,6 +29,7 @@ import mpm_abi as A  # noqa: E402
 import scenes as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published peak
+STEP_BYTES_PER_PARTICLE_UPDATE = 438  # 2*(24+24+72+72) + 40 + (2*(8+24) + 2*24)/8
 METRIC = "particle-updates/s 3D DP-APIC FP32 vs FP64 @stated err; served p50/p99 step ms"
 
 
@@ -102,6 +103,69 @@ def cpu_baseline(cfg, plates, objs, x, steps=3):
             "sample": f"{steps} steps (after 1 warm-up step) of the same {n}-particle scene, {secs:.2f} s"}
 
 
+COPY_BYTES = 256 * 1024 * 1024
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "c2_fp64_traffic.txt")
+
+
+def copy_calibration():
+    """Three device copies of a 256 MiB buffer, so that a counter run (rocprofv3 --pmc FETCH_SIZE or
+    WRITE_SIZE) records what the counter reads for a copy of known size next to the engine's kernels."""
+    import torch
+    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def summarize_pmc(fetch_csv, write_csv):
+    """Per-kernel HBM bytes per dispatch from two rocprofv3 counter-collection CSVs (FETCH_SIZE, WRITE_SIZE,
+    both KiB per dispatch), each scaled by the factor that makes the 256 MiB copy read/write 256 MiB."""
+    import csv
+    import collections
+
+    def per_kernel(path, counter):
+        agg = collections.defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                agg[r["Kernel_Name"]].append((int(r["Grid_Size"]), float(r["Counter_Value"]) * 1024.0))
+        return agg
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+
+    def copy_factor(agg, which):
+        big = [v for k, lst in agg.items() if "copyBuffer" in k for g, v in lst if v > COPY_BYTES / 4]
+        if not big:
+            raise SystemExit(f"no 256 MiB copy dispatch found in the {which} run (run it with --calibrate-copy)")
+        return COPY_BYTES / (sum(big) / len(big))
+
+    ff, fw = copy_factor(fetch, "FETCH_SIZE"), copy_factor(write, "WRITE_SIZE")
+    lines = [f"# copy calibration: FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f} (256 MiB device copy)",
+             "# kernel  dispatches  hbm_read_bytes_per_dispatch  hbm_write_bytes_per_dispatch"]
+    for k in sorted(fetch):
+        if not k.startswith("void mpm::") and not k.startswith("mpm::"):
+            continue
+        name = k.split("mpm::", 1)[1].split("<")[0].split("(")[0]
+        fv = [v for _, v in fetch[k]]
+        wv = [v for _, v in write.get(k, [])]
+        if not wv:
+            continue
+        lines.append(f"{name} {len(fv)} {ff * sum(fv) / len(fv):.0f} {fw * sum(wv) / len(wv):.0f}")
+    return "\n".join(lines)
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch (read + write) of `kernel` from the committed counter summary, or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 4 and f[0] == kernel:
+            return float(f[2]) + float(f[3])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,12 +174,20 @@ def main():
     ap.add_argument("--n-axis", type=int, default=None, help="lattice points per axis (default 160: 4,096,000)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true", help="run a 256 MiB device copy first (counter runs)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the per-kernel HBM traffic summary of two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        print(summarize_pmc(*args.summarize_pmc))
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding "
               "is a design in DESIGN.md, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        copy_calibration()
     cfg, plates, objs, x = S.dp_collapse(n_per_axis=args.n_axis, musl=True)
     n = x.shape[0]
     eng = A.Engine(A.product_library(), cfg, plates, objs)
@@ -145,9 +217,17 @@ def main():
                          "algorithmic_bytes": b, "GBps": (b / (avg_ms * 1e-3) / 1e9) if avg_ms > 0 and b else None}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
+    traffic = read_traffic(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (d["GBps"] / HBM_PEAK_GBS) if d["GBps"] else None, "traffic": None,
+                "frac": (d["GBps"] / HBM_PEAK_GBS) if d["GBps"] else None, "traffic": traffic,
+                "traffic_source": os.path.relpath(TRAFFIC_FILE, REPO) if traffic is not None else None,
                 "avg_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
+    # whole step against the compulsory traffic of one particle-update (x, v, F, B read and written, plastic
+    # scalars, grid share): 438 B in 3D FP64 APIC Drucker-Prager at 8 particles per cell
+    step_bytes = STEP_BYTES_PER_PARTICLE_UPDATE * n
+    step_gbs = step_bytes / (wall / args.steps) / 1e9
+    step_roofline = {"bytes_per_particle_update": STEP_BYTES_PER_PARTICLE_UPDATE, "achieved": step_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
     result = {
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -156,6 +236,7 @@ def main():
                    "particles": n, "dx": cfg.dx, "ppc": 8, "spline": "quadratic", "boundary": "SlipFree floor mu=0.4",
                    "grid_nodes_last_step": grid_nodes, "active_nodes_last_step": active, "precision": "fp64"},
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "kernels": kernels,
     }
     if not args.no_parity:
