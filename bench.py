@@ -33,35 +33,41 @@ STEP_BYTES_PER_PARTICLE_UPDATE = 438  # 2*(24+24+72+72) + 40 + (2*(8+24) + 2*24)
 METRIC = "particle-updates/s 3D DP-APIC FP32 vs FP64 @stated err; served p50/p99 step ms"
 
 
-def algorithmic_bytes(kernel, cfg, n_particles, active_nodes, grid_nodes):
+def algorithmic_bytes(kernel, cfg, n_particles, active_nodes, grid_nodes, blocks=None, tile_nodes=None):
     """Compulsory HBM bytes of one launch: particle state the kernel must read and write once, plus node
-    data it must read/write once. FP64 (8 B) throughout in the plain form."""
+    data it must read/write once (the staged block tiles count as node data: each tile is written by the
+    scatter and read once by the node pass). FP64 (8 B) throughout in the plain form."""
     D = cfg.dim
     f8 = 8
     apic = cfg.flip_ratio < 0
     flip = cfg.flip_ratio >= -1
     tau = 6 if D == 3 else 3
     mibf = bool(cfg.use_mibf)
+    nv = 1 + D + (D if flip else 0) + (1 if mibf else 0)
+    staged = (blocks or 0) * (tile_nodes or 0)
     if kernel == "k_p2g":
         part = D + D + (D * D if apic else 0) + tau + (1 if mibf else 0)
-        node = 1 + D + D + (1 if mibf else 0)
-        return f8 * (part * n_particles + node * active_nodes)
+        return f8 * (part * n_particles + nv * staged) + 4 * 3 * n_particles  # + perm read/write, ids
+    if kernel == "k_grid_update":
+        out = 1 + D + (D if flip else 0) + (1 if mibf else 0)
+        return f8 * (nv * staged + out * grid_nodes)
     if kernel == "k_g2p_musl":
         part_r = D + (D if flip else 0)
         part_w = D + (D * D if apic else 0) + D
-        node = D + (D if flip else 0) + D  # gather v (+flip), scatter mv2
+        return f8 * ((part_r + part_w) * n_particles + (D + (D if flip else 0)) * active_nodes)
+    if kernel == "k_musl_scatter":
+        return f8 * ((D + D + (D * D if apic else 0)) * n_particles + D * staged)
+    if kernel == "k_musl_update":
+        return f8 * (D * staged + (1 + D) * grid_nodes)
+    if kernel in ("k_defgrad", "k_g2p_usl"):
+        part_r = D + D * D + 2 + (D if kernel == "k_g2p_usl" else 0)
+        part_w = D * D + tau + 3 + (D + D + (D * D if apic else 0) if kernel == "k_g2p_usl" else 0)
+        node = D + (D if (kernel == "k_g2p_usl" and flip) else 0)
         return f8 * ((part_r + part_w) * n_particles + node * active_nodes)
-    if kernel in ("k_defgrad", "k_g2p_fused"):
-        part_r = D + D * D + 2 + (D if kernel == "k_g2p_fused" else 0)
-        part_w = D * D + tau + 3 + (D + D + (D * D if apic else 0) if kernel == "k_g2p_fused" else 0)
-        node = D + (D if (kernel == "k_g2p_fused" and flip) else 0)
-        return f8 * ((part_r + part_w) * n_particles + node * active_nodes)
-    if kernel == "k_grid_update":
-        return f8 * ((1 + D + D) + (1 + D + D)) * grid_nodes
-    if kernel == "k_zero_grid":
-        return f8 * (1 + D + D + (D if cfg.use_musl else 0)) * grid_nodes
-    if kernel == "k_musl_norm":
-        return f8 * (1 + D + D) * grid_nodes
+    if kernel == "k_bin_count":
+        return f8 * D * n_particles + 4 * n_particles
+    if kernel == "k_bin_scatter":
+        return 4 * 2 * n_particles
     return 0
 
 
@@ -209,10 +215,13 @@ def main():
     g = eng.get_grid()
     active = int(np.count_nonzero(g["mass"] > 0))
     grid_nodes = int(g["info"].grid_nodes)
+    _, keys = eng.get_binning()
+    blocks = int(np.unique(keys).size)  # non-empty blocks of the last step
+    tile_nodes = (4 + (3 if cfg.spline_degree == 2 else 4) - 1) ** 3 if cfg.dim == 3 else (8 + (3 if cfg.spline_degree == 2 else 4) - 1) ** 2
     kernels = {}
     for name, (ms, launches) in kt.items():
         avg_ms = ms / max(launches, 1)
-        b = algorithmic_bytes(name, cfg, n, active, grid_nodes)
+        b = algorithmic_bytes(name, cfg, n, active, grid_nodes, blocks, tile_nodes)
         kernels[name] = {"avg_ms": avg_ms, "launches": launches, "total_ms": ms,
                          "algorithmic_bytes": b, "GBps": (b / (avg_ms * 1e-3) / 1e9) if avg_ms > 0 and b else None}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
@@ -234,7 +243,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded jittered lattice, SplitMix64 seed 42)",
         "config": {"workload": "3D granular column collapse, Drucker-Prager 30 deg, APIC, MUSL, plain FP64",
                    "particles": n, "dx": cfg.dx, "ppc": 8, "spline": "quadratic", "boundary": "SlipFree floor mu=0.4",
-                   "grid_nodes_last_step": grid_nodes, "active_nodes_last_step": active, "precision": "fp64"},
+                   "grid_nodes_last_step": grid_nodes, "active_nodes_last_step": active, "blocks_last_step": blocks,
+                   "precision": "fp64"},
         "roofline": roofline,
         "step_roofline": step_roofline,
         "kernels": kernels,

@@ -219,7 +219,9 @@ uint32_t mpm_num_particles(const mpm_context* ctx);
 /* Downloads the dense grid of the last step in the reference's row-major node order
  * ind(i,j,k) = (i*Ny + j)*Nz + k (simulation.hpp:197-203): node coordinates
  * (x[Nx], y[Ny], z[Nz]), mass[grid_nodes], v and flip [grid_nodes*dim], friction[grid_nodes]
- * (use_mibf only). Any output pointer may be NULL. Call first with all NULL to get the sizes. */
+ * (use_mibf only). flip is formed only when something reads it (flip_ratio >= -1, or a step run with
+ * MPM_STEP_CHECK_CONSERVATION); otherwise it is returned as zeros. Any output pointer may be NULL.
+ * Call first with all NULL to get the sizes. */
 int mpm_get_grid(mpm_context* ctx, mpm_grid_info* info, double* x, double* y, double* z,
                  double* mass, double* v, double* flip, double* friction);
 
@@ -230,9 +232,11 @@ int mpm_get_plates(mpm_context* ctx, mpm_plate* plates, uint32_t capacity, uint3
 /* Cell binning of the current particle positions in the grid frame of the last step, in the order of
  * mpm_set_particles: cell_base[n*dim] = floor((x - origin) / dx) - 1 per axis, the first node of the
  * 4-node window of p2g.cpp:21-24 / g2p.cpp:29-32 (before the P2G clamp at 0); block_key[n] = the
- * engine's sort key, (bx*nby + by)*nbz + bz with b = max(cell_base, 0) / 4 per axis (3D), or
- * bx*nby + by with blocks of 8 nodes (2D). Either pointer may be NULL. The reference has no binning;
- * the cell index replaces the i_base/j_base/k_base expressions of the lines cited above. */
+ * engine's sort key, (bx*nby + by)*nbz + bz with b = w / 4 per axis (3D), or bx*nby + by with
+ * b = w / 8 (2D), where w = max(cell_base, 0) is moved up by one node for the quadratic spline when
+ * that node's weight is exactly zero (w is then the first node of the non-zero window).
+ * Either pointer may be NULL. The reference has no binning; the cell index replaces the
+ * i_base/j_base/k_base expressions of the lines cited above. */
 int mpm_get_binning(mpm_context* ctx, int32_t* cell_base, uint32_t* block_key);
 
 /* Accumulated device time of one kernel of the step pipeline. */

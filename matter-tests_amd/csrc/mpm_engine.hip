@@ -29,7 +29,11 @@ struct mpm_context {
     PartState ps[2]{};
     SortBufs sb{};
     uint32_t* bbuf = nullptr;   // block histogram, offsets, cursors
+    uint16_t* cellbuf = nullptr; // per-block dual-cell offsets
+    double* stgbuf = nullptr;   // per-block node tiles (staging of the scatter kernels)
     uint32_t cap_blocks = 0;
+    Staging stg{};
+    bool flip_formed = false;   // the last enqueued step wrote grid flip
     // grid
     double* gbuf = nullptr;
     double* cbuf = nullptr;
@@ -49,10 +53,11 @@ struct mpm_context {
 };
 
 namespace {
-const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p_fused", "k_g2p_musl",
-                                    "k_musl_norm", "k_defgrad", "k_finish", "k_bin_count", "k_bin_scan", "k_bin_scatter"};
-enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P_FUSED, KS_G2P_MUSL, KS_MUSL_NORM, KS_DEFGRAD, KS_FINISH,
-                KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_COUNT };
+const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p_usl", "k_g2p_musl",
+                                    "k_musl_scatter", "k_musl_update", "k_defgrad", "k_finish", "k_bin_count", "k_bin_scan",
+                                    "k_bin_scatter"};
+enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P_USL, KS_G2P_MUSL, KS_MUSL_SCATTER, KS_MUSL_UPDATE, KS_DEFGRAD,
+                KS_FINISH, KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_COUNT };
 
 // Records an event pair around one launch when profiling; returns the launch's slot.
 struct LaunchTimer {
@@ -121,6 +126,10 @@ void derive_params(const mpm_config& c, Params& P) {
     P.rho_s = c.rho_s; P.grain_diameter = c.grain_diameter; P.I_ref = c.I_ref; P.mu_1 = c.mu_1; P.mu_2 = c.mu_2;
 }
 
+size_t tile_nodes(int D, int spl) {
+    return D == 3 ? (spl == 2 ? (size_t)BG<3, 2>::NT : (size_t)BG<3, 3>::NT) : (spl == 2 ? (size_t)BG<2, 2>::NT : (size_t)BG<2, 3>::NT);
+}
+
 uint32_t particle_fields(int D) { return D + D + D * D * 2 + (D == 3 ? 6 : 3) + 6; }  // x, v, B, F, tau, scalars
 
 int alloc_particles(mpm_context* c, uint32_t n) {
@@ -172,9 +181,18 @@ int ensure_grid(mpm_context* c, uint32_t batch) {
     for (int d = 0; d < D; ++d) want_blocks *= (c->est_n[d] + 2 * (batch + 4) + be - 1) / be;
     if (want_blocks > c->cap_blocks) {
         if (c->bbuf) { (void)hipFree(c->bbuf); c->bbuf = nullptr; }
-        if (hipMalloc(&c->bbuf, (size_t)3 * want_blocks * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; c->cap_blocks = 0; return MPM_ERR_OUT_OF_MEMORY; }
+        if (c->cellbuf) { (void)hipFree(c->cellbuf); c->cellbuf = nullptr; }
+        if (c->stgbuf) { (void)hipFree(c->stgbuf); c->stgbuf = nullptr; }
+        c->cap_blocks = 0;
+        const size_t nt = tile_nodes(D, c->cfg.spline_degree);
+        if (want_blocks * nt > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
+        if (hipMalloc(&c->bbuf, (size_t)3 * want_blocks * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        if (hipMalloc(&c->cellbuf, (size_t)65 * want_blocks * sizeof(uint16_t)) != hipSuccess) { c->cellbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        if (hipMalloc(&c->stgbuf, (size_t)kMaxNV * nt * want_blocks * sizeof(double)) != hipSuccess) { c->stgbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
         c->cap_blocks = (uint32_t)want_blocks;
         c->sb.count = c->bbuf; c->sb.offset = c->bbuf + want_blocks; c->sb.cursor = c->bbuf + 2 * want_blocks;
+        c->sb.cell_off = c->cellbuf;
+        c->stg.base = c->stgbuf; c->stg.stride = (uint32_t)(nt * want_blocks);
     }
     if (want_axis <= c->cap_axis && want_nodes <= c->cap_nodes) return 0;
     if (want_nodes > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
@@ -182,19 +200,16 @@ int ensure_grid(mpm_context* c, uint32_t batch) {
     want_nodes = std::max(want_nodes, c->cap_nodes);
     if (c->gbuf) { (void)hipFree(c->gbuf); c->gbuf = nullptr; }
     if (c->cbuf) { (void)hipFree(c->cbuf); c->cbuf = nullptr; }
-    const int nf = 1 + 3 * D + 1 + 3 * D;  // mass, mv, f, fric, v, flip, mv2
+    const int nf = 1 + 2 * D + 1;  // mass, v, flip, fric
     if (hipMalloc(&c->gbuf, (size_t)nf * want_nodes * sizeof(double)) != hipSuccess) { c->gbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
     if (hipMalloc(&c->cbuf, (size_t)3 * want_axis * sizeof(double)) != hipSuccess) { c->cbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
     double* q = c->gbuf;
     auto take = [&]() { double* r = q; q += want_nodes; return r; };
     GridPtrs g{};
     g.mass = take();
-    for (int d = 0; d < D; ++d) g.mv[d] = take();
-    for (int d = 0; d < D; ++d) g.f[d] = take();
     g.fric = take();
     for (int d = 0; d < D; ++d) g.v[d] = take();
     for (int d = 0; d < D; ++d) g.flip[d] = take();
-    for (int d = 0; d < D; ++d) g.mv2[d] = take();
     for (int d = 0; d < 3; ++d) g.coord[d] = c->cbuf + (size_t)d * want_axis;
     c->gp = g;
     c->cap_axis = want_axis; c->cap_nodes = want_nodes;
@@ -209,38 +224,54 @@ __global__ void k_control(StepState* S, uint32_t stop_at_frame_end, uint32_t che
     S->cap_axis = cap_axis; S->cap_nodes = cap_nodes; S->cap_blocks = cap_blocks;
 }
 
+// P2G and the grid update for one choice of separate force (flip needed or conservation checked) and MIBF.
+template <int D, int SPL, bool SEP, bool MIBF>
+void enqueue_p2g(mpm_context* c, const PartState& src, const PartState& dst, uint32_t gblocks) {
+    constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF>::NV>::TB;
+    hipStream_t s = c->stream;
+    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL, SEP, MIBF><<<c->cap_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, c->gp, c->stg); }
+    { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, c->gp, c->stg); }
+}
+
 template <int D, int SPL>
-void enqueue_step(mpm_context* c, uint32_t xsel) {
+void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     hipStream_t s = c->stream;
     const Params& P = c->P;
     const PartState& src = c->ps[xsel & 1];
     const PartState& dst = c->ps[(xsel + 1) & 1];
     const uint32_t n = c->n;
     const uint32_t pblocks = (n + kBlock - 1) / kBlock;
-    const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 4096);
+    const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 8192);
+    const uint32_t zblocks = (c->cap_blocks + kBlock - 1) / kBlock;
     { LaunchTimer t(c, KS_SETUP); k_setup<D><<<1, 64, 0, s>>>(P, c->S, c->gp); }
-    { LaunchTimer t(c, KS_ZERO); k_zero_grid<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp, c->sb); }
-    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, n); }
+    { LaunchTimer t(c, KS_ZERO); k_zero_grid<<<zblocks, kBlock, 0, s>>>(c->S, c->sb); }
+    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, c->gp, n); }
     { LaunchTimer t(c, KS_BIN_SCAN); k_bin_scan<<<1, 1024, 0, s>>>(c->S, c->sb); }
-    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<D><<<pblocks, kBlock, 0, s>>>(c->S, src, dst, c->sb, n); }
-    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, c->gp, n); }
-    { LaunchTimer t(c, KS_GRID); k_grid_update<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp); }
+    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<<<pblocks, kBlock, 0, s>>>(c->S, c->sb, n); }
+    const bool sep = P.flip_ratio >= -1 || check_cons;
+    c->flip_formed = sep;
+    if (sep && P.use_mibf) enqueue_p2g<D, SPL, true, true>(c, src, dst, gblocks);
+    else if (sep) enqueue_p2g<D, SPL, true, false>(c, src, dst, gblocks);
+    else if (P.use_mibf) enqueue_p2g<D, SPL, false, true>(c, src, dst, gblocks);
+    else enqueue_p2g<D, SPL, false, false>(c, src, dst, gblocks);
     if (P.use_musl) {
-        { LaunchTimer t(c, KS_G2P_MUSL); k_g2p_musl<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, n); }
-        { LaunchTimer t(c, KS_MUSL_NORM); k_musl_norm<D><<<gblocks, kBlock, 0, s>>>(P, c->S, c->gp); }
+        constexpr int TBM = ScatterShape<D, SPL, D>::TB;
+        { LaunchTimer t(c, KS_G2P_MUSL); k_g2p_musl<D, SPL><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp); }
+        { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL><<<c->cap_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, c->stg); }
+        { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, c->gp, c->stg); }
         { LaunchTimer t(c, KS_DEFGRAD); k_defgrad<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, n); }
     } else {
-        LaunchTimer t(c, KS_G2P_FUSED); k_g2p_fused<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, n);
+        LaunchTimer t(c, KS_G2P_USL); k_g2p_usl<D, SPL><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp);
     }
     { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
 }
 
-void enqueue(mpm_context* c, uint32_t xsel) {
+void enqueue(mpm_context* c, uint32_t xsel, bool check_cons) {
     const int D = c->cfg.dim, SPL = c->cfg.spline_degree;
-    if (D == 3 && SPL == 2) enqueue_step<3, 2>(c, xsel);
-    else if (D == 3 && SPL == 3) enqueue_step<3, 3>(c, xsel);
-    else if (D == 2 && SPL == 2) enqueue_step<2, 2>(c, xsel);
-    else enqueue_step<2, 3>(c, xsel);
+    if (D == 3 && SPL == 2) enqueue_step<3, 2>(c, xsel, check_cons);
+    else if (D == 3 && SPL == 3) enqueue_step<3, 3>(c, xsel, check_cons);
+    else if (D == 2 && SPL == 2) enqueue_step<2, 2>(c, xsel, check_cons);
+    else enqueue_step<2, 3>(c, xsel, check_cons);
 }
 
 int sync_state(mpm_context* c) {
@@ -303,6 +334,8 @@ void mpm_destroy(mpm_context* c) {
     if (c->pbuf) (void)hipFree(c->pbuf);
     if (c->ibuf) (void)hipFree(c->ibuf);
     if (c->bbuf) (void)hipFree(c->bbuf);
+    if (c->cellbuf) (void)hipFree(c->cellbuf);
+    if (c->stgbuf) (void)hipFree(c->stgbuf);
     if (c->gbuf) (void)hipFree(c->gbuf);
     if (c->cbuf) (void)hipFree(c->cbuf);
     if (c->S) (void)hipFree(c->S);
@@ -428,7 +461,7 @@ int mpm_step(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* inf
         k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u,
                                            (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u, c->cap_axis, (uint32_t)c->cap_nodes,
                                            c->cap_blocks);
-        for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j);
+        for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, (flags & MPM_STEP_CHECK_CONSERVATION) != 0);
         if (hipGetLastError() != hipSuccess) { status |= MPM_ERR_HIP; break; }
         rc = sync_state(c);
         if (rc) { status |= rc; break; }
@@ -507,7 +540,8 @@ int mpm_get_grid(mpm_context* c, mpm_grid_info* info, double* x, double* y, doub
     std::vector<double> col(nn);
     for (int d = 0; d < D; ++d) {
         if (v) { HIP_TRY(hipMemcpy(col.data(), c->gp.v[d], sizeof(double) * nn, hipMemcpyDeviceToHost)); for (uint32_t l = 0; l < nn; ++l) v[(size_t)l * D + d] = col[l]; }
-        if (flip) { HIP_TRY(hipMemcpy(col.data(), c->gp.flip[d], sizeof(double) * nn, hipMemcpyDeviceToHost)); for (uint32_t l = 0; l < nn; ++l) flip[(size_t)l * D + d] = col[l]; }
+        if (flip && !c->flip_formed) { for (uint32_t l = 0; l < nn; ++l) flip[(size_t)l * D + d] = 0.0; }
+        else if (flip) { HIP_TRY(hipMemcpy(col.data(), c->gp.flip[d], sizeof(double) * nn, hipMemcpyDeviceToHost)); for (uint32_t l = 0; l < nn; ++l) flip[(size_t)l * D + d] = col[l]; }
     }
     return 0;
 }
@@ -537,8 +571,11 @@ int mpm_get_binning(mpm_context* c, int32_t* cell_base, uint32_t* block_key) {
     HIP_TRY(hipMalloc(&dbase, sizeof(int32_t) * (size_t)n * D));
     if (hipMalloc(&dkey, sizeof(uint32_t) * n) != hipSuccess) { (void)hipFree(dbase); return MPM_ERR_OUT_OF_MEMORY; }
     const uint32_t pblocks = (n + kBlock - 1) / kBlock;
-    if (D == 3) k_binning<3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, n, dbase, dkey);
-    else k_binning<2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, n, dbase, dkey);
+    const int spl = c->cfg.spline_degree;
+    if (D == 3 && spl == 2) k_binning<3, 2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, c->gp, n, dbase, dkey);
+    else if (D == 3) k_binning<3, 3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, c->gp, n, dbase, dkey);
+    else if (spl == 2) k_binning<2, 2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, c->gp, n, dbase, dkey);
+    else k_binning<2, 3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, c->gp, n, dbase, dkey);
     std::vector<int32_t> b((size_t)n * D); std::vector<uint32_t> kk(n), id(n);
     int rc = 0;
     if (hipStreamSynchronize(c->stream) != hipSuccess ||

@@ -1,28 +1,38 @@
 // mpm_kernels.hpp — gfx950 kernels of one MPM step (plain FP64 form).
 //
+// Particles are binned every step into blocks of E^dim dual cells (E = 4 in 3D, 8 in 2D). The dual cell of
+// a particle is the first node of its non-zero B-spline window per axis, so every particle of a block
+// touches only the block's tile of (E + W - 1)^dim nodes (W = 3 non-zero nodes per axis for the quadratic
+// spline, 4 for the cubic). All particle and grid kernels work one block per workgroup.
+//
 // Step pipeline (one HIP stream, no host synchronisation inside a batch of steps):
-//   k_setup        1 workgroup: dt (updateDt), grid geometry (remeshFixedInit/Cont + arange), capacity check
-//   k_zero_grid    clear the node accumulators of the dense grid (resizeGrid) and the block histogram
-//   k_bin_count    block key of every particle (4x4x4-node blocks in 3D, 8x8 in 2D) + histogram
+//   k_setup        1 wavefront: dt (updateDt), grid geometry (remeshFixedInit/Cont + arange), capacity check
+//   k_zero_grid    clear the block histogram
+//   k_bin_count    block of every particle + histogram (wavefront-aggregated atomics)
 //   k_bin_scan     exclusive prefix sum of the histogram (one workgroup)
-//   k_bin_scatter  counting-sort permutation: sorted position -> particle of the previous layout
-//   k_p2g          scatter mass, APIC momentum, MIBF friction and stress force of 256 sorted particles
-//                  into an LDS tile over their stencils' bounding box, then flush the tile with one
-//                  global atomic per non-zero node value (P2G + force part of explicitEulerUpdate)
-//   k_grid_update  normalise, explicit update, boundary conditions, flip       (P2G tail + grid loop of
-//                  explicitEulerUpdate + boundaryCollision)
-//   USL:  k_g2p_fused    gather, F update, return map, position/velocity update  (G2P + deformationUpdate
-//                        + plasticity + positionUpdate), reductions for the next step
-//   MUSL: k_g2p_musl     gather, velocity update, new position, LDS-tiled momentum re-scatter (G2P + MUSL)
-//         k_musl_norm    divide by m_i/m_p                                         (MUSL tail)
-//         k_defgrad      gather grad v, F update, return map                       (deformationUpdate)
-//   k_finish       1 thread: moveObjects, time/frame bookkeeping of simulate()
+//   k_bin_scatter  counting-sort permutation: sorted slot -> particle of the previous layout
+//   k_p2g          per block: order the particles by (dual cell, previous slot); one thread per
+//                  (cell, first-axis window offset) sums its cell's contributions in registers; the
+//                  partial sums are added node by node in a fixed order into the block's tile, which is
+//                  written to a staging area (P2G + the stress force of explicitEulerUpdate)
+//   k_grid_update  per node: sum the tiles of the (at most 2^dim) blocks that cover it in block order,
+//                  normalise, explicit update, boundary conditions, flip
+//   USL:  k_g2p_usl       gather from an LDS copy of the block's tile; F update, return map, position and
+//                         velocity update (G2P + deformationUpdate + plasticity + positionUpdate)
+//   MUSL: k_g2p_musl      gather, velocity update, new position (G2P + positionUpdate)
+//         k_musl_scatter  re-scatter of the new momentum, like k_p2g (MUSL)
+//         k_musl_update   per node: sum the tiles, divide by m_i/m_p (MUSL tail)
+//         k_defgrad       gather grad v, F update, return map (deformationUpdate + plasticity)
+//   k_finish       1 wavefront: moveObjects, time/frame bookkeeping of simulate()
+//
+// No floating-point atomics are used: every sum has a fixed order given the particle state, so a step is
+// deterministic and two runs with equal inputs give equal bits. (A block with more than kMaxBP particles
+// takes a slower path that accumulates its tile with LDS atomics; its sums are then not ordered.)
 //
 // Particle state is double-buffered. Every step reads the previous layout through the sort permutation
-// and writes all fields in sorted order into the other buffer, so particles stay spatially sorted at no
-// extra traffic; each particle carries its original index (id) for downloads.
-// Every kernel returns at once when StepState::stop is set, so the host can enqueue many steps and
-// let the device stop at an error, a frame end, the final time or a capacity overflow.
+// and writes all fields in sorted order into the other buffer; each particle carries its original index
+// (id) for downloads. Every kernel returns at once when StepState::stop is set, so the host can enqueue
+// many steps and let the device stop at an error, a frame end, the final time or a capacity overflow.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "mpm_math.hpp"
@@ -42,28 +52,45 @@ struct PartState {
 
 // Counting-sort work arrays.
 struct SortBufs {
-    uint32_t* key;     // block key per particle of the source layout
-    uint32_t* perm;    // sorted position -> particle of the source layout
-    uint32_t* count;   // particles per block
-    uint32_t* offset;  // exclusive prefix sum of count
-    uint32_t* cursor;  // scatter cursors
+    uint32_t* key;       // block of every particle of the source layout
+    uint32_t* perm;      // sorted slot -> particle of the source layout
+    uint32_t* count;     // particles per block
+    uint32_t* offset;    // exclusive prefix sum of count
+    uint32_t* cursor;    // scatter cursors
+    uint16_t* cell_off;  // per block: first sorted slot (relative to offset) of each dual cell, 65 entries
 };
 
 struct GridPtrs {
-    double* mass;    // sum of weights during P2G, m_i afterwards
-    double* mv[3];   // momentum (sum of w (v + B d D^-1))
-    double* f[3];    // sum of tau grad w
-    double* fric;    // MIBF: sum of w muI, then divided by sum of w
-    double* v[3];    // node velocity after the update (MUSL: after re-scatter)
-    double* flip[3];
-    double* mv2[3];  // MUSL re-scatter accumulator
+    double* mass;    // m_i
+    double* v[3];    // node velocity after the update (MUSL: after the re-scatter)
+    double* flip[3]; // v_new - v_old (formed when flip_ratio >= -1 or conservation is checked)
+    double* fric;    // MIBF friction coefficient
     double* coord[3];
 };
 
+// Per-block node tiles written by the scatter kernels: value v of tile node q of block b is at
+// base[v * stride + b * NT + q].
+struct Staging {
+    double* base;
+    uint32_t stride;
+};
+
 constexpr int kBlock = 256;
-constexpr int kTileCap = 576;  // nodes of one LDS tile (7 x 7 x 11 covers a chunk straddling two 3D blocks)
+constexpr int kMaxBP = 2048;   // particles of a block ordered in LDS; larger blocks take the LDS-atomic path
+constexpr int kMaxNV = 8;      // values per tile node (mass, 3 momentum, 3 force, friction)
 
 template <int D> struct BlockEdge { static constexpr int v = D == 3 ? 4 : 8; };
+
+// Block geometry.
+template <int D, int SPL> struct BG {
+    static constexpr int E = BlockEdge<D>::v;              // dual cells per axis
+    static constexpr int W = SPL == 2 ? 3 : 4;              // non-zero window nodes per axis
+    static constexpr int T = E + W - 1;                     // tile nodes per axis
+    static constexpr int NC = D == 3 ? E * E * E : E * E;   // dual cells per block (64)
+    static constexpr int NT = D == 3 ? T * T * T : T * T;   // tile nodes
+    static constexpr int NS = D == 3 ? W * W : W;           // window nodes with a fixed first-axis offset
+};
+
 
 // order-preserving int64 key of a double (for exact atomic min/max)
 __device__ __forceinline__ long long ord_key(double d) { long long b = __double_as_longlong(d); return b >= 0 ? b : (b ^ 0x7FFFFFFFFFFFFFFFLL); }
@@ -124,22 +151,29 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t* ctr, uint32_t key, bool
     return slot;
 }
 
-// Per-step reductions of the new particle state (max speed for updateDt, bounds for remesh).
-// Inactive lanes pass neutral values.
-template <int D>
-__device__ __forceinline__ void reduce_step_bounds(StepState* S, bool active, const double* x, const double* v) {
+// Per-thread accumulation of the step reductions (max speed for updateDt, bounds for remesh) over the
+// particles a thread handles.
+struct BoundsAcc {
     unsigned long long v2 = 0ull;
-    long long mn[3], mx[3];
-    for (int d = 0; d < 3; ++d) { mn[d] = 0x7FFFFFFFFFFFFFFFLL; mx[d] = (-0x7FFFFFFFFFFFFFFFLL - 1); }
-    if (active) {
-        double s = v[0] * v[0]; for (int d = 1; d < D; ++d) s += v[d] * v[d];
-        v2 = (unsigned long long)__double_as_longlong(s);
-        for (int d = 0; d < D; ++d) { mn[d] = ord_key(x[d]); mx[d] = ord_key(x[d]); }
+    long long mn[3] = {0x7FFFFFFFFFFFFFFFLL, 0x7FFFFFFFFFFFFFFFLL, 0x7FFFFFFFFFFFFFFFLL};
+    long long mx[3] = {(-0x7FFFFFFFFFFFFFFFLL - 1), (-0x7FFFFFFFFFFFFFFFLL - 1), (-0x7FFFFFFFFFFFFFFFLL - 1)};
+    template <int D> __device__ void add(const double* x, const double* v) {
+        double s = v[0] * v[0];
+        for (int d = 1; d < D; ++d) s += v[d] * v[d];
+        const unsigned long long b = (unsigned long long)__double_as_longlong(s);
+        v2 = b > v2 ? b : v2;
+        for (int d = 0; d < D; ++d) { const long long k = ord_key(x[d]); mn[d] = k < mn[d] ? k : mn[d]; mx[d] = k > mx[d] ? k : mx[d]; }
     }
-    v2 = wave_max_u64(v2);
+};
+
+// Workgroup reduction of BoundsAcc (all threads call), then one set of atomics per workgroup into its slot.
+template <int D>
+__device__ __forceinline__ void reduce_bounds(StepState* S, const BoundsAcc& ba) {
+    unsigned long long v2 = wave_max_u64(ba.v2);
+    long long mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) { mn[d] = ba.mn[d]; mx[d] = ba.mx[d]; }
     for (int d = 0; d < D; ++d) { mn[d] = wave_min_i64(mn[d]); mx[d] = wave_max_i64(mx[d]); }
-    // workgroup combine through LDS, then one set of atomics per workgroup into its slot
-    __shared__ long long s_red[kBlock / 64][7];
+    __shared__ long long s_red[1024 / 64][7];
     const int wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     if ((threadIdx.x & 63) == 0) {
         s_red[wid][0] = (long long)v2;
@@ -250,59 +284,6 @@ __device__ __forceinline__ uint32_t node_index(int i, int j, int k, const uint32
     return D == 3 ? node_index3(i, j, k, n) : (uint32_t)i * n[1] + (uint32_t)j;
 }
 
-// LDS tile over the node bounding box of the 4-node windows of a workgroup's particles.
-struct Tile {
-    int lo[3], ext[3], vol;
-    bool fits;
-};
-
-// Workgroup-wide bounding box of the windows base..base+W-1 of the participating lanes (all threads call).
-template <int D, int W>
-__device__ __forceinline__ Tile tile_bbox(bool part, const int* base, int* s_box) {
-    int mn[3], mx[3];
-    for (int d = 0; d < 3; ++d) { mn[d] = part && d < D ? base[d] : 0x7FFFFFFF; mx[d] = part && d < D ? base[d] + W - 1 : -0x7FFFFFFF; }
-    if (threadIdx.x < 6) s_box[threadIdx.x] = threadIdx.x < 3 ? 0x7FFFFFFF : -0x7FFFFFFF;
-    __syncthreads();
-    for (int d = 0; d < D; ++d) { mn[d] = wave_min_i32(mn[d]); mx[d] = wave_max_i32(mx[d]); }
-    if ((threadIdx.x & 63) == 0) for (int d = 0; d < D; ++d) { atomicMin(&s_box[d], mn[d]); atomicMax(&s_box[3 + d], mx[d]); }
-    __syncthreads();
-    Tile t;
-    t.vol = 1;
-    bool empty = false;
-    for (int d = 0; d < 3; ++d) {
-        if (d < D) {
-            t.lo[d] = s_box[d];
-            empty = empty || s_box[3 + d] < s_box[d];  // no participating lane
-            t.ext[d] = empty ? 1 : s_box[3 + d] - s_box[d] + 1;
-        } else { t.lo[d] = 0; t.ext[d] = 1; }
-    }
-    if (empty) { t.vol = 0; t.fits = true; for (int d = 0; d < 3; ++d) { t.lo[d] = 0; t.ext[d] = 1; } return t; }
-    long long vol = 1;
-    for (int d = 0; d < D; ++d) vol *= t.ext[d];
-    t.fits = vol <= kTileCap;
-    t.vol = t.fits ? (int)vol : 0;
-    return t;
-}
-
-template <int D>
-__device__ __forceinline__ int tile_local(const Tile& t, int i, int j, int k) {
-    return D == 3 ? ((i - t.lo[0]) * t.ext[1] + (j - t.lo[1])) * t.ext[2] + (k - t.lo[2]) : (i - t.lo[0]) * t.ext[1] + (j - t.lo[1]);
-}
-
-// Adds the non-zero tile values to the grid (one global atomic each); all threads call.
-template <int D, int NV>
-__device__ __forceinline__ void tile_flush(const Tile& t, const double* tile, double* const* dst, const uint32_t* n) {
-    for (int q = threadIdx.x; q < t.vol; q += blockDim.x) {
-        int i, j, k;
-        if (D == 3) { k = q % t.ext[2]; int r = q / t.ext[2]; j = r % t.ext[1]; i = r / t.ext[1]; }
-        else { j = q % t.ext[1]; i = q / t.ext[1]; k = 0; }
-        const uint32_t node = node_index<D>(t.lo[0] + i, t.lo[1] + j, t.lo[2] + k, n);
-        for (int v = 0; v < NV; ++v) {
-            const double a = tile[v * kTileCap + q];
-            if (a != 0.0) atomic_add_f64(dst[v] + node, a);
-        }
-    }
-}
 
 // ----------------------------------------------------------------------------------------------
 // k_prepare: Kirchhoff stress of the initial F and the reductions that the first step needs.
@@ -326,7 +307,9 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, Part
         for (int d = 0; d < D; ++d) { x[d] = ps.x[d][p]; v[d] = ps.v[d][p]; }
         ps.id[p] = p;
     }
-    reduce_step_bounds<D>(S, active, x, v);
+    BoundsAcc ba;
+    if (active) ba.add<D>(x, v);
+    reduce_bounds<D>(S, ba);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -435,38 +418,42 @@ __global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrs g
     }
 }
 
+
 // ----------------------------------------------------------------------------------------------
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_zero_grid(Params P, const StepState* S, GridPtrs g, SortBufs sb) {
+// k_zero_grid: clears the block histogram (the grid arrays are written whole by k_grid_update).
+// ----------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_zero_grid(const StepState* S, SortBufs sb) {
     if (S->stop) return;
-    const uint32_t nn = S->grid_nodes;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        g.mass[i] = 0.0;
-        for (int d = 0; d < D; ++d) { g.mv[d][i] = 0.0; g.f[d][i] = 0.0; }
-        if (P.use_mibf) g.fric[i] = 0.0;
-        if (P.use_musl) for (int d = 0; d < D; ++d) g.mv2[d][i] = 0.0;
-    }
     const uint32_t nb = S->nblocks;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) sb.count[i] = 0;
 }
 
-// Block key of a particle: the P2G window base (clamped at 0) divided by the block edge, clamped into
-// the block grid (particles whose window leaves the grid are reported by the scatter/gather kernels).
+// Block coordinates of linear block b.
 template <int D>
-__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, double h1) {
+__device__ __forceinline__ void block_coords(uint32_t b, const StepState* S, int* bc) {
+    if (D == 3) { bc[2] = (int)(b % S->nb[2]); const uint32_t r = b / S->nb[2]; bc[1] = (int)(r % S->nb[1]); bc[0] = (int)(r / S->nb[1]); }
+    else { bc[1] = (int)(b % S->nb[1]); bc[0] = (int)(b / S->nb[1]); bc[2] = 0; }
+}
+
+// Block of a particle from the first node of its non-zero window (P2G's clamped base). A particle whose
+// window leaves the grid goes to block 0, where k_p2g reports it.
+template <int D, int SPL>
+__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrs& g, double h1) {
+    const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
+    const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
+    Stencil<D, SPL> st;
+    if (!make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, true)) return 0u;
     uint32_t c[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) {
-        int v = (int)floor((xp[d] - S->origin[d]) * h1) - 1;
-        if (v < 0) v = 0;
-        uint32_t b = (uint32_t)v / BlockEdge<D>::v;
+        const uint32_t b = (uint32_t)st.base[d] / BlockEdge<D>::v;
         c[d] = b < S->nb[d] ? b : S->nb[d] - 1;
     }
     return D == 3 ? (c[0] * S->nb[1] + c[1]) * S->nb[2] + c[2] : c[0] * S->nb[1] + c[1];
 }
 
-// k_bin_count: block key and histogram (wavefront-aggregated atomics).
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState* S, PartState src, SortBufs sb, uint32_t n) {
+// k_bin_count: block of every particle and the histogram (wavefront-aggregated atomics).
+template <int D, int SPL>
+__global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState* S, PartState src, SortBufs sb, GridPtrs g, uint32_t n) {
     if (S->stop) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = p < n;
@@ -474,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState*
     if (active) {
         double xp[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
-        key = block_key<D>(xp, S, P.one_over_dx);
+        key = block_key<D, SPL>(xp, S, g, P.one_over_dx);
         sb.key[p] = key;
     }
     (void)wave_claim(sb.count, key, active);
@@ -493,7 +480,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const StepState* S, SortBufs 
     part[threadIdx.x] = s;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan of the run sums
-        uint32_t t = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+        const uint32_t t = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
         __syncthreads();
         part[threadIdx.x] += t;
         __syncthreads();
@@ -502,163 +489,507 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const StepState* S, SortBufs 
     for (uint32_t b = b0; b < b1; ++b) { sb.offset[b] = run; sb.cursor[b] = run; run += sb.count[b]; }
 }
 
-// k_bin_scatter: counting-sort permutation; also carries the particle ids into the new layout.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, PartState src, PartState dst, SortBufs sb, uint32_t n) {
+// k_bin_scatter: counting-sort permutation (the order inside a block is fixed later by k_p2g).
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, SortBufs sb, uint32_t n) {
     if (S->stop) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = p < n;
     const uint32_t key = active ? sb.key[p] : 0u;
     const uint32_t pos = wave_claim(sb.cursor, key, active);
-    if (active) { sb.perm[pos] = p; dst.id[pos] = src.id[p]; }
+    if (active) sb.perm[pos] = p;
 }
 
-// k_binning: cell base index and block key of every particle of a layout (diagnostic download).
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_binning(Params P, const StepState* S, PartState ps, uint32_t n, int32_t* base_out,
-                                                     uint32_t* key_out) {
+// k_binning: cell base index and block of every particle of a layout (diagnostic download).
+template <int D, int SPL>
+__global__ __launch_bounds__(kBlock) void k_binning(Params P, const StepState* S, PartState ps, GridPtrs g, uint32_t n,
+                                                     int32_t* base_out, uint32_t* key_out) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     double xp[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) xp[d] = ps.x[d][p];
     for (int d = 0; d < D; ++d) base_out[(size_t)p * D + d] = (int)floor((xp[d] - S->origin[d]) * P.one_over_dx) - 1;
-    key_out[p] = block_key<D>(xp, S, P.one_over_dx);
+    key_out[p] = block_key<D, SPL>(xp, S, g, P.one_over_dx);
 }
 
 // ----------------------------------------------------------------------------------------------
-// k_p2g: p2g.cpp:12-66 (mass, APIC momentum, MIBF friction; weight > 1e-25) fused with the stress
-// part of explicit_euler_update.cpp:15-168 (tau grad w). The reference adds the force only where
-// m_i > 0; nodes with m_i = 0 are never updated or read, so the force is scattered unconditionally.
-// One workgroup = 256 consecutive sorted particles, accumulated in an LDS tile over their windows'
-// bounding box and flushed with global atomics; a chunk whose box exceeds the tile uses global atomics.
+// Block scatter (k_p2g, k_musl_scatter). One workgroup per block. Worker thread (cell, slice, half) owns
+// the window nodes of its dual cell with first-axis offset `slice` (and, when the partial sums would not
+// fit in registers, half of the second-axis offsets); it adds the contributions of its cell's particles
+// in their sorted order. The partial sums then enter each tile node in the fixed order (a, b, c) of the
+// window offsets, and the tile goes to the staging area. Nothing depends on the timing of threads.
 // ----------------------------------------------------------------------------------------------
+template <int D, int SPL, int NV> struct ScatterShape {
+    using G = BG<D, SPL>;
+    static constexpr int SPLIT = (G::NS * NV > 72) ? 2 : 1;   // halves of the second-axis offsets
+    static constexpr int OBN = G::W / SPLIT;                   // second-axis offsets per worker
+    static constexpr int NSH = D == 3 ? OBN * G::W : OBN;     // window nodes per worker
+    static constexpr int WORKERS = G::NC * G::W * SPLIT;
+    static constexpr int TB = WORKERS > 256 ? 512 : 256;       // workgroup size
+};
+
+template <int D, int SPL, int NV> struct ScatterLDS {
+    using G = BG<D, SPL>;
+    uint32_t p[kMaxBP];              // source slots of the block's particles, ordered by (cell, slot)
+    uint32_t cstart[G::NC + 2];      // bucket c = cell c occupies p[cstart[c] .. cstart[c+1]); bucket NC: outside
+    union {
+        struct { uint32_t tmp[kMaxBP]; uint32_t ccount[G::NC + 1]; uint32_t ccur[G::NC + 1]; uint8_t cell[kMaxBP]; } sort;
+        double part[G::NS * NV * G::NC];  // partial sums of one slice: [window offset][value][cell]
+        double tile[G::NT * NV];          // blocks above kMaxBP particles: [value][node], LDS atomics
+    } u;
+};
+
+// Tile-local index of tile node (t0, t1, t2).
 template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_p2g(Params P, StepState* S, PartState src, SortBufs sb, GridPtrs g, uint32_t n) {
-    if (S->stop) return;
-    __shared__ double tile[8 * kTileCap];
-    __shared__ int s_box[6];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = i < n;
+__device__ __forceinline__ int tile_index(int t0, int t1, int t2) {
+    using G = BG<D, SPL>;
+    return D == 3 ? (t0 * G::T + t1) * G::T + t2 : t0 * G::T + t1;
+}
+
+// Orders a block's particles by (dual cell, source slot) into L.p / L.cstart and writes the sorted
+// permutation, the destination ids and the cell offsets. Particles whose window is outside the block
+// (their window leaves the grid) go to the last bucket and are reported.
+template <int D, int SPL, int NV>
+__device__ uint32_t order_block(ScatterLDS<D, SPL, NV>& L, const Params& P, const StepState* S, const PartState& src,
+                                const PartState& dst, const SortBufs& sb, const GridPtrs& g, uint32_t b, uint32_t start,
+                                uint32_t cnt, const int* bc) {
+    using G = BG<D, SPL>;
+    const int tid = threadIdx.x;
+    for (int c = tid; c <= G::NC; c += blockDim.x) L.u.sort.ccount[c] = 0;
+    __syncthreads();
     const double h1 = P.one_over_dx;
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
-    double xp[3] = {0, 0, 0}, vp[3] = {0, 0, 0};
-    Stencil<D, SPL> st;
-    bool ok = false;
-    uint32_t p = 0;
-    if (active) {
-        p = sb.perm[i];
-        for (int d = 0; d < D; ++d) { xp[d] = src.x[d][p]; vp[d] = src.v[d][p]; }
-        ok = make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, true);
-        if (!ok) atomicOr(&S->err, MPM_ERR_GRID_BOUNDS);
+    uint32_t err = 0;
+    for (uint32_t k = tid; k < cnt; k += blockDim.x) {
+        const uint32_t p = sb.perm[start + k];
+        double xp[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
+        Stencil<D, SPL> st;
+        int cell = G::NC;
+        if (make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, true)) {
+            int l[3] = {0, 0, 0};
+            bool in = true;
+            for (int d = 0; d < D; ++d) { l[d] = st.base[d] - G::E * bc[d]; in = in && l[d] >= 0 && l[d] < G::E; }
+            if (in) cell = D == 3 ? (l[0] * G::E + l[1]) * G::E + l[2] : l[0] * G::E + l[1];
+        }
+        if (cell == G::NC) err |= MPM_ERR_GRID_BOUNDS;
+        L.u.sort.tmp[k] = p;
+        L.u.sort.cell[k] = (uint8_t)cell;
+        atomicAdd(&L.u.sort.ccount[cell], 1u);
     }
-    const Tile t = tile_bbox<D, Stencil<D, SPL>::cnt>(ok, st.base, s_box);
-    const int nv = P.use_mibf ? 8 : 7;
-    if (t.fits) {
-        for (int q = threadIdx.x; q < nv * kTileCap; q += blockDim.x) tile[q] = 0.0;
+    __syncthreads();
+    if (tid < 64) {  // exclusive scan of the 64 cell buckets by one wavefront; the outside bucket goes last
+        const uint32_t v = L.u.sort.ccount[tid];
+        uint32_t incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if (tid >= o) incl += t;
+        }
+        L.cstart[tid + 1] = incl;
+        L.u.sort.ccur[tid] = incl - v;
+        if (tid == 0) L.cstart[0] = 0;
+        if (tid == 63) { L.u.sort.ccur[64] = incl; L.cstart[65] = incl + L.u.sort.ccount[64]; }
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < cnt; k += blockDim.x) {
+        const uint32_t slot = atomicAdd(&L.u.sort.ccur[L.u.sort.cell[k]], 1u);
+        L.p[slot] = L.u.sort.tmp[k];
+    }
+    __syncthreads();
+    if (tid <= G::NC) {  // insertion sort of each bucket by source slot
+        const uint32_t s0 = L.cstart[tid], s1 = L.cstart[tid + 1];
+        for (uint32_t i = s0 + 1; i < s1; ++i) {
+            const uint32_t key = L.p[i];
+            uint32_t j = i;
+            while (j > s0 && L.p[j - 1] > key) { L.p[j] = L.p[j - 1]; --j; }
+            L.p[j] = key;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < cnt; k += blockDim.x) {
+        const uint32_t p = L.p[k];
+        sb.perm[start + k] = p;
+        dst.id[start + k] = src.id[p];
+    }
+    for (int c = tid; c <= G::NC; c += blockDim.x) sb.cell_off[(size_t)b * (G::NC + 1) + c] = (uint16_t)L.cstart[c];
+    return err;
+}
+
+// Ordered accumulation (see above) of the block's particles into its tile, written to staging.
+// contrib.load(p, i, part) fetches source particle p (sorted slot i); contrib.add(part, a, half, acc, base)
+// adds it to acc for window offsets (a, half) and returns false if its window leaves the grid.
+template <int D, int SPL, int NV, class Contrib>
+__device__ void scatter_block(ScatterLDS<D, SPL, NV>& L, const Contrib& contrib, const Staging& stg, uint32_t b,
+                              uint32_t start) {
+    using G = BG<D, SPL>;
+    using SS = ScatterShape<D, SPL, NV>;
+    const int tid = threadIdx.x;
+    double acc[SS::NSH][NV];
+#pragma unroll
+    for (int o = 0; o < SS::NSH; ++o)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[o][v] = 0.0;
+    const bool worker = tid < SS::WORKERS;
+    const int cell = tid % G::NC, r = tid / G::NC;
+    const int slice = r % G::W, half = r / G::W;
+    if (worker) {
+        const uint32_t k0 = L.cstart[cell], k1 = L.cstart[cell + 1];
+        int base[3];
+        typename Contrib::Part cur, nxt;
+        if (k0 < k1) contrib.load(L.p[k0], start + k0, cur);
+        for (uint32_t k = k0; k < k1; ++k) {
+            if (k + 1 < k1) contrib.load(L.p[k + 1], start + k + 1, nxt);  // in flight while cur is added
+            (void)contrib.add(cur, slice, half, acc, base);
+            cur = nxt;
+        }
+    }
+    constexpr int QN = (G::NT + SS::TB - 1) / SS::TB;
+    double tacc[QN][NV];
+#pragma unroll
+    for (int qq = 0; qq < QN; ++qq)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tacc[qq][v] = 0.0;
+    for (int a = 0; a < G::W; ++a) {
         __syncthreads();
+        if (worker && slice == a) {
+#pragma unroll
+            for (int o = 0; o < SS::NSH; ++o)
+#pragma unroll
+                for (int v = 0; v < NV; ++v) L.u.part[((half * SS::NSH + o) * NV + v) * G::NC + cell] = acc[o][v];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int qq = 0; qq < QN; ++qq) {
+            const int q = tid + qq * SS::TB;
+            if (q >= G::NT) continue;
+            int t0, t1, t2 = 0;
+            if (D == 3) { t2 = q % G::T; const int rr = q / G::T; t1 = rr % G::T; t0 = rr / G::T; }
+            else { t1 = q % G::T; t0 = q / G::T; }
+            const int c0 = t0 - a;
+            if (c0 < 0 || c0 >= G::E) continue;
+            for (int ob = 0; ob < G::W; ++ob) {
+                const int c1 = t1 - ob;
+                if (c1 < 0 || c1 >= G::E) continue;
+                if (D == 3) {
+                    for (int oc = 0; oc < G::W; ++oc) {
+                        const int c2 = t2 - oc;
+                        if (c2 < 0 || c2 >= G::E) continue;
+                        const int cq = (c0 * G::E + c1) * G::E + c2, o = ob * G::W + oc;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) tacc[qq][v] += L.u.part[(o * NV + v) * G::NC + cq];
+                    }
+                } else {
+                    const int cq = c0 * G::E + c1, o = ob;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) tacc[qq][v] += L.u.part[(o * NV + v) * G::NC + cq];
+                }
+            }
+        }
     }
-    if (ok) {
-        const bool apic = P.flip_ratio < 0;
-        double Bm[9];
-        if (apic) for (int k = 0; k < D * D; ++k) Bm[k] = src.B[k][p];
-        double t6[6]; for (int k = 0; k < SymPack<D>::n; ++k) t6[k] = src.tau[k][p];
-        double tau[3][3];
-        if (D == 3) { tau[0][0] = t6[0]; tau[1][1] = t6[1]; tau[2][2] = t6[2]; tau[0][1] = tau[1][0] = t6[3]; tau[0][2] = tau[2][0] = t6[4]; tau[1][2] = tau[2][1] = t6[5]; }
-        else { tau[0][0] = t6[0]; tau[1][1] = t6[1]; tau[0][1] = tau[1][0] = t6[2]; }
-        const double muI = P.use_mibf ? src.muI[p] : 0.0;
-        const double aD = P.apicDinverse;
-        const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
 #pragma unroll
-        for (int ia = 0; ia < Stencil<D, SPL>::cnt; ++ia) {
-            const int a = ia;
+    for (int qq = 0; qq < QN; ++qq) {
+        const int q = tid + qq * SS::TB;
+        if (q >= G::NT) continue;
 #pragma unroll
-            for (int ib = 0; ib < Stencil<D, SPL>::cnt; ++ib) {
-                const int b = ib;
+        for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = tacc[qq][v];
+    }
+}
+
+// Tile of a block with more than kMaxBP particles: every particle adds its window with LDS atomics.
+template <int D, int SPL, int NV, class Contrib>
+__device__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV>& L, const Contrib& contrib, const Staging& stg, const SortBufs& sb,
+                                         uint32_t b, uint32_t start, uint32_t cnt, const int* bc) {
+    using G = BG<D, SPL>;
+    using SS = ScatterShape<D, SPL, NV>;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < G::NT * NV; q += blockDim.x) L.u.tile[q] = 0.0;
+    __syncthreads();
+    uint32_t err = 0;
+    for (uint32_t k = tid; k < cnt; k += blockDim.x) {
+        const uint32_t i = start + k, p = sb.perm[i];
+        bool ok = true;
+        for (int a = 0; a < G::W && ok; ++a) {
+            for (int half = 0; half < SS::SPLIT && ok; ++half) {
+                double acc[SS::NSH][NV];
 #pragma unroll
-                for (int ic = 0; ic < c2; ++ic) {
-                    const int c = (D == 3) ? ic : 0;
-                    double weight = st.w[0][a] * st.w[1][b];
-                    if (D == 3) weight = weight * st.w[2][c];
-                    const int gi = st.base[0] + a, gj = st.base[1] + b, gk = (D == 3) ? st.base[2] + c : 0;
-                    double val[8];
-                    bool massive = weight > 1e-25;
-                    val[0] = weight;
-                    if (massive) {
-                        const double pd[3] = {st.xi[0][a] - xp[0], st.xi[1][b] - xp[1], D == 3 ? st.xi[2][c] - xp[2] : 0.0};
-                        for (int r = 0; r < D; ++r) {
-                            double m = vp[r] * weight;
-                            if (apic) {
-                                double bd = Bm[r] * pd[0];
-                                for (int k = 1; k < D; ++k) bd += Bm[r + k * D] * pd[k];
-                                m += bd * aD * weight;
-                            }
-                            val[1 + r] = m;
-                        }
-                        val[7] = muI * weight;
-                    }
-                    double gr[3];
+                for (int o = 0; o < SS::NSH; ++o)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[o][v] = 0.0;
+                int base[3] = {0, 0, 0};
+                typename Contrib::Part q;
+                contrib.load(p, i, q);
+                ok = contrib.add(q, a, half, acc, base);
+                int l[3] = {0, 0, 0};
+                for (int d = 0; d < D; ++d) { l[d] = base[d] - G::E * bc[d]; ok = ok && l[d] >= 0 && l[d] < G::E; }
+                if (!ok) { err |= MPM_ERR_GRID_BOUNDS; break; }
+#pragma unroll
+                for (int obl = 0; obl < SS::OBN; ++obl) {
+                    const int ob = half * SS::OBN + obl;
                     if (D == 3) {
-                        gr[0] = st.dw[0][a] * st.w[1][b] * st.w[2][c] * h1;
-                        gr[1] = st.dw[1][b] * st.w[0][a] * st.w[2][c] * h1;
-                        gr[2] = st.dw[2][c] * st.w[0][a] * st.w[1][b] * h1;
-                    } else {
-                        gr[0] = st.dw[0][a] * st.w[1][b] * h1;
-                        gr[1] = st.dw[1][b] * st.w[0][a] * h1;
-                    }
-                    for (int r = 0; r < D; ++r) {
-                        double fr = tau[r][0] * gr[0];
-                        for (int k = 1; k < D; ++k) fr += tau[r][k] * gr[k];
-                        val[4 + r] = fr;
-                    }
-                    if (t.fits) {
-                        const int q = tile_local<D>(t, gi, gj, gk);
-                        if (massive) {
-                            atomicAdd(&tile[q], val[0]);
-                            for (int r = 0; r < D; ++r) atomicAdd(&tile[(1 + r) * kTileCap + q], val[1 + r]);
-                            if (P.use_mibf) atomicAdd(&tile[7 * kTileCap + q], val[7]);
+#pragma unroll
+                        for (int oc = 0; oc < G::W; ++oc) {
+                            const int q = tile_index<D, SPL>(l[0] + a, l[1] + ob, l[2] + oc);
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], acc[obl * G::W + oc][v]);
                         }
-                        for (int r = 0; r < D; ++r) atomicAdd(&tile[(4 + r) * kTileCap + q], val[4 + r]);
                     } else {
-                        const uint32_t node = node_index<D>(gi, gj, gk, nn);
-                        if (massive) {
-                            atomic_add_f64(&g.mass[node], val[0]);
-                            for (int r = 0; r < D; ++r) atomic_add_f64(&g.mv[r][node], val[1 + r]);
-                            if (P.use_mibf) atomic_add_f64(&g.fric[node], val[7]);
-                        }
-                        for (int r = 0; r < D; ++r) atomic_add_f64(&g.f[r][node], val[4 + r]);
+                        const int q = tile_index<D, SPL>(l[0] + a, l[1] + ob, 0);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], acc[obl][v]);
                     }
                 }
             }
         }
-        if (S->check_conservation)
-            for (int d = 0; d < D; ++d) atomic_add_f64(&S->cons_part_mom[d], P.particle_mass * vp[d]);
     }
-    if (t.fits) {
-        __syncthreads();
-        double* dst[8] = {g.mass, g.mv[0], g.mv[1], g.mv[2], g.f[0], g.f[1], g.f[2], g.fric};
-        if (D == 2) { dst[3] = nullptr; dst[6] = nullptr; }
-        // tile rows: 0 mass, 1..3 momentum, 4..6 force, 7 friction (unused rows stay zero)
-        for (int q = threadIdx.x; q < t.vol; q += blockDim.x) {
-            int ii, jj, kk;
-            if (D == 3) { kk = q % t.ext[2]; int r = q / t.ext[2]; jj = r % t.ext[1]; ii = r / t.ext[1]; }
-            else { jj = q % t.ext[1]; ii = q / t.ext[1]; kk = 0; }
-            const uint32_t node = node_index<D>(t.lo[0] + ii, t.lo[1] + jj, t.lo[2] + kk, nn);
-            for (int v = 0; v < 8; ++v) {
-                if (!dst[v] || (v == 7 && !P.use_mibf)) continue;
-                const double a = tile[v * kTileCap + q];
-                if (a != 0.0) atomic_add_f64(dst[v] + node, a);
+    __syncthreads();
+    for (int q = tid; q < G::NT; q += blockDim.x)
+        for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = L.u.tile[v * G::NT + q];
+    return err;
+}
+
+// Weight, derivative and coordinate of window node `off` on axis d, evaluated as make_stencil does
+// (the same expression on the same operands gives the same bits) but for a run-time offset.
+template <int SPL>
+__device__ __forceinline__ void axis_node(double* const* coord, int d, int base, int off, double x, double h1, double& w, double& dw, double& xi) {
+    xi = coord[d][base + off];
+    const double t = (x - xi) * h1;
+    w = BSpline<SPL>::N(t);
+    dw = BSpline<SPL>::dN(t);
+}
+
+// Window offsets (a, second-axis offsets of `half`, all third-axis offsets) of one particle: weights,
+// offsets to the particle and weight gradients. f(ol, weight, pd, gr) is called per node with the
+// worker-local offset index ol.
+template <int D, int SPL, int OBN, class F>
+__device__ __forceinline__ void for_slice(const Stencil<D, SPL>& st, double* const* coord, const double* xp, double h1, int a, int half, F&& f) {
+    constexpr int W = BG<D, SPL>::W;
+    double w0, dw0, xi0;
+    axis_node<SPL>(coord, 0, st.base[0], a, xp[0], h1, w0, dw0, xi0);
+#pragma unroll
+    for (int obl = 0; obl < OBN; ++obl) {
+        double w1, dw1, xi1;
+        if (OBN == W) { w1 = st.w[1][obl]; dw1 = st.dw[1][obl]; xi1 = st.xi[1][obl]; }
+        else axis_node<SPL>(coord, 1, st.base[1], half * OBN + obl, xp[1], h1, w1, dw1, xi1);
+        if (D == 3) {
+#pragma unroll
+            for (int oc = 0; oc < W; ++oc) {
+                const double w2 = st.w[2][oc], dw2 = st.dw[2][oc];
+                double weight = w0 * w1;
+                weight = weight * w2;
+                const double pd[3] = {xi0 - xp[0], xi1 - xp[1], st.xi[2][oc] - xp[2]};
+                const double gr[3] = {dw0 * w1 * w2 * h1, dw1 * w0 * w2 * h1, dw2 * w0 * w1 * h1};
+                f(obl * W + oc, weight, pd, gr);
             }
+        } else {
+            const double weight = w0 * w1;
+            const double pd[3] = {xi0 - xp[0], xi1 - xp[1], 0.0};
+            const double gr[3] = {dw0 * w1 * h1, dw1 * w0 * h1, 0.0};
+            f(obl, weight, pd, gr);
         }
     }
 }
 
+// P2G contributions (p2g.cpp:12-66 and the force of explicit_euler_update.cpp:15-168): value 0 sum of
+// weights, 1..D momentum sum w (v + B (x_i - x_p) D^-1) (weight > 1e-25 only), then either the force sum
+// tau grad w in separate values (SEP) or the force folded into the momentum as - dt V0 / m_p tau grad w,
+// and last the MIBF friction sum w muI. load() fetches a particle, add() computes its stencil and adds
+// it, so that a worker can fetch the next particle while it adds the current one.
+template <int D, int SPL, bool SEP, bool MIBF>
+struct P2GContrib {
+    static constexpr int NV = 1 + D + (SEP ? D : 0) + (MIBF ? 1 : 0);
+    using SS = ScatterShape<D, SPL, NV>;
+    struct Part { double x[3], v[3], B[9], t6[6], muI; };
+    PartState src;
+    double* coord[3];
+    double org[3];
+    uint32_t nn[3];
+    double h1, aD, kf;
+    bool apic;
+    __device__ __forceinline__ void load(uint32_t p, uint32_t, Part& q) const {
+        for (int d = 0; d < D; ++d) { q.x[d] = src.x[d][p]; q.v[d] = src.v[d][p]; }
+        for (int k = 0; k < D * D; ++k) q.B[k] = apic ? src.B[k][p] : 0.0;
+        for (int k = 0; k < SymPack<D>::n; ++k) q.t6[k] = src.tau[k][p];
+        q.muI = MIBF ? src.muI[p] : 0.0;
+    }
+    __device__ __forceinline__ bool add(const Part& q, int a, int half, double (&acc)[SS::NSH][NV], int* base) const {
+        Stencil<D, SPL> st;
+        if (!make_stencil<D, SPL>(st, q.x, org, nn, coord, h1, true)) return false;
+        for (int d = 0; d < D; ++d) base[d] = st.base[d];
+        double tau[3][3];
+        if (D == 3) { tau[0][0] = q.t6[0]; tau[1][1] = q.t6[1]; tau[2][2] = q.t6[2]; tau[0][1] = tau[1][0] = q.t6[3]; tau[0][2] = tau[2][0] = q.t6[4]; tau[1][2] = tau[2][1] = q.t6[5]; }
+        else { tau[0][0] = q.t6[0]; tau[1][1] = q.t6[1]; tau[0][1] = tau[1][0] = q.t6[2]; }
+        for_slice<D, SPL, SS::OBN>(st, coord, q.x, h1, a, half, [&](int ol, double weight, const double* pd, const double* gr) __attribute__((always_inline)) {
+            const bool massive = weight > 1e-25;
+            double fr[3];
+            for (int r = 0; r < D; ++r) {
+                double s = tau[r][0] * gr[0];
+                for (int k = 1; k < D; ++k) s += tau[r][k] * gr[k];
+                fr[r] = s;
+            }
+            acc[ol][0] += massive ? weight : 0.0;
+            for (int r = 0; r < D; ++r) {
+                double m = q.v[r] * weight;
+                if (apic) {
+                    double bd = q.B[r] * pd[0];
+                    for (int k = 1; k < D; ++k) bd += q.B[r + k * D] * pd[k];
+                    m += bd * aD * weight;
+                }
+                m = massive ? m : 0.0;
+                if (SEP) { acc[ol][1 + r] += m; acc[ol][1 + D + r] += fr[r]; }
+                else acc[ol][1 + r] += m - kf * fr[r];
+            }
+            if (MIBF) acc[ol][NV - 1] += massive ? q.muI * weight : 0.0;
+        });
+        return true;
+    }
+};
+
+// MUSL re-scatter contributions (musl.cpp:28-85): sum w (v_new + B_new (x_i - x_p) D^-1) over
+// weight > 1e-25, with the position of the step's start (source layout, slot p) and the new velocity
+// and affine matrix (destination layout, sorted slot i).
+template <int D, int SPL>
+struct MuslContrib {
+    static constexpr int NV = D;
+    using SS = ScatterShape<D, SPL, NV>;
+    struct Part { double x[3], v[3], B[9]; };
+    PartState src, dst;
+    double* coord[3];
+    double org[3];
+    uint32_t nn[3];
+    double h1, aD;
+    bool apic;
+    __device__ __forceinline__ void load(uint32_t p, uint32_t i, Part& q) const {
+        for (int d = 0; d < D; ++d) { q.x[d] = src.x[d][p]; q.v[d] = dst.v[d][i]; }
+        for (int k = 0; k < D * D; ++k) q.B[k] = apic ? dst.B[k][i] : 0.0;
+    }
+    __device__ __forceinline__ bool add(const Part& q, int a, int half, double (&acc)[SS::NSH][NV], int* base) const {
+        Stencil<D, SPL> st;
+        if (!make_stencil<D, SPL>(st, q.x, org, nn, coord, h1, true)) return false;
+        for (int d = 0; d < D; ++d) base[d] = st.base[d];
+        for_slice<D, SPL, SS::OBN>(st, coord, q.x, h1, a, half, [&](int ol, double weight, const double* pd, const double*) __attribute__((always_inline)) {
+            if (!(weight > 1e-25)) return;
+            for (int r = 0; r < D; ++r) {
+                double m = q.v[r] * weight;
+                if (apic) {
+                    double bd = q.B[r] * pd[0];
+                    for (int k = 1; k < D; ++k) bd += q.B[r + k * D] * pd[k];
+                    m += bd * aD * weight;
+                }
+                acc[ol][r] += m;
+            }
+        });
+        return true;
+    }
+};
+
 // ----------------------------------------------------------------------------------------------
-// k_grid_update: normalisation (p2g.cpp:84-98), explicit update and boundary conditions
+// k_p2g: P2G (p2g.cpp:12-66) fused with the stress force of explicitEulerUpdate
+// (explicit_euler_update.cpp:15-168). The reference adds the force only where m_i > 0; nodes with
+// m_i = 0 are never updated or read, so the force is scattered unconditionally.
+// ----------------------------------------------------------------------------------------------
+template <int D, int SPL, bool SEP, bool MIBF>
+__global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF>::NV>::TB))
+void k_p2g(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g, Staging stg) {
+    constexpr int NV = P2GContrib<D, SPL, SEP, MIBF>::NV;
+    __shared__ ScatterLDS<D, SPL, NV> L;
+    if (S->stop) return;
+    const uint32_t b = blockIdx.x;
+    if (b >= S->nblocks) return;
+    const uint32_t cnt = sb.count[b];
+    if (cnt == 0) return;
+    const uint32_t start = sb.offset[b];
+    int bc[3];
+    block_coords<D>(b, S, bc);
+    P2GContrib<D, SPL, SEP, MIBF> cf;
+    cf.src = src;
+    for (int d = 0; d < 3; ++d) { cf.coord[d] = g.coord[d]; cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
+    cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
+    cf.kf = S->dt * P.particle_volume / P.particle_mass;
+    uint32_t err = 0;
+    if (cnt > kMaxBP) {
+        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) dst.id[start + k] = src.id[sb.perm[start + k]];
+        err |= scatter_block_atomic<D, SPL, NV>(L, cf, stg, sb, b, start, cnt, bc);
+    } else {
+        err |= order_block<D, SPL, NV>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
+        scatter_block<D, SPL, NV>(L, cf, stg, b, start);
+    }
+    if (S->check_conservation) {  // sum of m_p v_p (simulation.cpp:292-317)
+        double pm[3] = {0, 0, 0};
+        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+            const uint32_t p = sb.perm[start + k];
+            for (int d = 0; d < D; ++d) pm[d] += P.particle_mass * src.v[d][p];
+        }
+        for (int d = 0; d < D; ++d) pm[d] = wave_sum_f64(pm[d]);
+        if ((threadIdx.x & 63) == 0) for (int d = 0; d < D; ++d) atomic_add_f64(&S->cons_part_mom[d], pm[d]);
+    }
+    err = wave_or_u32(err);
+    if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
+}
+
+// k_musl_scatter: the MUSL re-scatter, in the particle order that k_p2g fixed for the step.
+template <int D, int SPL>
+__global__ __launch_bounds__((ScatterShape<D, SPL, D>::TB))
+void k_musl_scatter(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g, Staging stg) {
+    using G = BG<D, SPL>;
+    __shared__ ScatterLDS<D, SPL, D> L;
+    if (S->stop) return;
+    const uint32_t b = blockIdx.x;
+    if (b >= S->nblocks) return;
+    const uint32_t cnt = sb.count[b];
+    if (cnt == 0) return;
+    const uint32_t start = sb.offset[b];
+    int bc[3];
+    block_coords<D>(b, S, bc);
+    MuslContrib<D, SPL> cf;
+    cf.src = src; cf.dst = dst;
+    for (int d = 0; d < 3; ++d) { cf.coord[d] = g.coord[d]; cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
+    cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
+    uint32_t err = 0;
+    if (cnt > kMaxBP) {
+        err |= scatter_block_atomic<D, SPL, D>(L, cf, stg, sb, b, start, cnt, bc);
+    } else {
+        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) L.p[k] = sb.perm[start + k];
+        for (int c = threadIdx.x; c <= G::NC; c += blockDim.x) L.cstart[c] = sb.cell_off[(size_t)b * (G::NC + 1) + c];
+        if (threadIdx.x == 0) L.cstart[G::NC + 1] = cnt;
+        __syncthreads();
+        scatter_block<D, SPL, D>(L, cf, stg, b, start);
+    }
+    err = wave_or_u32(err);
+    if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
+}
+
+// Sum over the blocks that cover node gx (at most two per axis), in increasing block order, of the
+// staged tile values.
+template <int D, int SPL, int NV>
+__device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& sb, const Staging& stg, const int* gx, double (&s)[NV]) {
+    using G = BG<D, SPL>;
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+        const int l = gx[d] - (G::T - 1);
+        lo[d] = l <= 0 ? 0 : (l + G::E - 1) / G::E;
+        hi[d] = min(gx[d] / G::E, (int)S->nb[d] - 1);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] = 0.0;
+    for (int b0 = lo[0]; b0 <= hi[0]; ++b0)
+        for (int b1 = lo[1]; b1 <= hi[1]; ++b1)
+            for (int b2 = lo[2]; b2 <= hi[2]; ++b2) {
+                const uint32_t bl = D == 3 ? ((uint32_t)b0 * S->nb[1] + (uint32_t)b1) * S->nb[2] + (uint32_t)b2 : (uint32_t)b0 * S->nb[1] + (uint32_t)b1;
+                if (sb.count[bl] == 0) continue;
+                const int q = tile_index<D, SPL>(gx[0] - G::E * b0, gx[1] - G::E * b1, D == 3 ? gx[2] - G::E * b2 : 0);
+                const size_t at = (size_t)bl * G::NT + q;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) s[v] += stg.base[(size_t)v * stg.stride + at];
+            }
+}
+
+// ----------------------------------------------------------------------------------------------
+// k_grid_update: node sums, normalisation (p2g.cpp:84-98), explicit update and boundary conditions
 // (explicit_euler_update.cpp:183-232, boundary_collision.cpp:5-394), flip = v_new - v_old.
 // ----------------------------------------------------------------------------------------------
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, GridPtrs g) {
+template <int D, int SPL, bool SEP, bool MIBF>
+__global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, SortBufs sb, GridPtrs g, Staging stg) {
+    constexpr int NV = P2GContrib<D, SPL, SEP, MIBF>::NV;
     if (S->stop) return;
     const uint32_t nn = S->grid_nodes;
     const uint32_t n[3] = {S->n[0], S->n[1], S->n[2]};
@@ -668,20 +999,23 @@ __global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, 
     const bool cons = S->check_conservation;
     double cm = 0.0, cmv[3] = {0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        const double sw = g.mass[i];
+        int gx[3] = {0, 0, 0};
+        if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
+        else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
+        double s[NV];
+        sum_staging<D, SPL, NV>(S, sb, stg, gx, s);
+        const double sw = s[0];
         V<D> v;
-        double fr = 0.0;
-        if (sw > 0) { for (int d = 0; d < D; ++d) v[d] = g.mv[d][i] / sw; }
+        if (sw > 0) { for (int d = 0; d < D; ++d) v[d] = s[1 + d] / sw; }
         else { for (int d = 0; d < D; ++d) v[d] = 0.0; }
-        if (P.use_mibf) fr = g.fric[i] / sw;
+        const double fr = MIBF ? s[NV - 1] / sw : 0.0;
         const double mi = sw * P.particle_mass;
         if (cons) { cm += mi; for (int d = 0; d < D; ++d) cmv[d] += v[d] * mi; }
         V<D> vn, flip;
         if (mi > 0) {
-            for (int d = 0; d < D; ++d) vn[d] = v[d] + (((-dtV) * g.f[d][i]) / mi + dtg[d]);
+            for (int d = 0; d < D; ++d) vn[d] = SEP ? v[d] + (((-dtV) * s[1 + D + d]) / mi + dtg[d]) : v[d] + dtg[d];
             V<D> X;
-            if (D == 3) { uint32_t ii = i / (n[1] * n[2]); uint32_t jj = (i / n[2]) % n[1]; uint32_t kk = i % n[2]; X[0] = g.coord[0][ii]; X[1] = g.coord[1][jj]; X[D - 1] = g.coord[2][kk]; }
-            else { uint32_t ii = i / n[1]; uint32_t jj = i % n[1]; X[0] = g.coord[0][ii]; X[1] = g.coord[1][jj]; }
+            for (int d = 0; d < D; ++d) X[d] = g.coord[d][gx[d]];
             boundary_collision<D>(P, S->plates, dt, fr, X, vn);
             for (int d = 0; d < D; ++d) flip[d] = vn[d] - v[d];
         } else {
@@ -689,8 +1023,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, 
             for (int d = 0; d < D; ++d) flip[d] = 0.0;
         }
         g.mass[i] = mi;
-        for (int d = 0; d < D; ++d) { g.v[d][i] = vn[d]; g.flip[d][i] = flip[d]; }
-        if (P.use_mibf) g.fric[i] = fr;
+        for (int d = 0; d < D; ++d) g.v[d][i] = vn[d];
+        if (SEP) for (int d = 0; d < D; ++d) g.flip[d][i] = flip[d];
+        if (MIBF) g.fric[i] = fr;
     }
     if (cons) {
         cm = wave_sum_f64(cm);
@@ -698,6 +1033,28 @@ __global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, 
         if ((threadIdx.x & 63) == 0) {
             atomic_add_f64(&S->cons_grid_mass, cm);
             for (int d = 0; d < D; ++d) atomic_add_f64(&S->cons_grid_mom[d], cmv[d]);
+        }
+    }
+}
+
+// k_musl_update: MUSL normalisation, musl.cpp:182-188: v_i = (sum) / (m_i / m_p), zero where m_i == 0.
+template <int D, int SPL>
+__global__ __launch_bounds__(kBlock) void k_musl_update(Params P, const StepState* S, SortBufs sb, GridPtrs g, Staging stg) {
+    if (S->stop) return;
+    const uint32_t nn = S->grid_nodes;
+    const uint32_t n[3] = {S->n[0], S->n[1], S->n[2]};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+        int gx[3] = {0, 0, 0};
+        if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
+        else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
+        const double mi = g.mass[i];
+        if (mi > 0) {
+            double s[D];
+            sum_staging<D, SPL, D>(S, sb, stg, gx, s);
+            const double q = mi / P.particle_mass;
+            for (int d = 0; d < D; ++d) g.v[d][i] = s[d] / q;
+        } else {
+            for (int d = 0; d < D; ++d) g.v[d][i] = 0.0;
         }
     }
 }
@@ -755,23 +1112,58 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
 
 // Gather of G2P (g2p.cpp:35-67) and of the velocity gradient (deformation_update.cpp:217-230) over the
 // non-zero nodes of the window, in the reference's (i,j,k) order.
+
+// ----------------------------------------------------------------------------------------------
+// Gather kernels: one workgroup per block; the block's tile of node values is copied to LDS once and
+// every particle of the block gathers from it.
+// ----------------------------------------------------------------------------------------------
+
+// Copies ncomp node arrays over the block's tile into LDS ([component][node]); nodes outside the grid
+// read as zero (no particle window reaches them).
+template <int D, int SPL>
+__device__ __forceinline__ void load_tile(double* s, double* const* src, int ncomp, const int* bc, const uint32_t* n) {
+    using G = BG<D, SPL>;
+    for (int q = threadIdx.x; q < G::NT; q += blockDim.x) {
+        int t[3] = {0, 0, 0};
+        if (D == 3) { t[2] = q % G::T; const int r = q / G::T; t[1] = r % G::T; t[0] = r / G::T; }
+        else { t[1] = q % G::T; t[0] = q / G::T; }
+        bool in = true;
+        int gx[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) { gx[d] = G::E * bc[d] + t[d]; in = in && gx[d] < (int)n[d]; }
+        const uint32_t node = in ? node_index<D>(gx[0], gx[1], gx[2], n) : 0u;
+        for (int c = 0; c < ncomp; ++c) s[c * G::NT + q] = in ? src[c][node] : 0.0;
+    }
+}
+
+// Stencil of a gathering particle (G2P does not clamp, g2p.cpp:29-32) and its tile-local base.
+template <int D, int SPL>
+__device__ __forceinline__ bool gather_stencil(Stencil<D, SPL>& st, int* tl, const double* xp, const double* org, const uint32_t* n,
+                                               double* const* coord, double h1, const int* bc) {
+    using G = BG<D, SPL>;
+    if (!make_stencil<D, SPL>(st, xp, org, n, coord, h1, false)) return false;
+    bool in = true;
+    tl[2] = 0;
+    for (int d = 0; d < D; ++d) { tl[d] = st.base[d] - G::E * bc[d]; in = in && tl[d] >= 0 && tl[d] < G::E; }
+    return in;
+}
+
+// Gather of G2P (g2p.cpp:35-67) and of the velocity gradient (deformation_update.cpp:217-230) over the
+// non-zero nodes of the window, in the reference's (i,j,k) order, from the LDS tile.
 template <int D, int SPL, bool PIC, bool GRAD>
-__device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* xp, const GridPtrs& g, const uint32_t* n,
-                                       double h1, bool apic, bool useflip, double* pic, double* fl, double (&Bp)[3][3],
-                                       double (&gv)[3][3]) {
+__device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const int* tl, const double* xp, const double* s_v, const double* s_flip,
+                                       double h1, bool apic, bool useflip, double* pic, double* fl, double (&Bp)[3][3], double (&gv)[3][3]) {
+    using G = BG<D, SPL>;
     const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
 #pragma unroll
-    for (int ia = 0; ia < Stencil<D, SPL>::cnt; ++ia) {
-        const int a = ia;
+    for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) {
 #pragma unroll
-        for (int ib = 0; ib < Stencil<D, SPL>::cnt; ++ib) {
-            const int b = ib;
+        for (int b = 0; b < Stencil<D, SPL>::cnt; ++b) {
 #pragma unroll
             for (int ic = 0; ic < c2; ++ic) {
                 const int c = (D == 3) ? ic : 0;
-                const uint32_t node = node_index<D>(st.base[0] + a, st.base[1] + b, (D == 3) ? st.base[2] + c : 0, n);
+                const int q = tile_index<D, SPL>(tl[0] + a, tl[1] + b, tl[2] + c);
                 double vi[3];
-                for (int d = 0; d < D; ++d) vi[d] = g.v[d][node];
+                for (int d = 0; d < D; ++d) vi[d] = s_v[d * G::NT + q];
                 if (PIC) {
                     double weight = st.w[0][a] * st.w[1][b];
                     if (D == 3) weight = weight * st.w[2][c];
@@ -780,7 +1172,7 @@ __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* 
                         pic[r] += vi[r] * weight;
                         if (apic) for (int k = 0; k < D; ++k) Bp[r][k] += vi[r] * pd[k] * weight;
                     }
-                    if (useflip) for (int r = 0; r < D; ++r) fl[r] += g.flip[r][node] * weight;
+                    if (useflip) for (int r = 0; r < D; ++r) fl[r] += s_flip[r * G::NT + q] * weight;
                 }
                 if (GRAD) {
                     double gr[3];
@@ -799,44 +1191,60 @@ __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* 
     }
 }
 
+// Block prologue shared by the gather kernels: returns false when the block has nothing to do.
+struct BlockRange { uint32_t b, start, cnt; int bc[3]; };
+template <int D>
+__device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& sb, BlockRange& r) {
+    if (S->stop) return false;
+    r.b = blockIdx.x;
+    if (r.b >= S->nblocks) return false;
+    r.cnt = sb.count[r.b];
+    if (r.cnt == 0) return false;
+    r.start = sb.offset[r.b];
+    block_coords<D>(r.b, S, r.bc);
+    return true;
+}
+
 // ----------------------------------------------------------------------------------------------
-// k_g2p_fused (USL): G2P (g2p.cpp:17-77), deformationUpdate + plasticity, positionUpdate.
+// k_g2p_usl (USL): G2P (g2p.cpp:17-77), deformationUpdate + plasticity, positionUpdate.
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_g2p_fused(Params P, StepState* S, PartState src, PartState dst, SortBufs sb,
-                                                      GridPtrs g, uint32_t n) {
-    if (S->stop) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g) {
+    using G = BG<D, SPL>;
+    __shared__ double s_v[3 * G::NT], s_f[3 * G::NT];
+    BlockRange br;
+    if (!block_range<D>(S, sb, br)) return;
+    const double h1 = P.one_over_dx, dt = S->dt;
+    const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
+    const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
+    const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
+    load_tile<D, SPL>(s_v, g.v, D, br.bc, nn);
+    if (useflip) load_tile<D, SPL>(s_f, g.flip, D, br.bc, nn);
+    __syncthreads();
     uint32_t err = 0, pcount = 0;
-    double xnew[3] = {0, 0, 0}, vnew[3] = {0, 0, 0};
-    bool live = false;
-    if (i < n) {
-        const uint32_t p = sb.perm[i];
-        const double h1 = P.one_over_dx, dt = S->dt;
-        const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
-        const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
+    BoundsAcc ba;
+    for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) {
+        const uint32_t i = br.start + k, p = sb.perm[i];
         double xp[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
         Stencil<D, SPL> st;
-        if (!make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, false)) {
-            err |= MPM_ERR_GRID_BOUNDS;
-        } else {
-            live = true;
-            const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
-            double pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3] = {{0}};
-            gather<D, SPL, true, true>(st, xp, g, nn, h1, apic, useflip, pic, fl, Bp, gv);
-            for (int d = 0; d < D; ++d) {
-                const double vo = useflip ? src.v[d][p] : 0.0;
-                vnew[d] = blend_velocity(P.flip_ratio, vo, fl[d], pic[d]);
-            }
-            for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : 0.0;
-            deform_and_project<D>(P, dt, src, p, dst, i, gv, err, pcount);
-            for (int d = 0; d < D; ++d) xnew[d] = xp[d] + dt * pic[d];
-            if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
-            for (int d = 0; d < D; ++d) { dst.x[d][i] = xnew[d]; dst.v[d][i] = vnew[d]; }
+        int tl[3];
+        if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, g.coord, h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
+        double pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3] = {{0}};
+        gather<D, SPL, true, true>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gv);
+        double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) {
+            const double vo = useflip ? src.v[d][p] : 0.0;
+            vnew[d] = blend_velocity(P.flip_ratio, vo, fl[d], pic[d]);
         }
+        for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : 0.0;
+        deform_and_project<D>(P, dt, src, p, dst, i, gv, err, pcount);
+        for (int d = 0; d < D; ++d) xnew[d] = xp[d] + dt * pic[d];
+        if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
+        for (int d = 0; d < D; ++d) { dst.x[d][i] = xnew[d]; dst.v[d][i] = vnew[d]; }
+        ba.add<D>(xnew, vnew);
     }
-    reduce_step_bounds<D>(S, live, xnew, vnew);
+    reduce_bounds<D>(S, ba);
     pcount = wave_sum_u32(pcount);
     err = wave_or_u32(err);
     if ((threadIdx.x & 63) == 0) {
@@ -846,113 +1254,85 @@ __global__ __launch_bounds__(kBlock) void k_g2p_fused(Params P, StepState* S, Pa
 }
 
 // ----------------------------------------------------------------------------------------------
-// k_g2p_musl: G2P (g2p.cpp:17-77), MUSL velocity update and re-scatter (musl.cpp:13-85), position
-// update of positionUpdate (position_update.cpp:12). The re-scatter goes through an LDS tile like P2G.
+// k_g2p_musl (MUSL): G2P (g2p.cpp:17-77), the MUSL velocity update (musl.cpp:13-25) and the position
+// update of positionUpdate (position_update.cpp:12).
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, PartState src, PartState dst, SortBufs sb,
-                                                     GridPtrs g, uint32_t n) {
-    if (S->stop) return;
-    __shared__ double tile[3 * kTileCap];
-    __shared__ int s_box[6];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t err = 0;
-    double xnew[3] = {0, 0, 0}, vnew[3] = {0, 0, 0}, xp[3] = {0, 0, 0};
-    double Bp[3][3] = {{0}};
-    bool live = false;
+__global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g) {
+    using G = BG<D, SPL>;
+    __shared__ double s_v[3 * G::NT], s_f[3 * G::NT];
+    BlockRange br;
+    if (!block_range<D>(S, sb, br)) return;
     const double h1 = P.one_over_dx, dt = S->dt;
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
     const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
-    Stencil<D, SPL> st;
-    if (i < n) {
-        const uint32_t p = sb.perm[i];
+    load_tile<D, SPL>(s_v, g.v, D, br.bc, nn);
+    if (useflip) load_tile<D, SPL>(s_f, g.flip, D, br.bc, nn);
+    __syncthreads();
+    uint32_t err = 0;
+    BoundsAcc ba;
+    for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) {
+        const uint32_t i = br.start + k, p = sb.perm[i];
+        double xp[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
-        if (!make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, false)) {
-            err |= MPM_ERR_GRID_BOUNDS;
-        } else {
-            live = true;
-            double pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, gv[3][3];
-            gather<D, SPL, true, false>(st, xp, g, nn, h1, apic, useflip, pic, fl, Bp, gv);
-            for (int d = 0; d < D; ++d) {
-                const double vo = useflip ? src.v[d][p] : 0.0;
-                vnew[d] = blend_velocity(P.flip_ratio, vo, fl[d], pic[d]);
-                xnew[d] = xp[d] + dt * pic[d];
-            }
-            if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
-            for (int d = 0; d < D; ++d) { dst.v[d][i] = vnew[d]; dst.x[d][i] = xnew[d]; }
-            for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : 0.0;
-            // the re-scatter clamps the base at 0 as P2G does (musl.cpp:131-135); it differs from the
-            // gather window only if the gather window already left the grid, which is reported above
+        Stencil<D, SPL> st;
+        int tl[3];
+        if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, g.coord, h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
+        double pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3];
+        gather<D, SPL, true, false>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gv);
+        double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) {
+            const double vo = useflip ? src.v[d][p] : 0.0;
+            vnew[d] = blend_velocity(P.flip_ratio, vo, fl[d], pic[d]);
+            xnew[d] = xp[d] + dt * pic[d];
         }
+        if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
+        for (int d = 0; d < D; ++d) { dst.v[d][i] = vnew[d]; dst.x[d][i] = xnew[d]; }
+        for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : 0.0;
+        ba.add<D>(xnew, vnew);
     }
-    const Tile t = tile_bbox<D, Stencil<D, SPL>::cnt>(live, st.base, s_box);
-    if (t.fits) {
-        for (int q = threadIdx.x; q < 3 * kTileCap; q += blockDim.x) tile[q] = 0.0;
-        __syncthreads();
-    }
-    if (live) {
-        const double aD = P.apicDinverse;
-        const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
-#pragma unroll
-        for (int ia = 0; ia < Stencil<D, SPL>::cnt; ++ia) {
-            const int a = ia;
-#pragma unroll
-            for (int ib = 0; ib < Stencil<D, SPL>::cnt; ++ib) {
-                const int b = ib;
-#pragma unroll
-                for (int ic = 0; ic < c2; ++ic) {
-                    const int c = (D == 3) ? ic : 0;
-                    double weight = st.w[0][a] * st.w[1][b];
-                    if (D == 3) weight = weight * st.w[2][c];
-                    if (!(weight > 1e-25)) continue;
-                    const int gi = st.base[0] + a, gj = st.base[1] + b, gk = (D == 3) ? st.base[2] + c : 0;
-                    const double pd[3] = {st.xi[0][a] - xp[0], st.xi[1][b] - xp[1], D == 3 ? st.xi[2][c] - xp[2] : 0.0};
-                    double m[3];
-                    for (int r = 0; r < D; ++r) {
-                        m[r] = vnew[r] * weight;
-                        if (apic) {
-                            double bd = Bp[r][0] * pd[0];
-                            for (int k = 1; k < D; ++k) bd += Bp[r][k] * pd[k];
-                            m[r] += bd * aD * weight;
-                        }
-                    }
-                    if (t.fits) {
-                        const int q = tile_local<D>(t, gi, gj, gk);
-                        for (int r = 0; r < D; ++r) atomicAdd(&tile[r * kTileCap + q], m[r]);
-                    } else {
-                        const uint32_t node = node_index<D>(gi, gj, gk, nn);
-                        for (int r = 0; r < D; ++r) atomic_add_f64(&g.mv2[r][node], m[r]);
-                    }
-                }
-            }
-        }
-    }
-    if (t.fits) {
-        __syncthreads();
-        tile_flush<D, D>(t, tile, g.mv2, nn);
-    }
-    reduce_step_bounds<D>(S, live, xnew, vnew);
+    reduce_bounds<D>(S, ba);
     err = wave_or_u32(err);
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
 
-// MUSL normalisation, musl.cpp:182-188: v_i = (sum) / (m_i / m_p), zero where m_i == 0.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_musl_norm(Params P, const StepState* S, GridPtrs g) {
-    if (S->stop) return;
-    const uint32_t nn = S->grid_nodes;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        const double mi = g.mass[i];
-        if (mi > 0) { const double q = mi / P.particle_mass; for (int d = 0; d < D; ++d) g.v[d][i] = g.mv2[d][i] / q; }
-        else for (int d = 0; d < D; ++d) g.v[d][i] = 0.0;
+// ----------------------------------------------------------------------------------------------
+// k_defgrad (MUSL): deformationUpdate (deformation_update.cpp:16-46) on the re-scattered velocity.
+// One thread per sorted particle slot (the return map is long and register-heavy, so this kernel is
+// laid out for occupancy rather than LDS reuse); the velocity gradient gathers from the grid arrays.
+// ----------------------------------------------------------------------------------------------
+template <int D, int SPL>
+__device__ __forceinline__ void gather_grad_global(const Stencil<D, SPL>& st, const GridPtrs& g, const uint32_t* n, double h1,
+                                                   double (&gv)[3][3]) {
+    const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
+#pragma unroll
+    for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) {
+#pragma unroll
+        for (int b = 0; b < Stencil<D, SPL>::cnt; ++b) {
+#pragma unroll
+            for (int ic = 0; ic < c2; ++ic) {
+                const int c = (D == 3) ? ic : 0;
+                const uint32_t node = node_index<D>(st.base[0] + a, st.base[1] + b, (D == 3) ? st.base[2] + c : 0, n);
+                double vi[3];
+                for (int d = 0; d < D; ++d) vi[d] = g.v[d][node];
+                double gr[3];
+                if (D == 3) {
+                    gr[0] = st.dw[0][a] * st.w[1][b] * st.w[2][c] * h1;
+                    gr[1] = st.dw[1][b] * st.w[0][a] * st.w[2][c] * h1;
+                    gr[2] = st.dw[2][c] * st.w[0][a] * st.w[1][b] * h1;
+                } else {
+                    gr[0] = st.dw[0][a] * st.w[1][b] * h1;
+                    gr[1] = st.dw[1][b] * st.w[0][a] * h1;
+                }
+                for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) gv[r][k] += vi[r] * gr[k];
+            }
+        }
     }
 }
 
-// k_defgrad (MUSL): deformationUpdate (deformation_update.cpp:16-46) on the re-scattered velocity.
 template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartState src, PartState dst, SortBufs sb,
-                                                    GridPtrs g, uint32_t n) {
+__global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g, uint32_t n) {
     if (S->stop) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t err = 0, pcount = 0;
@@ -967,8 +1347,8 @@ __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, Part
         if (!make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, false)) {
             err |= MPM_ERR_GRID_BOUNDS;
         } else {
-            double gv[3][3] = {{0}}, pic[3], fl[3], Bp[3][3];
-            gather<D, SPL, false, true>(st, xp, g, nn, h1, false, false, pic, fl, Bp, gv);
+            double gv[3][3] = {{0}};
+            gather_grad_global<D, SPL>(st, g, nn, h1, gv);
             deform_and_project<D>(P, dt, src, p, dst, i, gv, err, pcount);
         }
     }
