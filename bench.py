@@ -29,41 +29,44 @@ import mpm_abi as A  # noqa: E402
 import scenes as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published peak
-STEP_BYTES_PER_PARTICLE_UPDATE = 438  # 2*(24+24+72+72) + 40 + (2*(8+24) + 2*24)/8
 METRIC = "particle-updates/s 3D DP-APIC FP32 vs FP64 @stated err; served p50/p99 step ms"
 
 
 def algorithmic_bytes(kernel, cfg, n_particles, active_nodes, grid_nodes, blocks=None, tile_nodes=None):
     """Compulsory HBM bytes of one launch: particle state the kernel must read and write once, plus node
     data it must read/write once (the staged block tiles count as node data: each tile is written by the
-    scatter and read once by the node pass). FP64 (8 B) throughout in the plain form."""
+    scatter and read once by the node pass). Position, deformation gradient and plastic scalars are 8 B;
+    velocity, affine matrix, Kirchhoff stress and node values are 8 B in the plain form, 4 B in the split form."""
     D = cfg.dim
     f8 = 8
+    ft = 4 if cfg.precision == A.PRECISION["split"] else 8
     apic = cfg.flip_ratio < 0
     flip = cfg.flip_ratio >= -1
     tau = 6 if D == 3 else 3
     mibf = bool(cfg.use_mibf)
     nv = 1 + D + (D if flip else 0) + (1 if mibf else 0)
     staged = (blocks or 0) * (tile_nodes or 0)
+    B = D * D if apic else 0
     if kernel == "k_p2g":
-        part = D + D + (D * D if apic else 0) + tau + (1 if mibf else 0)
-        return f8 * (part * n_particles + nv * staged) + 4 * 3 * n_particles  # + perm read/write, ids
+        part = f8 * D + ft * (D + B + tau) + (f8 if mibf else 0)
+        return part * n_particles + ft * nv * staged + 4 * 3 * n_particles  # + permutation read/write, ids
     if kernel == "k_grid_update":
         out = 1 + D + (D if flip else 0) + (1 if mibf else 0)
-        return f8 * (nv * staged + out * grid_nodes)
+        return ft * (nv * staged + out * grid_nodes)
     if kernel == "k_g2p_musl":
-        part_r = D + (D if flip else 0)
-        part_w = D + (D * D if apic else 0) + D
-        return f8 * ((part_r + part_w) * n_particles + (D + (D if flip else 0)) * active_nodes)
+        part_r = f8 * D + (ft * D if flip else 0)
+        part_w = f8 * D + ft * (B + D)
+        return (part_r + part_w) * n_particles + ft * (D + (D if flip else 0)) * active_nodes
     if kernel == "k_musl_scatter":
-        return f8 * ((D + D + (D * D if apic else 0)) * n_particles + D * staged)
+        return (f8 * D + ft * (D + B)) * n_particles + ft * D * staged
     if kernel == "k_musl_update":
-        return f8 * (D * staged + (1 + D) * grid_nodes)
+        return ft * (D * staged + (1 + D) * grid_nodes)
     if kernel in ("k_defgrad", "k_g2p_usl"):
-        part_r = D + D * D + 2 + (D if kernel == "k_g2p_usl" else 0)
-        part_w = D * D + tau + 3 + (D + D + (D * D if apic else 0) if kernel == "k_g2p_usl" else 0)
-        node = D + (D if (kernel == "k_g2p_usl" and flip) else 0)
-        return f8 * ((part_r + part_w) * n_particles + node * active_nodes)
+        usl = kernel == "k_g2p_usl"
+        part_r = f8 * (D + D * D + 2) + (ft * D if usl and flip else 0)
+        part_w = f8 * (D * D + 3) + ft * tau + ((f8 * D + ft * (D + B)) if usl else 0)
+        node = ft * (D + (D if (usl and flip) else 0))
+        return (part_r + part_w) * n_particles + node * active_nodes
     if kernel == "k_bin_count":
         return f8 * D * n_particles + 4 * n_particles
     if kernel == "k_bin_scatter":
@@ -71,11 +74,25 @@ def algorithmic_bytes(kernel, cfg, n_particles, active_nodes, grid_nodes, blocks
     return 0
 
 
-def parity_check(steps=10, n_axis=20):
-    """Same configuration at n_axis^3 particles: engine vs single-threaded oracle after `steps` steps."""
+def step_bytes_per_particle_update(cfg):
+    """Compulsory bytes of one particle-update of the 3D APIC Drucker-Prager step at 8 particles per cell:
+    x, v, F, B read and written once, the plastic scalars (read eps_pl_dev, eps_pl_vol; write those and
+    delta_gamma), and the grid's share (mass and momentum written and read, velocity written and read, per
+    8 particles). Plain form 438 B; split form (float v, B, node values) 335 B."""
+    ft = 4 if cfg.precision == A.PRECISION["split"] else 8
+    D = cfg.dim
+    return 2 * (8 * D + ft * D + 8 * D * D + ft * D * D) + 40 + (2 * (ft + ft * D) + 2 * ft * D) / 8
+
+
+def parity_check(prec, n_axis=20):
+    """Same configuration at n_axis^3 particles: engine vs single-threaded oracle after 10 steps (plain
+    form) or 100 steps (split form)."""
+    steps = 10 if prec == "fp64" else 100
     cfg, plates, objs, x = S.dp_collapse(n_per_axis=n_axis, musl=True)
     out = {}
-    ge = A.Engine(A.product_library(), cfg, plates, objs)
+    gcfg = type(cfg).from_buffer_copy(cfg)
+    gcfg.precision = A.PRECISION[prec]
+    ge = A.Engine(A.product_library(), gcfg, plates, objs)
     oe = A.Engine(A.oracle_library(), cfg, plates, objs)
     ge.set_particles(x)
     oe.set_particles(x)
@@ -84,10 +101,13 @@ def parity_check(steps=10, n_axis=20):
     gp, op = ge.get_particles(), oe.get_particles()
     for k in ("x", "v", "F"):
         out[k] = float(np.max(np.abs(gp[k] - op[k])) / max(np.max(np.abs(op[k])), 1e-300))
+    out["x_norm_rel"] = float(np.linalg.norm(gp["x"] - op["x"]) / np.linalg.norm(op["x"]))
     gg, og = ge.get_grid(), oe.get_grid()
-    act = og["mass"] > 0
-    out["grid_mass"] = float(np.max(np.abs(gg["mass"][act] - og["mass"][act])) / np.max(og["mass"][act]))
-    out["grid_v"] = float(np.max(np.abs(gg["v"][act] - og["v"][act])) / max(np.max(np.abs(og["v"][act])), 1e-300))
+    if gg["mass"].shape == og["mass"].shape:
+        act = og["mass"] > 0
+        out["grid_mass"] = float(np.max(np.abs(gg["mass"][act] - og["mass"][act])) / np.max(og["mass"][act]))
+        mom_g, mom_o = gg["mass"][act, None] * gg["v"][act], og["mass"][act, None] * og["v"][act]
+        out["grid_momentum"] = float(np.max(np.abs(mom_g - mom_o)) / max(np.max(np.abs(mom_o)), 1e-300))
     out.update(particles=int(x.shape[0]), steps=steps, reference="CPU oracle, 1 thread")
     return out
 
@@ -110,7 +130,8 @@ def cpu_baseline(cfg, plates, objs, x, steps=3):
 
 
 COPY_BYTES = 256 * 1024 * 1024
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "c2_fp64_traffic.txt")
+def traffic_file(prec):
+    return os.path.join(REPO, "profiles", f"c2_{prec}_traffic.txt")
 
 
 def copy_calibration():
@@ -161,11 +182,12 @@ def summarize_pmc(fetch_csv, write_csv):
     return "\n".join(lines)
 
 
-def read_traffic(kernel):
+def read_traffic(kernel, prec):
     """HBM bytes per launch (read + write) of `kernel` from the committed counter summary, or None."""
-    if not os.path.exists(TRAFFIC_FILE):
+    path = traffic_file(prec)
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         f = line.split()
         if len(f) == 4 and f[0] == kernel:
             return float(f[2]) + float(f[3])
@@ -181,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 256 MiB device copy first (counter runs)")
+    ap.add_argument("--precision", choices=["both", "split", "fp64"], default="both",
+                    help="forms to run; the first one run (split when both) gives the headline value")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-kernel HBM traffic summary of two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
@@ -194,7 +218,48 @@ def main():
 
     if args.calibrate_copy:
         copy_calibration()
-    cfg, plates, objs, x = S.dp_collapse(n_per_axis=args.n_axis, musl=True)
+    forms = ["split", "fp64"] if args.precision == "both" else [args.precision]
+    runs = {}
+    for prec in forms:
+        runs[prec] = timed_run(prec, args)
+    head = runs[forms[0]]
+    cfg, n = head["cfg"], head["n"]
+    result = {
+        "metric": METRIC, "value": head["value"], "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32/f64 split" if forms[0] == "split" else "f64",
+        "data": "synthetic (seeded jittered lattice, SplitMix64 seed 42)",
+        "config": {"workload": "3D granular column collapse, Drucker-Prager 30 deg, APIC, MUSL",
+                   "particles": n, "dx": cfg.dx, "ppc": 8, "spline": "quadratic", "boundary": "SlipFree floor mu=0.4",
+                   "grid_nodes_last_step": head["grid_nodes"], "active_nodes_last_step": head["active"],
+                   "blocks_last_step": head["blocks"], "precision": forms[0]},
+        "roofline": head["roofline"],
+        "step_roofline": head["step_roofline"],
+        "kernels": head["kernels"],
+    }
+    if "split" in runs and "fp64" in runs:
+        xs, xd = runs["split"]["x"], runs["fp64"]["x"]
+        result["fp64"] = {k: runs["fp64"][k] for k in ("value", "ms_per_step", "roofline", "step_roofline", "kernels")}
+        result["split_speedup_vs_fp64"] = runs["split"]["value"] / runs["fp64"]["value"]
+        result["split_error_vs_fp64_gpu"] = {
+            "position_rel": float(np.linalg.norm(xs - xd) / np.linalg.norm(xd)),
+            "position_max_abs_m": float(np.max(np.abs(xs - xd))),
+            "after_steps": args.warmup + args.steps, "particles": n,
+            "note": "same seeded input, both forms on the GPU; the plain form is held to the CPU oracle at 1e-10"}
+    if not args.no_parity:
+        result["error_vs_fp64_oracle"] = {prec: parity_check(prec) for prec in forms}
+    if not args.no_cpu_baseline:
+        plates, objs, x = head["scene"]
+        cfg64 = type(cfg).from_buffer_copy(cfg)
+        cfg64.precision = A.PRECISION["fp64"]
+        result["cpu_baseline"] = cpu_baseline(cfg64, plates, objs, x)
+    print(json.dumps(result), flush=True)
+
+
+def timed_run(prec, args):
+    """W warm-up steps, then K steps timed by the host clock around one synchronised mpm_step call, with
+    per-kernel HIP-event times over the K steps."""
+    cfg, plates, objs, x = S.dp_collapse(n_per_axis=args.n_axis, musl=True, precision=prec)
     n = x.shape[0]
     eng = A.Engine(A.product_library(), cfg, plates, objs)
     eng.set_particles(x)
@@ -211,13 +276,13 @@ def main():
     kt = eng.kernel_times()
     eng.set_profiling(False)
     wall = t1 - t0
-    value = n * args.steps / wall
     g = eng.get_grid()
     active = int(np.count_nonzero(g["mass"] > 0))
     grid_nodes = int(g["info"].grid_nodes)
     _, keys = eng.get_binning()
     blocks = int(np.unique(keys).size)  # non-empty blocks of the last step
-    tile_nodes = (4 + (3 if cfg.spline_degree == 2 else 4) - 1) ** 3 if cfg.dim == 3 else (8 + (3 if cfg.spline_degree == 2 else 4) - 1) ** 2
+    w = 3 if cfg.spline_degree == 2 else 4
+    tile_nodes = (4 + w - 1) ** 3 if cfg.dim == 3 else (8 + w - 1) ** 2
     kernels = {}
     for name, (ms, launches) in kt.items():
         avg_ms = ms / max(launches, 1)
@@ -226,34 +291,20 @@ def main():
                          "algorithmic_bytes": b, "GBps": (b / (avg_ms * 1e-3) / 1e9) if avg_ms > 0 and b else None}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
-    traffic = read_traffic(dom)
+    traffic = read_traffic(dom, prec)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (d["GBps"] / HBM_PEAK_GBS) if d["GBps"] else None, "traffic": traffic,
-                "traffic_source": os.path.relpath(TRAFFIC_FILE, REPO) if traffic is not None else None,
+                "traffic_source": os.path.relpath(traffic_file(prec), REPO) if traffic is not None else None,
                 "avg_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
-    # whole step against the compulsory traffic of one particle-update (x, v, F, B read and written, plastic
-    # scalars, grid share): 438 B in 3D FP64 APIC Drucker-Prager at 8 particles per cell
-    step_bytes = STEP_BYTES_PER_PARTICLE_UPDATE * n
-    step_gbs = step_bytes / (wall / args.steps) / 1e9
-    step_roofline = {"bytes_per_particle_update": STEP_BYTES_PER_PARTICLE_UPDATE, "achieved": step_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
-    result = {
-        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded jittered lattice, SplitMix64 seed 42)",
-        "config": {"workload": "3D granular column collapse, Drucker-Prager 30 deg, APIC, MUSL, plain FP64",
-                   "particles": n, "dx": cfg.dx, "ppc": 8, "spline": "quadratic", "boundary": "SlipFree floor mu=0.4",
-                   "grid_nodes_last_step": grid_nodes, "active_nodes_last_step": active, "blocks_last_step": blocks,
-                   "precision": "fp64"},
-        "roofline": roofline,
-        "step_roofline": step_roofline,
-        "kernels": kernels,
-    }
-    if not args.no_parity:
-        result["error_vs_fp64_oracle"] = parity_check()
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(cfg, plates, objs, x)
-    print(json.dumps(result), flush=True)
+    bpu = step_bytes_per_particle_update(cfg)
+    step_gbs = bpu * n / (wall / args.steps) / 1e9
+    step_roofline = {"bytes_per_particle_update": bpu, "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": step_gbs / HBM_PEAK_GBS}
+    out = dict(cfg=cfg, n=n, value=n * args.steps / wall, ms_per_step=wall * 1e3 / args.steps, kernels=kernels,
+               roofline=roofline, step_roofline=step_roofline, grid_nodes=grid_nodes, active=active, blocks=blocks,
+               x=eng.get_particles()["x"], scene=(plates, objs, x))
+    del eng
+    return out
 
 
 if __name__ == "__main__":

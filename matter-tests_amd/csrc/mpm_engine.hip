@@ -21,25 +21,29 @@ struct mpm_context {
     hipStream_t stream = nullptr;
     uint32_t n = 0;
     bool ready = false;
+    bool split = false;         // MPM_PRECISION_SPLIT: transfer quantities in float
     // particles: two layouts (double buffer) and the counting-sort arrays
-    double* pbuf = nullptr;
-    size_t pbuf_n = 0;
+    char* pbuf = nullptr;
+    size_t pbuf_bytes = 0;
     uint32_t* ibuf = nullptr;   // ids (2 layouts), keys, permutation
     size_t ibuf_n = 0;
-    PartState ps[2]{};
+    PartStateT<double> ps[2]{};  // plain form
+    PartStateT<float> pf[2]{};   // split form
     SortBufs sb{};
     uint32_t* bbuf = nullptr;   // block histogram, offsets, cursors
     uint16_t* cellbuf = nullptr; // per-block dual-cell offsets
-    double* stgbuf = nullptr;   // per-block node tiles (staging of the scatter kernels)
+    char* stgbuf = nullptr;     // per-block node tiles (staging of the scatter kernels)
     uint32_t cap_blocks = 0;
-    Staging stg{};
+    StagingT<double> stg{};
+    StagingT<float> stgf{};
     bool flip_formed = false;   // the last enqueued step wrote grid flip
     // grid
-    double* gbuf = nullptr;
+    char* gbuf = nullptr;
     double* cbuf = nullptr;
     uint64_t cap_nodes = 0;
     uint32_t cap_axis = 0;
-    GridPtrs gp{};
+    GridPtrsT<double> gp{};
+    GridPtrsT<float> gf{};
     // step state
     StepState* S = nullptr;   // device
     StepState* Sh = nullptr;  // pinned host mirror
@@ -53,6 +57,17 @@ struct mpm_context {
 };
 
 namespace {
+// The typed views of a context's buffers for transfer type TF.
+template <class TF> PartStateT<TF>* layouts(mpm_context* c);
+template <> PartStateT<double>* layouts<double>(mpm_context* c) { return c->ps; }
+template <> PartStateT<float>* layouts<float>(mpm_context* c) { return c->pf; }
+template <class TF> GridPtrsT<TF>& grid_of(mpm_context* c);
+template <> GridPtrsT<double>& grid_of<double>(mpm_context* c) { return c->gp; }
+template <> GridPtrsT<float>& grid_of<float>(mpm_context* c) { return c->gf; }
+template <class TF> StagingT<TF>& staging_of(mpm_context* c);
+template <> StagingT<double>& staging_of<double>(mpm_context* c) { return c->stg; }
+template <> StagingT<float>& staging_of<float>(mpm_context* c) { return c->stgf; }
+
 const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p_usl", "k_g2p_musl",
                                     "k_musl_scatter", "k_musl_update", "k_defgrad", "k_finish", "k_bin_count", "k_bin_scan",
                                     "k_bin_scatter"};
@@ -130,16 +145,20 @@ size_t tile_nodes(int D, int spl) {
     return D == 3 ? (spl == 2 ? (size_t)BG<3, 2>::NT : (size_t)BG<3, 3>::NT) : (spl == 2 ? (size_t)BG<2, 2>::NT : (size_t)BG<2, 3>::NT);
 }
 
-uint32_t particle_fields(int D) { return D + D + D * D * 2 + (D == 3 ? 6 : 3) + 6; }  // x, v, B, F, tau, scalars
+// Bytes of one particle layout: double x, F and six plastic scalars; TF v, B and the symmetric tau.
+size_t layout_bytes(int D, bool split) {
+    const size_t nd = D + D * D + 6, nt = D + D * D + (D == 3 ? 6 : 3);
+    return nd * sizeof(double) + nt * (split ? sizeof(float) : sizeof(double));
+}
 
 int alloc_particles(mpm_context* c, uint32_t n) {
     const int D = c->cfg.dim;
-    size_t need = (size_t)2 * particle_fields(D) * n;
-    size_t ineed = (size_t)4 * n;  // id x2, key, perm
-    if (!(c->pbuf && c->pbuf_n >= need)) {
+    const size_t need = 2 * layout_bytes(D, c->split) * n + 64;
+    const size_t ineed = (size_t)4 * n;  // id x2, key, perm
+    if (!(c->pbuf && c->pbuf_bytes >= need)) {
         if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
-        if (hipMalloc(&c->pbuf, need * sizeof(double)) != hipSuccess) { c->pbuf = nullptr; c->pbuf_n = 0; return MPM_ERR_OUT_OF_MEMORY; }
-        c->pbuf_n = need;
+        if (hipMalloc(&c->pbuf, need) != hipSuccess) { c->pbuf = nullptr; c->pbuf_bytes = 0; return MPM_ERR_OUT_OF_MEMORY; }
+        c->pbuf_bytes = need;
     }
     if (!(c->ibuf && c->ibuf_n >= ineed)) {
         if (c->ibuf) { (void)hipFree(c->ibuf); c->ibuf = nullptr; }
@@ -149,20 +168,22 @@ int alloc_particles(mpm_context* c, uint32_t n) {
     return 0;
 }
 
+template <class TF>
 void layout_particles(mpm_context* c, uint32_t n) {
     const int D = c->cfg.dim;
-    double* q = c->pbuf;
-    auto take = [&]() { double* r = q; q += n; return r; };
+    char* q = c->pbuf;
+    auto taked = [&]() { double* r = (double*)q; q += sizeof(double) * n; return r; };
+    auto taket = [&]() { TF* r = (TF*)q; q += sizeof(TF) * n; return r; };
     for (int s = 0; s < 2; ++s) {
-        PartState ps{};
-        for (int d = 0; d < D; ++d) ps.x[d] = take();
-        for (int d = 0; d < D; ++d) ps.v[d] = take();
-        for (int i = 0; i < D * D; ++i) ps.B[i] = take();
-        for (int i = 0; i < D * D; ++i) ps.F[i] = take();
-        for (int i = 0; i < (D == 3 ? 6 : 3); ++i) ps.tau[i] = take();
-        ps.eps_dev = take(); ps.eps_vol = take(); ps.eps_prad = take(); ps.dgamma = take(); ps.visc = take(); ps.muI = take();
+        PartStateT<TF> ps{};
+        for (int d = 0; d < D; ++d) ps.x[d] = taked();
+        for (int i = 0; i < D * D; ++i) ps.F[i] = taked();
+        ps.eps_dev = taked(); ps.eps_vol = taked(); ps.eps_prad = taked(); ps.dgamma = taked(); ps.visc = taked(); ps.muI = taked();
+        for (int d = 0; d < D; ++d) ps.v[d] = taket();
+        for (int i = 0; i < D * D; ++i) ps.B[i] = taket();
+        for (int i = 0; i < (D == 3 ? 6 : 3); ++i) ps.tau[i] = taket();
         ps.id = c->ibuf + (size_t)s * n;
-        c->ps[s] = ps;
+        layouts<TF>(c)[s] = ps;
     }
     c->sb.key = c->ibuf + (size_t)2 * n;
     c->sb.perm = c->ibuf + (size_t)3 * n;
@@ -188,11 +209,13 @@ int ensure_grid(mpm_context* c, uint32_t batch) {
         if (want_blocks * nt > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
         if (hipMalloc(&c->bbuf, (size_t)3 * want_blocks * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
         if (hipMalloc(&c->cellbuf, (size_t)65 * want_blocks * sizeof(uint16_t)) != hipSuccess) { c->cellbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
-        if (hipMalloc(&c->stgbuf, (size_t)kMaxNV * nt * want_blocks * sizeof(double)) != hipSuccess) { c->stgbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        const size_t tf = c->split ? sizeof(float) : sizeof(double);
+        if (hipMalloc(&c->stgbuf, (size_t)kMaxNV * nt * want_blocks * tf) != hipSuccess) { c->stgbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
         c->cap_blocks = (uint32_t)want_blocks;
         c->sb.count = c->bbuf; c->sb.offset = c->bbuf + want_blocks; c->sb.cursor = c->bbuf + 2 * want_blocks;
         c->sb.cell_off = c->cellbuf;
-        c->stg.base = c->stgbuf; c->stg.stride = (uint32_t)(nt * want_blocks);
+        c->stg.base = (double*)c->stgbuf; c->stg.stride = (uint32_t)(nt * want_blocks);
+        c->stgf.base = (float*)c->stgbuf; c->stgf.stride = (uint32_t)(nt * want_blocks);
     }
     if (want_axis <= c->cap_axis && want_nodes <= c->cap_nodes) return 0;
     if (want_nodes > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
@@ -201,17 +224,21 @@ int ensure_grid(mpm_context* c, uint32_t batch) {
     if (c->gbuf) { (void)hipFree(c->gbuf); c->gbuf = nullptr; }
     if (c->cbuf) { (void)hipFree(c->cbuf); c->cbuf = nullptr; }
     const int nf = 1 + 2 * D + 1;  // mass, v, flip, fric
-    if (hipMalloc(&c->gbuf, (size_t)nf * want_nodes * sizeof(double)) != hipSuccess) { c->gbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
+    const size_t tf = c->split ? sizeof(float) : sizeof(double);
+    if (hipMalloc(&c->gbuf, (size_t)nf * want_nodes * tf) != hipSuccess) { c->gbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
     if (hipMalloc(&c->cbuf, (size_t)3 * want_axis * sizeof(double)) != hipSuccess) { c->cbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
-    double* q = c->gbuf;
-    auto take = [&]() { double* r = q; q += want_nodes; return r; };
-    GridPtrs g{};
-    g.mass = take();
-    g.fric = take();
-    for (int d = 0; d < D; ++d) g.v[d] = take();
-    for (int d = 0; d < D; ++d) g.flip[d] = take();
-    for (int d = 0; d < 3; ++d) g.coord[d] = c->cbuf + (size_t)d * want_axis;
-    c->gp = g;
+    auto carve = [&](auto& g) {
+        using TF = std::remove_pointer_t<decltype(g.mass)>;
+        TF* q = (TF*)c->gbuf;
+        auto take = [&]() { TF* r = q; q += want_nodes; return r; };
+        g.mass = take();
+        g.fric = take();
+        for (int d = 0; d < D; ++d) g.v[d] = take();
+        for (int d = 0; d < D; ++d) g.flip[d] = take();
+        for (int d = 0; d < 3; ++d) g.coord[d] = c->cbuf + (size_t)d * want_axis;
+    };
+    carve(c->gp);
+    carve(c->gf);
     c->cap_axis = want_axis; c->cap_nodes = want_nodes;
     return 0;
 }
@@ -225,59 +252,172 @@ __global__ void k_control(StepState* S, uint32_t stop_at_frame_end, uint32_t che
 }
 
 // P2G and the grid update for one choice of separate force (flip needed or conservation checked) and MIBF.
-template <int D, int SPL, bool SEP, bool MIBF>
-void enqueue_p2g(mpm_context* c, const PartState& src, const PartState& dst, uint32_t gblocks) {
-    constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF>::NV>::TB;
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+void enqueue_p2g(mpm_context* c, const PartStateT<TF>& src, const PartStateT<TF>& dst, uint32_t gblocks) {
+    constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB;
     hipStream_t s = c->stream;
-    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL, SEP, MIBF><<<c->cap_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, c->gp, c->stg); }
-    { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, c->gp, c->stg); }
+    const GridPtrsT<TF>& g = grid_of<TF>(c);
+    const StagingT<TF>& stg = staging_of<TF>(c);
+    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL, SEP, MIBF, TF><<<c->cap_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg); }
+    { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF, TF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, g, stg); }
 }
 
-template <int D, int SPL>
+template <int D, int SPL, class TF>
 void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     hipStream_t s = c->stream;
     const Params& P = c->P;
-    const PartState& src = c->ps[xsel & 1];
-    const PartState& dst = c->ps[(xsel + 1) & 1];
+    const PartStateT<TF>& src = layouts<TF>(c)[xsel & 1];
+    const PartStateT<TF>& dst = layouts<TF>(c)[(xsel + 1) & 1];
+    const GridPtrsT<TF>& g = grid_of<TF>(c);
+    const StagingT<TF>& stg = staging_of<TF>(c);
     const uint32_t n = c->n;
     const uint32_t pblocks = (n + kBlock - 1) / kBlock;
     const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 8192);
     const uint32_t zblocks = (c->cap_blocks + kBlock - 1) / kBlock;
-    { LaunchTimer t(c, KS_SETUP); k_setup<D><<<1, 64, 0, s>>>(P, c->S, c->gp); }
+    { LaunchTimer t(c, KS_SETUP); k_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g); }
     { LaunchTimer t(c, KS_ZERO); k_zero_grid<<<zblocks, kBlock, 0, s>>>(c->S, c->sb); }
-    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, c->gp, n); }
+    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, g, n); }
     { LaunchTimer t(c, KS_BIN_SCAN); k_bin_scan<<<1, 1024, 0, s>>>(c->S, c->sb); }
     { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<<<pblocks, kBlock, 0, s>>>(c->S, c->sb, n); }
     const bool sep = P.flip_ratio >= -1 || check_cons;
     c->flip_formed = sep;
-    if (sep && P.use_mibf) enqueue_p2g<D, SPL, true, true>(c, src, dst, gblocks);
-    else if (sep) enqueue_p2g<D, SPL, true, false>(c, src, dst, gblocks);
-    else if (P.use_mibf) enqueue_p2g<D, SPL, false, true>(c, src, dst, gblocks);
-    else enqueue_p2g<D, SPL, false, false>(c, src, dst, gblocks);
+    if (sep && P.use_mibf) enqueue_p2g<D, SPL, true, true, TF>(c, src, dst, gblocks);
+    else if (sep) enqueue_p2g<D, SPL, true, false, TF>(c, src, dst, gblocks);
+    else if (P.use_mibf) enqueue_p2g<D, SPL, false, true, TF>(c, src, dst, gblocks);
+    else enqueue_p2g<D, SPL, false, false, TF>(c, src, dst, gblocks);
     if (P.use_musl) {
-        constexpr int TBM = ScatterShape<D, SPL, D>::TB;
-        { LaunchTimer t(c, KS_G2P_MUSL); k_g2p_musl<D, SPL><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp); }
-        { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL><<<c->cap_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, c->stg); }
-        { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, c->gp, c->stg); }
-        { LaunchTimer t(c, KS_DEFGRAD); k_defgrad<D, SPL><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp, n); }
+        constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
+        { LaunchTimer t(c, KS_G2P_MUSL); k_g2p_musl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g); }
+        { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->cap_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
+        { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL, TF><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, g, stg); }
+        { LaunchTimer t(c, KS_DEFGRAD); k_defgrad<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g, n); }
     } else {
-        LaunchTimer t(c, KS_G2P_USL); k_g2p_usl<D, SPL><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, c->gp);
+        LaunchTimer t(c, KS_G2P_USL); k_g2p_usl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
     }
     { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
 }
 
-void enqueue(mpm_context* c, uint32_t xsel, bool check_cons) {
+template <class TF>
+void enqueue_tf(mpm_context* c, uint32_t xsel, bool check_cons) {
     const int D = c->cfg.dim, SPL = c->cfg.spline_degree;
-    if (D == 3 && SPL == 2) enqueue_step<3, 2>(c, xsel, check_cons);
-    else if (D == 3 && SPL == 3) enqueue_step<3, 3>(c, xsel, check_cons);
-    else if (D == 2 && SPL == 2) enqueue_step<2, 2>(c, xsel, check_cons);
-    else enqueue_step<2, 3>(c, xsel, check_cons);
+    if (D == 3 && SPL == 2) enqueue_step<3, 2, TF>(c, xsel, check_cons);
+    else if (D == 3 && SPL == 3) enqueue_step<3, 3, TF>(c, xsel, check_cons);
+    else if (D == 2 && SPL == 2) enqueue_step<2, 2, TF>(c, xsel, check_cons);
+    else enqueue_step<2, 3, TF>(c, xsel, check_cons);
+}
+
+void enqueue(mpm_context* c, uint32_t xsel, bool check_cons) {
+    if (c->split) enqueue_tf<float>(c, xsel, check_cons);
+    else enqueue_tf<double>(c, xsel, check_cons);
 }
 
 int sync_state(mpm_context* c) {
     HIP_TRY(hipMemcpyAsync(c->Sh, c->S, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+// Upload of the caller's particle arrays (interleaved dim / dim*dim per particle) into layout 0.
+template <class TF>
+int upload_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in) {
+    const int D = c->cfg.dim;
+    layout_particles<TF>(c, n);
+    std::vector<double> col(n);
+    std::vector<TF> colt(n);
+    auto up = [&](double* dst) { return hipMemcpy(dst, col.data(), sizeof(double) * n, hipMemcpyHostToDevice); };
+    auto upt = [&](TF* dst) {
+        for (uint32_t p = 0; p < n; ++p) colt[p] = (TF)col[p];
+        return hipMemcpy(dst, colt.data(), sizeof(TF) * n, hipMemcpyHostToDevice);
+    };
+    const PartStateT<TF>& L0 = layouts<TF>(c)[0];
+    for (int d = 0; d < D; ++d) {
+        for (uint32_t p = 0; p < n; ++p) col[p] = in->x[(size_t)p * D + d];
+        HIP_TRY(up(L0.x[d]));
+        for (uint32_t p = 0; p < n; ++p) col[p] = in->v ? in->v[(size_t)p * D + d] : 0.0;
+        HIP_TRY(upt(L0.v[d]));
+    }
+    for (int i = 0; i < D * D; ++i) {
+        const bool diag = (i % (D + 1)) == 0;
+        for (uint32_t p = 0; p < n; ++p) col[p] = in->F ? in->F[(size_t)p * D * D + i] : (diag ? 1.0 : 0.0);
+        HIP_TRY(up(L0.F[i]));
+        for (uint32_t p = 0; p < n; ++p) col[p] = in->Bmat ? in->Bmat[(size_t)p * D * D + i] : 0.0;
+        HIP_TRY(upt(L0.B[i]));
+    }
+    const double* sc[6] = {in->eps_pl_dev, in->eps_pl_vol, in->eps_pl_vol_pradhana, in->delta_gamma, in->viscosity, in->muI};
+    double* dst[6] = {L0.eps_dev, L0.eps_vol, L0.eps_prad, L0.dgamma, L0.visc, L0.muI};
+    for (int k = 0; k < 6; ++k) {
+        for (uint32_t p = 0; p < n; ++p) col[p] = sc[k] ? sc[k][p] : 0.0;
+        if (k == 5 && c->cfg.use_mibf) {  // simulate() fills muI when use_mibf (simulation.cpp:138-145)
+            const double fill = (c->cfg.plastic_model == MPM_PLASTIC_DPMUI || c->cfg.plastic_model == MPM_PLASTIC_MCCMUI) ? c->cfg.mu_1 : c->cfg.M;
+            std::fill(col.begin(), col.end(), fill);
+        }
+        HIP_TRY(up(dst[k]));
+    }
+    return 0;
+}
+
+// Download of the current layout into the caller's order (particle ids).
+template <class TF>
+int download_particles(mpm_context* c, mpm_particle_arrays* out) {
+    const int D = c->cfg.dim;
+    const uint32_t n = c->n;
+    const PartStateT<TF>& L = layouts<TF>(c)[c->Sh->xsel & 1];
+    std::vector<uint32_t> id(n);
+    HIP_TRY(hipMemcpy(id.data(), L.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    for (uint32_t p = 0; p < n; ++p) if (id[p] >= n) return MPM_ERR_HIP;
+    std::vector<double> col(n);
+    std::vector<TF> colt(n);
+    auto down = [&](const double* src) { return hipMemcpy(col.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost); };
+    auto downt = [&](const TF* src) {
+        hipError_t e = hipMemcpy(colt.data(), src, sizeof(TF) * n, hipMemcpyDeviceToHost);
+        for (uint32_t p = 0; p < n; ++p) col[p] = (double)colt[p];
+        return e;
+    };
+    for (int d = 0; d < D; ++d) {
+        if (out->x) { HIP_TRY(down(L.x[d])); for (uint32_t p = 0; p < n; ++p) out->x[(size_t)id[p] * D + d] = col[p]; }
+        if (out->v) { HIP_TRY(downt(L.v[d])); for (uint32_t p = 0; p < n; ++p) out->v[(size_t)id[p] * D + d] = col[p]; }
+    }
+    for (int i = 0; i < D * D; ++i) {
+        if (out->F) { HIP_TRY(down(L.F[i])); for (uint32_t p = 0; p < n; ++p) out->F[(size_t)id[p] * D * D + i] = col[p]; }
+        if (out->Bmat) { HIP_TRY(downt(L.B[i])); for (uint32_t p = 0; p < n; ++p) out->Bmat[(size_t)id[p] * D * D + i] = col[p]; }
+    }
+    double* dsts[6] = {out->eps_pl_dev, out->eps_pl_vol, out->eps_pl_vol_pradhana, out->delta_gamma, out->viscosity, out->muI};
+    const double* srcs[6] = {L.eps_dev, L.eps_vol, L.eps_prad, L.dgamma, L.visc, L.muI};
+    for (int k = 0; k < 6; ++k) {
+        if (!dsts[k]) continue;
+        HIP_TRY(down(srcs[k]));
+        for (uint32_t p = 0; p < n; ++p) dsts[k][id[p]] = col[p];
+    }
+    return 0;
+}
+
+// Download of grid node arrays (converted to double).
+template <class TF>
+int download_grid(mpm_context* c, uint32_t nn, double* mass, double* v, double* flip, double* friction) {
+    const int D = c->cfg.dim;
+    const GridPtrsT<TF>& g = grid_of<TF>(c);
+    std::vector<TF> col(nn);
+    auto down = [&](const TF* src) { return hipMemcpy(col.data(), src, sizeof(TF) * nn, hipMemcpyDeviceToHost); };
+    if (mass) { HIP_TRY(down(g.mass)); for (uint32_t l = 0; l < nn; ++l) mass[l] = (double)col[l]; }
+    if (friction && c->cfg.use_mibf) { HIP_TRY(down(g.fric)); for (uint32_t l = 0; l < nn; ++l) friction[l] = (double)col[l]; }
+    for (int d = 0; d < D; ++d) {
+        if (v) { HIP_TRY(down(g.v[d])); for (uint32_t l = 0; l < nn; ++l) v[(size_t)l * D + d] = (double)col[l]; }
+        if (flip && !c->flip_formed) { for (uint32_t l = 0; l < nn; ++l) flip[(size_t)l * D + d] = 0.0; }
+        else if (flip) { HIP_TRY(down(g.flip[d])); for (uint32_t l = 0; l < nn; ++l) flip[(size_t)l * D + d] = (double)col[l]; }
+    }
+    return 0;
+}
+
+template <class TF>
+void launch_binning(mpm_context* c, uint32_t n, int32_t* dbase, uint32_t* dkey) {
+    const PartStateT<TF>& L = layouts<TF>(c)[c->Sh->xsel & 1];
+    const GridPtrsT<TF>& g = grid_of<TF>(c);
+    const uint32_t pblocks = (n + kBlock - 1) / kBlock;
+    const int D = c->cfg.dim, spl = c->cfg.spline_degree;
+    if (D == 3 && spl == 2) k_binning<3, 2, TF><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, g, n, dbase, dkey);
+    else if (D == 3) k_binning<3, 3, TF><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, g, n, dbase, dkey);
+    else if (spl == 2) k_binning<2, 2, TF><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, g, n, dbase, dkey);
+    else k_binning<2, 3, TF><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, g, n, dbase, dkey);
 }
 
 }  // namespace
@@ -306,7 +446,7 @@ int mpm_create(const mpm_config* cfg, mpm_context** out) {
     const mpm_config& c = *cfg;
     if (c.dim != 2 && c.dim != 3) return MPM_ERR_INVALID_ARGUMENT;
     if (c.spline_degree != 2 && c.spline_degree != 3) return MPM_ERR_UNSUPPORTED;
-    if (c.precision != MPM_PRECISION_FP64) return MPM_ERR_UNSUPPORTED;
+    if (c.precision != MPM_PRECISION_FP64 && c.precision != MPM_PRECISION_SPLIT) return MPM_ERR_INVALID_ARGUMENT;
     if (c.pbc) return MPM_ERR_UNSUPPORTED;
     if (!valid_enum(c.elastic_model, 0, 1) || !valid_enum(c.plastic_model, 0, 10) || !valid_enum(c.hardening_law, 0, 4))
         return MPM_ERR_INVALID_ARGUMENT;
@@ -318,6 +458,7 @@ int mpm_create(const mpm_config* cfg, mpm_context** out) {
     mpm_context* ctx = new (std::nothrow) mpm_context();
     if (!ctx) return MPM_ERR_OUT_OF_MEMORY;
     ctx->cfg = c;
+    ctx->split = c.precision == MPM_PRECISION_SPLIT;
     derive_params(c, ctx->P);
     (void)hipGetDevice(&ctx->device);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MPM_ERR_HIP; }
@@ -389,34 +530,8 @@ int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in)
     int rc = alloc_particles(c, n);
     if (rc) return rc;
     c->n = n;
-    layout_particles(c, n);
-    // structure-of-arrays staging: one host vector per device column
-    std::vector<double> col(n);
-    auto up = [&](double* dst) { return hipMemcpy(dst, col.data(), sizeof(double) * n, hipMemcpyHostToDevice); };
-    const PartState& L0 = c->ps[0];
-    for (int d = 0; d < D; ++d) {
-        for (uint32_t p = 0; p < n; ++p) col[p] = in->x[(size_t)p * D + d];
-        HIP_TRY(up(L0.x[d]));
-        for (uint32_t p = 0; p < n; ++p) col[p] = in->v ? in->v[(size_t)p * D + d] : 0.0;
-        HIP_TRY(up(L0.v[d]));
-    }
-    for (int i = 0; i < D * D; ++i) {
-        const bool diag = (i % (D + 1)) == 0;
-        for (uint32_t p = 0; p < n; ++p) col[p] = in->F ? in->F[(size_t)p * D * D + i] : (diag ? 1.0 : 0.0);
-        HIP_TRY(up(L0.F[i]));
-        for (uint32_t p = 0; p < n; ++p) col[p] = in->Bmat ? in->Bmat[(size_t)p * D * D + i] : 0.0;
-        HIP_TRY(up(L0.B[i]));
-    }
-    const double* sc[6] = {in->eps_pl_dev, in->eps_pl_vol, in->eps_pl_vol_pradhana, in->delta_gamma, in->viscosity, in->muI};
-    double* dst[6] = {L0.eps_dev, L0.eps_vol, L0.eps_prad, L0.dgamma, L0.visc, L0.muI};
-    for (int k = 0; k < 6; ++k) {
-        for (uint32_t p = 0; p < n; ++p) col[p] = sc[k] ? sc[k][p] : 0.0;
-        if (k == 5 && c->cfg.use_mibf) {  // simulate() fills muI when use_mibf (simulation.cpp:138-145)
-            const double fill = (c->cfg.plastic_model == MPM_PLASTIC_DPMUI || c->cfg.plastic_model == MPM_PLASTIC_MCCMUI) ? c->cfg.mu_1 : c->cfg.M;
-            std::fill(col.begin(), col.end(), fill);
-        }
-        HIP_TRY(up(dst[k]));
-    }
+    rc = c->split ? upload_particles<float>(c, n, in) : upload_particles<double>(c, n, in);
+    if (rc) return rc;
     // initial grid estimate from the particle bounds
     for (int d = 0; d < D; ++d) {
         double mn = in->x[d], mx = in->x[d];
@@ -437,8 +552,13 @@ int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in)
     S.n[0] = S.n[1] = S.n[2] = 1;
     HIP_TRY(hipMemcpy(c->S, &S, sizeof(StepState), hipMemcpyHostToDevice));
     const uint32_t pblocks = (n + kBlock - 1) / kBlock;
-    if (D == 3) k_prepare<3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, c->ps[0], n);
-    else k_prepare<2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, c->ps[0], n);
+    if (c->split) {
+        if (D == 3) k_prepare<3, float><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, c->pf[0], n);
+        else k_prepare<2, float><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, c->pf[0], n);
+    } else {
+        if (D == 3) k_prepare<3, double><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, c->ps[0], n);
+        else k_prepare<2, double><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, c->ps[0], n);
+    }
     HIP_TRY(hipGetLastError());
     rc = sync_state(c);
     if (rc) return rc;
@@ -493,31 +613,8 @@ uint32_t mpm_num_particles(const mpm_context* c) { return c ? c->n : 0; }
 int mpm_get_particles(mpm_context* c, mpm_particle_arrays* out) {
     if (!c || !out) return MPM_ERR_INVALID_ARGUMENT;
     if (!c->ready) return MPM_ERR_NOT_READY;
-    const int D = c->cfg.dim;
-    const uint32_t n = c->n;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const PartState& L = c->ps[c->Sh->xsel & 1];
-    std::vector<uint32_t> id(n);
-    HIP_TRY(hipMemcpy(id.data(), L.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    for (uint32_t p = 0; p < n; ++p) if (id[p] >= n) return MPM_ERR_HIP;
-    std::vector<double> col(n);
-    auto down = [&](const double* src) { return hipMemcpy(col.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost); };
-    for (int d = 0; d < D; ++d) {
-        if (out->x) { HIP_TRY(down(L.x[d])); for (uint32_t p = 0; p < n; ++p) out->x[(size_t)id[p] * D + d] = col[p]; }
-        if (out->v) { HIP_TRY(down(L.v[d])); for (uint32_t p = 0; p < n; ++p) out->v[(size_t)id[p] * D + d] = col[p]; }
-    }
-    for (int i = 0; i < D * D; ++i) {
-        if (out->F) { HIP_TRY(down(L.F[i])); for (uint32_t p = 0; p < n; ++p) out->F[(size_t)id[p] * D * D + i] = col[p]; }
-        if (out->Bmat) { HIP_TRY(down(L.B[i])); for (uint32_t p = 0; p < n; ++p) out->Bmat[(size_t)id[p] * D * D + i] = col[p]; }
-    }
-    double* dsts[6] = {out->eps_pl_dev, out->eps_pl_vol, out->eps_pl_vol_pradhana, out->delta_gamma, out->viscosity, out->muI};
-    const double* srcs[6] = {L.eps_dev, L.eps_vol, L.eps_prad, L.dgamma, L.visc, L.muI};
-    for (int k = 0; k < 6; ++k) {
-        if (!dsts[k]) continue;
-        HIP_TRY(down(srcs[k]));
-        for (uint32_t p = 0; p < n; ++p) dsts[k][id[p]] = col[p];
-    }
-    return 0;
+    return c->split ? download_particles<float>(c, out) : download_particles<double>(c, out);
 }
 
 int mpm_get_grid(mpm_context* c, mpm_grid_info* info, double* x, double* y, double* z, double* mass, double* v,
@@ -535,15 +632,7 @@ int mpm_get_grid(mpm_context* c, mpm_grid_info* info, double* x, double* y, doub
     if (nn == 0 || !c->gbuf) return 0;
     double* axes[3] = {x, y, z};
     for (int d = 0; d < D; ++d) if (axes[d]) HIP_TRY(hipMemcpy(axes[d], c->gp.coord[d], sizeof(double) * S.n[d], hipMemcpyDeviceToHost));
-    if (mass) HIP_TRY(hipMemcpy(mass, c->gp.mass, sizeof(double) * nn, hipMemcpyDeviceToHost));
-    if (friction && c->cfg.use_mibf) HIP_TRY(hipMemcpy(friction, c->gp.fric, sizeof(double) * nn, hipMemcpyDeviceToHost));
-    std::vector<double> col(nn);
-    for (int d = 0; d < D; ++d) {
-        if (v) { HIP_TRY(hipMemcpy(col.data(), c->gp.v[d], sizeof(double) * nn, hipMemcpyDeviceToHost)); for (uint32_t l = 0; l < nn; ++l) v[(size_t)l * D + d] = col[l]; }
-        if (flip && !c->flip_formed) { for (uint32_t l = 0; l < nn; ++l) flip[(size_t)l * D + d] = 0.0; }
-        else if (flip) { HIP_TRY(hipMemcpy(col.data(), c->gp.flip[d], sizeof(double) * nn, hipMemcpyDeviceToHost)); for (uint32_t l = 0; l < nn; ++l) flip[(size_t)l * D + d] = col[l]; }
-    }
-    return 0;
+    return c->split ? download_grid<float>(c, nn, mass, v, flip, friction) : download_grid<double>(c, nn, mass, v, flip, friction);
 }
 
 int mpm_get_plates(mpm_context* c, mpm_plate* out, uint32_t cap, uint32_t* count) {
@@ -566,22 +655,18 @@ int mpm_get_binning(mpm_context* c, int32_t* cell_base, uint32_t* block_key) {
     const int D = c->cfg.dim;
     const uint32_t n = c->n;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const PartState& L = c->ps[c->Sh->xsel & 1];
     int32_t* dbase = nullptr; uint32_t* dkey = nullptr;
     HIP_TRY(hipMalloc(&dbase, sizeof(int32_t) * (size_t)n * D));
     if (hipMalloc(&dkey, sizeof(uint32_t) * n) != hipSuccess) { (void)hipFree(dbase); return MPM_ERR_OUT_OF_MEMORY; }
-    const uint32_t pblocks = (n + kBlock - 1) / kBlock;
-    const int spl = c->cfg.spline_degree;
-    if (D == 3 && spl == 2) k_binning<3, 2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, c->gp, n, dbase, dkey);
-    else if (D == 3) k_binning<3, 3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, c->gp, n, dbase, dkey);
-    else if (spl == 2) k_binning<2, 2><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, c->gp, n, dbase, dkey);
-    else k_binning<2, 3><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, c->gp, n, dbase, dkey);
+    if (c->split) launch_binning<float>(c, n, dbase, dkey);
+    else launch_binning<double>(c, n, dbase, dkey);
+    const uint32_t* ids = c->split ? c->pf[c->Sh->xsel & 1].id : c->ps[c->Sh->xsel & 1].id;
     std::vector<int32_t> b((size_t)n * D); std::vector<uint32_t> kk(n), id(n);
     int rc = 0;
     if (hipStreamSynchronize(c->stream) != hipSuccess ||
         hipMemcpy(b.data(), dbase, sizeof(int32_t) * b.size(), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(kk.data(), dkey, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(id.data(), L.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess) rc = MPM_ERR_HIP;
+        hipMemcpy(id.data(), ids, sizeof(uint32_t) * n, hipMemcpyDeviceToHost) != hipSuccess) rc = MPM_ERR_HIP;
     (void)hipFree(dbase); (void)hipFree(dkey);
     if (rc) return rc;
     for (uint32_t p = 0; p < n; ++p) {

@@ -39,13 +39,15 @@
 
 namespace mpm {
 
-// One layout of the particle state (structure of arrays).
-struct PartState {
+// One layout of the particle state (structure of arrays). TF is the type of the transfer quantities
+// (velocity, affine matrix, Kirchhoff stress): double in the plain form, float in the split form. Position,
+// deformation gradient and the plastic state are double in both.
+template <class TF> struct PartStateT {
     double* x[3];
-    double* v[3];
-    double* B[9];
+    TF* v[3];
+    TF* B[9];
     double* F[9];
-    double* tau[6];
+    TF* tau[6];
     double* eps_dev; double* eps_vol; double* eps_prad; double* dgamma; double* visc; double* muI;
     uint32_t* id;    // index of the particle in the caller's order
 };
@@ -60,18 +62,18 @@ struct SortBufs {
     uint16_t* cell_off;  // per block: first sorted slot (relative to offset) of each dual cell, 65 entries
 };
 
-struct GridPtrs {
-    double* mass;    // m_i
-    double* v[3];    // node velocity after the update (MUSL: after the re-scatter)
-    double* flip[3]; // v_new - v_old (formed when flip_ratio >= -1 or conservation is checked)
-    double* fric;    // MIBF friction coefficient
+template <class TF> struct GridPtrsT {
+    TF* mass;        // m_i
+    TF* v[3];        // node velocity after the update (MUSL: after the re-scatter)
+    TF* flip[3];     // v_new - v_old (formed when flip_ratio >= -1 or conservation is checked)
+    TF* fric;        // MIBF friction coefficient
     double* coord[3];
 };
 
 // Per-block node tiles written by the scatter kernels: value v of tile node q of block b is at
 // base[v * stride + b * NT + q].
-struct Staging {
-    double* base;
+template <class TF> struct StagingT {
+    TF* base;
     uint32_t stride;
 };
 
@@ -277,6 +279,24 @@ __device__ __forceinline__ bool make_stencil(Stencil<D, SPL>& st, const double* 
     return true;
 }
 
+// First node of the non-zero window per axis, as make_stencil chooses it (for the quadratic spline the
+// base node is skipped when its weight is exactly zero); false if the 4-node window leaves the grid.
+template <int D, int SPL>
+__device__ __forceinline__ bool window_base(int* wb, const double* xp, const double* org, const uint32_t* n, double* const* coord,
+                                            double h1, bool clamp) {
+    int b[3] = {0, 0, 0};
+    if (!stencil_base<D>(b, xp, org, n, h1, clamp)) return false;
+    for (int d = 0; d < D; ++d) {
+        int s = 0;
+        if (SPL == 2) {
+            const double t = (xp[d] - coord[d][b[d]]) * h1;
+            s = BSpline<2>::N(t) == 0.0 ? 1 : 0;
+        }
+        wb[d] = b[d] + s;
+    }
+    return true;
+}
+
 __device__ __forceinline__ uint32_t node_index3(int i, int j, int k, const uint32_t* n) { return ((uint32_t)i * n[1] + (uint32_t)j) * n[2] + (uint32_t)k; }
 
 template <int D>
@@ -288,8 +308,8 @@ __device__ __forceinline__ uint32_t node_index(int i, int j, int k, const uint32
 // ----------------------------------------------------------------------------------------------
 // k_prepare: Kirchhoff stress of the initial F and the reductions that the first step needs.
 // ----------------------------------------------------------------------------------------------
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, PartState ps, uint32_t n) {
+template <int D, class TF>
+__global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, PartStateT<TF> ps, uint32_t n) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = p < n;
     double x[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -303,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, Part
             V<D> ls; for (int i = 0; i < D; ++i) ls[i] = log(fabs(s[i]));
             hencky_tau<D>(U, ls, P.mu, P.lambda, tau);
         }
-        for (int i = 0; i < SymPack<D>::n; ++i) ps.tau[i][p] = tau[i];
+        for (int i = 0; i < SymPack<D>::n; ++i) ps.tau[i][p] = (TF)tau[i];
         for (int d = 0; d < D; ++d) { x[d] = ps.x[d][p]; v[d] = ps.v[d][p]; }
         ps.id[p] = p;
     }
@@ -316,8 +336,8 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, Part
 // k_setup: updateDt (update_dt.cpp:5-66) and remeshFixedInit/Cont (remesh.cpp:38-311).
 // One workgroup of 64 lanes; lane d builds the node coordinates of axis d (arange, tools.hpp:291-296).
 // ----------------------------------------------------------------------------------------------
-template <int D>
-__global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrs g) {
+template <int D, class TF>
+__global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrsT<TF> g) {
     __shared__ double lo_s[3], stop_s[3];
     __shared__ int go;
     fold_bound_slots<D>(S);
@@ -437,23 +457,23 @@ __device__ __forceinline__ void block_coords(uint32_t b, const StepState* S, int
 
 // Block of a particle from the first node of its non-zero window (P2G's clamped base). A particle whose
 // window leaves the grid goes to block 0, where k_p2g reports it.
-template <int D, int SPL>
-__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrs& g, double h1) {
+template <int D, int SPL, class TF>
+__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrsT<TF>& g, double h1) {
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
-    Stencil<D, SPL> st;
-    if (!make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, true)) return 0u;
+    int wb[3];
+    if (!window_base<D, SPL>(wb, xp, org, nn, g.coord, h1, true)) return 0u;
     uint32_t c[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) {
-        const uint32_t b = (uint32_t)st.base[d] / BlockEdge<D>::v;
+        const uint32_t b = (uint32_t)wb[d] / BlockEdge<D>::v;
         c[d] = b < S->nb[d] ? b : S->nb[d] - 1;
     }
     return D == 3 ? (c[0] * S->nb[1] + c[1]) * S->nb[2] + c[2] : c[0] * S->nb[1] + c[1];
 }
 
 // k_bin_count: block of every particle and the histogram (wavefront-aggregated atomics).
-template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState* S, PartState src, SortBufs sb, GridPtrs g, uint32_t n) {
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState* S, PartStateT<TF> src, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
     if (S->stop) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = p < n;
@@ -461,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState*
     if (active) {
         double xp[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
-        key = block_key<D, SPL>(xp, S, g, P.one_over_dx);
+        key = block_key<D, SPL, TF>(xp, S, g, P.one_over_dx);
         sb.key[p] = key;
     }
     (void)wave_claim(sb.count, key, active);
@@ -500,41 +520,47 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, Sort
 }
 
 // k_binning: cell base index and block of every particle of a layout (diagnostic download).
-template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_binning(Params P, const StepState* S, PartState ps, GridPtrs g, uint32_t n,
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_binning(Params P, const StepState* S, PartStateT<TF> ps, GridPtrsT<TF> g, uint32_t n,
                                                      int32_t* base_out, uint32_t* key_out) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     double xp[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) xp[d] = ps.x[d][p];
     for (int d = 0; d < D; ++d) base_out[(size_t)p * D + d] = (int)floor((xp[d] - S->origin[d]) * P.one_over_dx) - 1;
-    key_out[p] = block_key<D, SPL>(xp, S, g, P.one_over_dx);
+    key_out[p] = block_key<D, SPL, TF>(xp, S, g, P.one_over_dx);
 }
 
 // ----------------------------------------------------------------------------------------------
-// Block scatter (k_p2g, k_musl_scatter). One workgroup per block. Worker thread (cell, slice, half) owns
-// the window nodes of its dual cell with first-axis offset `slice` (and, when the partial sums would not
-// fit in registers, half of the second-axis offsets); it adds the contributions of its cell's particles
-// in their sorted order. The partial sums then enter each tile node in the fixed order (a, b, c) of the
-// window offsets, and the tile goes to the staging area. Nothing depends on the timing of threads.
+// Block scatter (k_p2g, k_musl_scatter). One workgroup per block; lane l of each wavefront owns dual
+// cell l (a block has 64 dual cells). The W first-axis window offsets ("slices") are split into NG
+// groups of SPP slices, one wavefront per group (and per half of the second-axis offsets when the
+// partial sums would not fit in registers). A lane adds the contributions of its cell's particles, in
+// their sorted order, in registers; then the wavefronts add their partial sums into the block's LDS tile
+// one after another, every lane to a different node at each instruction. The sums have a fixed order,
+// and the tile goes to the staging area.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL, int NV> struct ScatterShape {
+template <int D, int SPL, int NV, class TF> struct ScatterShape {
     using G = BG<D, SPL>;
-    static constexpr int SPLIT = (G::NS * NV > 72) ? 2 : 1;   // halves of the second-axis offsets
-    static constexpr int OBN = G::W / SPLIT;                   // second-axis offsets per worker
-    static constexpr int NSH = D == 3 ? OBN * G::W : OBN;     // window nodes per worker
-    static constexpr int WORKERS = G::NC * G::W * SPLIT;
-    static constexpr int TB = WORKERS > 256 ? 512 : 256;       // workgroup size
+    static constexpr int REGS = 80;                                    // 32-bit registers for partial sums
+    static constexpr int WORDS = NV * (int)(sizeof(TF) / 4);
+    static constexpr int SPP0 = REGS / (G::NS * WORDS);
+    static constexpr int SPP = SPP0 < 1 ? 1 : (SPP0 > G::W ? G::W : SPP0);   // slices per wavefront
+    static constexpr int NG = (G::W + SPP - 1) / SPP;                  // slice groups
+    static constexpr int SPLIT = (SPP0 < 1 && G::W % 2 == 0) ? 2 : 1;  // halves of the second-axis offsets
+    static constexpr int OBN = G::W / SPLIT;                            // second-axis offsets per wavefront
+    static constexpr int NSH = D == 3 ? OBN * G::W : OBN;              // window nodes per slice per wavefront
+    static constexpr int WAVES = NG * SPLIT;
+    static constexpr int TB = 64 * WAVES;                               // workgroup size
 };
 
-template <int D, int SPL, int NV> struct ScatterLDS {
+template <int D, int SPL, int NV, class TF> struct ScatterLDS {
     using G = BG<D, SPL>;
     uint32_t p[kMaxBP];              // source slots of the block's particles, ordered by (cell, slot)
     uint32_t cstart[G::NC + 2];      // bucket c = cell c occupies p[cstart[c] .. cstart[c+1]); bucket NC: outside
     union {
         struct { uint32_t tmp[kMaxBP]; uint32_t ccount[G::NC + 1]; uint32_t ccur[G::NC + 1]; uint8_t cell[kMaxBP]; } sort;
-        double part[G::NS * NV * G::NC];  // partial sums of one slice: [window offset][value][cell]
-        double tile[G::NT * NV];          // blocks above kMaxBP particles: [value][node], LDS atomics
+        TF tile[G::NT * NV];         // the block's node sums, [value][node]
     } u;
 };
 
@@ -548,9 +574,9 @@ __device__ __forceinline__ int tile_index(int t0, int t1, int t2) {
 // Orders a block's particles by (dual cell, source slot) into L.p / L.cstart and writes the sorted
 // permutation, the destination ids and the cell offsets. Particles whose window is outside the block
 // (their window leaves the grid) go to the last bucket and are reported.
-template <int D, int SPL, int NV>
-__device__ uint32_t order_block(ScatterLDS<D, SPL, NV>& L, const Params& P, const StepState* S, const PartState& src,
-                                const PartState& dst, const SortBufs& sb, const GridPtrs& g, uint32_t b, uint32_t start,
+template <int D, int SPL, int NV, class TF>
+__device__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, const Params& P, const StepState* S, const PartStateT<TF>& src,
+                                const PartStateT<TF>& dst, const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b, uint32_t start,
                                 uint32_t cnt, const int* bc) {
     using G = BG<D, SPL>;
     const int tid = threadIdx.x;
@@ -564,12 +590,12 @@ __device__ uint32_t order_block(ScatterLDS<D, SPL, NV>& L, const Params& P, cons
         const uint32_t p = sb.perm[start + k];
         double xp[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
-        Stencil<D, SPL> st;
+        int wb[3];
         int cell = G::NC;
-        if (make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, true)) {
+        if (window_base<D, SPL>(wb, xp, org, nn, g.coord, h1, true)) {
             int l[3] = {0, 0, 0};
             bool in = true;
-            for (int d = 0; d < D; ++d) { l[d] = st.base[d] - G::E * bc[d]; in = in && l[d] >= 0 && l[d] < G::E; }
+            for (int d = 0; d < D; ++d) { l[d] = wb[d] - G::E * bc[d]; in = in && l[d] >= 0 && l[d] < G::E; }
             if (in) cell = D == 3 ? (l[0] * G::E + l[1]) * G::E + l[2] : l[0] * G::E + l[1];
         }
         if (cell == G::NC) err |= MPM_ERR_GRID_BOUNDS;
@@ -596,8 +622,8 @@ __device__ uint32_t order_block(ScatterLDS<D, SPL, NV>& L, const Params& P, cons
         L.p[slot] = L.u.sort.tmp[k];
     }
     __syncthreads();
-    if (tid <= G::NC) {  // insertion sort of each bucket by source slot
-        const uint32_t s0 = L.cstart[tid], s1 = L.cstart[tid + 1];
+    for (int c = tid; c <= G::NC; c += blockDim.x) {  // insertion sort of each bucket by source slot
+        const uint32_t s0 = L.cstart[c], s1 = L.cstart[c + 1];
         for (uint32_t i = s0 + 1; i < s1; ++i) {
             const uint32_t key = L.p[i];
             uint32_t j = i;
@@ -615,135 +641,145 @@ __device__ uint32_t order_block(ScatterLDS<D, SPL, NV>& L, const Params& P, cons
     return err;
 }
 
-// Ordered accumulation (see above) of the block's particles into its tile, written to staging.
-// contrib.load(p, i, part) fetches source particle p (sorted slot i); contrib.add(part, a, half, acc, base)
-// adds it to acc for window offsets (a, half) and returns false if its window leaves the grid.
-template <int D, int SPL, int NV, class Contrib>
-__device__ void scatter_block(ScatterLDS<D, SPL, NV>& L, const Contrib& contrib, const Staging& stg, uint32_t b,
-                              uint32_t start) {
+template <int D, int SPL, int NV, class TF>
+__device__ __forceinline__ void zero_tile(ScatterLDS<D, SPL, NV, TF>& L) {
+    for (int q = threadIdx.x; q < BG<D, SPL>::NT * NV; q += blockDim.x) L.u.tile[q] = TF(0);
+}
+
+template <int D, int SPL, int NV, class TF>
+__device__ __forceinline__ void write_tile(const ScatterLDS<D, SPL, NV, TF>& L, const StagingT<TF>& stg, uint32_t b) {
     using G = BG<D, SPL>;
-    using SS = ScatterShape<D, SPL, NV>;
-    const int tid = threadIdx.x;
-    double acc[SS::NSH][NV];
+    for (int q = threadIdx.x; q < G::NT; q += blockDim.x)
+        for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = L.u.tile[v * G::NT + q];
+}
+
+// Adds one wavefront's partial sums acc (slices a0 .. a0+SPP-1, second-axis half `half`) of dual cell
+// `cell` into the tile. At each instruction every lane adds to a different node, but a later instruction
+// may add to a node that another lane updated earlier, so the adds are LDS atomics: a plain load/add/store
+// would let the compiler move later loads above earlier stores (legal for one lane, a lost update across
+// lanes). A wavefront's LDS instructions execute in program order, so each node still receives its terms
+// in a fixed order.
+template <int D, int SPL, int NV, class TF>
+__device__ __forceinline__ void add_partials(ScatterLDS<D, SPL, NV, TF>& L, int cell, int a0, int half,
+                                             const TF (&acc)[ScatterShape<D, SPL, NV, TF>::SPP][ScatterShape<D, SPL, NV, TF>::NSH][NV]) {
+    using G = BG<D, SPL>;
+    using SS = ScatterShape<D, SPL, NV, TF>;
+    int c[3] = {0, 0, 0};
+    if (D == 3) { c[2] = cell % G::E; c[1] = (cell / G::E) % G::E; c[0] = cell / (G::E * G::E); }
+    else { c[1] = cell % G::E; c[0] = cell / G::E; }
 #pragma unroll
-    for (int o = 0; o < SS::NSH; ++o)
+    for (int s = 0; s < SS::SPP; ++s) {
+        const int a = a0 + s;
+        if (a >= G::W) break;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[o][v] = 0.0;
-    const bool worker = tid < SS::WORKERS;
-    const int cell = tid % G::NC, r = tid / G::NC;
-    const int slice = r % G::W, half = r / G::W;
-    if (worker) {
-        const uint32_t k0 = L.cstart[cell], k1 = L.cstart[cell + 1];
-        int base[3];
-        typename Contrib::Part cur, nxt;
-        if (k0 < k1) contrib.load(L.p[k0], start + k0, cur);
-        for (uint32_t k = k0; k < k1; ++k) {
-            if (k + 1 < k1) contrib.load(L.p[k + 1], start + k + 1, nxt);  // in flight while cur is added
-            (void)contrib.add(cur, slice, half, acc, base);
-            cur = nxt;
-        }
-    }
-    constexpr int QN = (G::NT + SS::TB - 1) / SS::TB;
-    double tacc[QN][NV];
+        for (int obl = 0; obl < SS::OBN; ++obl) {
+            const int ob = half * SS::OBN + obl;
+            if (D == 3) {
 #pragma unroll
-    for (int qq = 0; qq < QN; ++qq)
+                for (int oc = 0; oc < G::W; ++oc) {
+                    const int q = tile_index<D, SPL>(c[0] + a, c[1] + ob, c[2] + oc);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) tacc[qq][v] = 0.0;
-    for (int a = 0; a < G::W; ++a) {
-        __syncthreads();
-        if (worker && slice == a) {
-#pragma unroll
-            for (int o = 0; o < SS::NSH; ++o)
-#pragma unroll
-                for (int v = 0; v < NV; ++v) L.u.part[((half * SS::NSH + o) * NV + v) * G::NC + cell] = acc[o][v];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int qq = 0; qq < QN; ++qq) {
-            const int q = tid + qq * SS::TB;
-            if (q >= G::NT) continue;
-            int t0, t1, t2 = 0;
-            if (D == 3) { t2 = q % G::T; const int rr = q / G::T; t1 = rr % G::T; t0 = rr / G::T; }
-            else { t1 = q % G::T; t0 = q / G::T; }
-            const int c0 = t0 - a;
-            if (c0 < 0 || c0 >= G::E) continue;
-            for (int ob = 0; ob < G::W; ++ob) {
-                const int c1 = t1 - ob;
-                if (c1 < 0 || c1 >= G::E) continue;
-                if (D == 3) {
-                    for (int oc = 0; oc < G::W; ++oc) {
-                        const int c2 = t2 - oc;
-                        if (c2 < 0 || c2 >= G::E) continue;
-                        const int cq = (c0 * G::E + c1) * G::E + c2, o = ob * G::W + oc;
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) tacc[qq][v] += L.u.part[(o * NV + v) * G::NC + cq];
-                    }
-                } else {
-                    const int cq = c0 * G::E + c1, o = ob;
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) tacc[qq][v] += L.u.part[(o * NV + v) * G::NC + cq];
+                    for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], acc[s][obl * G::W + oc][v]);
                 }
+            } else {
+                const int q = tile_index<D, SPL>(c[0] + a, c[1] + ob, 0);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], acc[s][obl][v]);
             }
         }
     }
+}
+
+// Ordered accumulation (see above) of the block's particles into its tile, written to staging.
+// contrib.load(p, i, part) fetches source particle p (sorted slot i); contrib.add(part, a0, half, acc, base)
+// adds it to acc for slices a0.. and second-axis half `half` and returns false if its window leaves the grid.
+template <int D, int SPL, int NV, class TF, class Contrib>
+__device__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, uint32_t b,
+                              uint32_t start) {
+    using SS = ScatterShape<D, SPL, NV, TF>;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int grp = wave % SS::NG, half = wave / SS::NG, a0 = grp * SS::SPP;
+    TF acc[SS::SPP][SS::NSH][NV];
 #pragma unroll
-    for (int qq = 0; qq < QN; ++qq) {
-        const int q = tid + qq * SS::TB;
-        if (q >= G::NT) continue;
+    for (int s = 0; s < SS::SPP; ++s)
 #pragma unroll
-        for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = tacc[qq][v];
+        for (int o = 0; o < SS::NSH; ++o)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[s][o][v] = TF(0);
+    {
+        const uint32_t k0 = L.cstart[lane], k1 = L.cstart[lane + 1];
+        int base[3];
+        for (uint32_t k = k0; k < k1; ++k) {
+            typename Contrib::Part q;
+            contrib.load(L.p[k], start + k, q);
+            (void)contrib.add(q, a0, half, acc, base);
+        }
     }
+    zero_tile<D, SPL, NV, TF>(L);
+    for (int w = 0; w < SS::WAVES; ++w) {
+        __syncthreads();
+        if (wave == w) add_partials<D, SPL, NV, TF>(L, lane, a0, half, acc);
+    }
+    __syncthreads();
+    write_tile<D, SPL, NV, TF>(L, stg, b);
 }
 
 // Tile of a block with more than kMaxBP particles: every particle adds its window with LDS atomics.
-template <int D, int SPL, int NV, class Contrib>
-__device__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV>& L, const Contrib& contrib, const Staging& stg, const SortBufs& sb,
+template <int D, int SPL, int NV, class TF, class Contrib>
+__device__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, const SortBufs& sb,
                                          uint32_t b, uint32_t start, uint32_t cnt, const int* bc) {
     using G = BG<D, SPL>;
-    using SS = ScatterShape<D, SPL, NV>;
+    using SS = ScatterShape<D, SPL, NV, TF>;
     const int tid = threadIdx.x;
-    for (int q = tid; q < G::NT * NV; q += blockDim.x) L.u.tile[q] = 0.0;
+    zero_tile<D, SPL, NV, TF>(L);
     __syncthreads();
     uint32_t err = 0;
     for (uint32_t k = tid; k < cnt; k += blockDim.x) {
         const uint32_t i = start + k, p = sb.perm[i];
+        typename Contrib::Part q;
+        contrib.load(p, i, q);
         bool ok = true;
-        for (int a = 0; a < G::W && ok; ++a) {
+        for (int grp = 0; grp < SS::NG && ok; ++grp) {
             for (int half = 0; half < SS::SPLIT && ok; ++half) {
-                double acc[SS::NSH][NV];
+                const int a0 = grp * SS::SPP;
+                TF acc[SS::SPP][SS::NSH][NV];
 #pragma unroll
-                for (int o = 0; o < SS::NSH; ++o)
+                for (int s = 0; s < SS::SPP; ++s)
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) acc[o][v] = 0.0;
+                    for (int o = 0; o < SS::NSH; ++o)
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) acc[s][o][v] = TF(0);
                 int base[3] = {0, 0, 0};
-                typename Contrib::Part q;
-                contrib.load(p, i, q);
-                ok = contrib.add(q, a, half, acc, base);
+                ok = contrib.add(q, a0, half, acc, base);
                 int l[3] = {0, 0, 0};
                 for (int d = 0; d < D; ++d) { l[d] = base[d] - G::E * bc[d]; ok = ok && l[d] >= 0 && l[d] < G::E; }
                 if (!ok) { err |= MPM_ERR_GRID_BOUNDS; break; }
 #pragma unroll
-                for (int obl = 0; obl < SS::OBN; ++obl) {
-                    const int ob = half * SS::OBN + obl;
-                    if (D == 3) {
+                for (int s = 0; s < SS::SPP; ++s) {
+                    const int a = a0 + s;
+                    if (a >= G::W) break;
 #pragma unroll
-                        for (int oc = 0; oc < G::W; ++oc) {
-                            const int q = tile_index<D, SPL>(l[0] + a, l[1] + ob, l[2] + oc);
+                    for (int obl = 0; obl < SS::OBN; ++obl) {
+                        const int ob = half * SS::OBN + obl;
+                        if (D == 3) {
 #pragma unroll
-                            for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], acc[obl * G::W + oc][v]);
+                            for (int oc = 0; oc < G::W; ++oc) {
+                                const int qn = tile_index<D, SPL>(l[0] + a, l[1] + ob, l[2] + oc);
+#pragma unroll
+                                for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + qn], acc[s][obl * G::W + oc][v]);
+                            }
+                        } else {
+                            const int qn = tile_index<D, SPL>(l[0] + a, l[1] + ob, 0);
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + qn], acc[s][obl][v]);
                         }
-                    } else {
-                        const int q = tile_index<D, SPL>(l[0] + a, l[1] + ob, 0);
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], acc[obl][v]);
                     }
                 }
             }
         }
     }
     __syncthreads();
-    for (int q = tid; q < G::NT; q += blockDim.x)
-        for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = L.u.tile[v * G::NT + q];
+    write_tile<D, SPL, NV, TF>(L, stg, b);
     return err;
 }
 
@@ -757,34 +793,61 @@ __device__ __forceinline__ void axis_node(double* const* coord, int d, int base,
     dw = BSpline<SPL>::dN(t);
 }
 
-// Window offsets (a, second-axis offsets of `half`, all third-axis offsets) of one particle: weights,
-// offsets to the particle and weight gradients. f(ol, weight, pd, gr) is called per node with the
-// worker-local offset index ol.
-template <int D, int SPL, int OBN, class F>
-__device__ __forceinline__ void for_slice(const Stencil<D, SPL>& st, double* const* coord, const double* xp, double h1, int a, int half, F&& f) {
+// Stencil values in the transfer type: weight, derivative and node-minus-particle offset per axis node.
+template <int D, int SPL, class TF> struct WStencil {
+    static constexpr int W = BG<D, SPL>::W;
+    TF w[3][W], dw[3][W], pd[3][W];
+};
+
+template <int D, int SPL, class TF>
+__device__ __forceinline__ void to_wstencil(WStencil<D, SPL, TF>& ws, const Stencil<D, SPL>& st, const double* xp) {
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int a = 0; a < BG<D, SPL>::W; ++a) {
+            ws.w[d][a] = (TF)st.w[d][a];
+            ws.dw[d][a] = (TF)st.dw[d][a];
+            ws.pd[d][a] = (TF)(st.xi[d][a] - xp[d]);
+        }
+}
+
+// The window nodes of slices a0 .. a0+SPP-1 (second-axis offsets of `half`, all third-axis offsets) of one
+// particle: weights, offsets to the particle and weight gradients in TF. f(s, ol, weight, pd, gr) is called
+// per node with the slice index s and the wavefront-local offset index ol. (For TF = double the values and
+// the order of the products are those of make_stencil and grad_wip.)
+template <int D, int SPL, int SPP, int OBN, class TF, class F>
+__device__ __forceinline__ void for_group(const WStencil<D, SPL, TF>& ws, const Stencil<D, SPL>& st, double* const* coord, const double* xp,
+                                          double h1, int a0, int half, F&& f) {
     constexpr int W = BG<D, SPL>::W;
-    double w0, dw0, xi0;
-    axis_node<SPL>(coord, 0, st.base[0], a, xp[0], h1, w0, dw0, xi0);
+    const TF h1t = (TF)h1;
 #pragma unroll
-    for (int obl = 0; obl < OBN; ++obl) {
-        double w1, dw1, xi1;
-        if (OBN == W) { w1 = st.w[1][obl]; dw1 = st.dw[1][obl]; xi1 = st.xi[1][obl]; }
-        else axis_node<SPL>(coord, 1, st.base[1], half * OBN + obl, xp[1], h1, w1, dw1, xi1);
-        if (D == 3) {
+    for (int s = 0; s < SPP; ++s) {
+        const int a = a0 + s;
+        if (a >= W) break;
+        TF w0, dw0, pd0;
+        if (SPP == W) { w0 = ws.w[0][s]; dw0 = ws.dw[0][s]; pd0 = ws.pd[0][s]; }
+        else { double w, dw, xi; axis_node<SPL>(coord, 0, st.base[0], a, xp[0], h1, w, dw, xi); w0 = (TF)w; dw0 = (TF)dw; pd0 = (TF)(xi - xp[0]); }
 #pragma unroll
-            for (int oc = 0; oc < W; ++oc) {
-                const double w2 = st.w[2][oc], dw2 = st.dw[2][oc];
-                double weight = w0 * w1;
-                weight = weight * w2;
-                const double pd[3] = {xi0 - xp[0], xi1 - xp[1], st.xi[2][oc] - xp[2]};
-                const double gr[3] = {dw0 * w1 * w2 * h1, dw1 * w0 * w2 * h1, dw2 * w0 * w1 * h1};
-                f(obl * W + oc, weight, pd, gr);
+        for (int obl = 0; obl < OBN; ++obl) {
+            TF w1, dw1, pd1;
+            if (OBN == W) { w1 = ws.w[1][obl]; dw1 = ws.dw[1][obl]; pd1 = ws.pd[1][obl]; }
+            else { double w, dw, xi; axis_node<SPL>(coord, 1, st.base[1], half * OBN + obl, xp[1], h1, w, dw, xi); w1 = (TF)w; dw1 = (TF)dw; pd1 = (TF)(xi - xp[1]); }
+            if (D == 3) {
+#pragma unroll
+                for (int oc = 0; oc < W; ++oc) {
+                    const TF w2 = ws.w[2][oc], dw2 = ws.dw[2][oc];
+                    TF weight = w0 * w1;
+                    weight = weight * w2;
+                    const TF pd[3] = {pd0, pd1, ws.pd[2][oc]};
+                    const TF gr[3] = {dw0 * w1 * w2 * h1t, dw1 * w0 * w2 * h1t, dw2 * w0 * w1 * h1t};
+                    f(s, obl * W + oc, weight, pd, gr);
+                }
+            } else {
+                const TF weight = w0 * w1;
+                const TF pd[3] = {pd0, pd1, TF(0)};
+                const TF gr[3] = {dw0 * w1 * h1t, dw1 * w0 * h1t, TF(0)};
+                f(s, obl, weight, pd, gr);
             }
-        } else {
-            const double weight = w0 * w1;
-            const double pd[3] = {xi0 - xp[0], xi1 - xp[1], 0.0};
-            const double gr[3] = {dw0 * w1 * h1, dw1 * w0 * h1, 0.0};
-            f(obl, weight, pd, gr);
         }
     }
 }
@@ -794,12 +857,12 @@ __device__ __forceinline__ void for_slice(const Stencil<D, SPL>& st, double* con
 // tau grad w in separate values (SEP) or the force folded into the momentum as - dt V0 / m_p tau grad w,
 // and last the MIBF friction sum w muI. load() fetches a particle, add() computes its stencil and adds
 // it, so that a worker can fetch the next particle while it adds the current one.
-template <int D, int SPL, bool SEP, bool MIBF>
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
 struct P2GContrib {
     static constexpr int NV = 1 + D + (SEP ? D : 0) + (MIBF ? 1 : 0);
-    using SS = ScatterShape<D, SPL, NV>;
-    struct Part { double x[3], v[3], B[9], t6[6], muI; };
-    PartState src;
+    using SS = ScatterShape<D, SPL, NV, TF>;
+    struct Part { double x[3]; TF v[3], B[9], t6[6]; double muI; };
+    PartStateT<TF> src;
     double* coord[3];
     double org[3];
     uint32_t nn[3];
@@ -807,38 +870,41 @@ struct P2GContrib {
     bool apic;
     __device__ __forceinline__ void load(uint32_t p, uint32_t, Part& q) const {
         for (int d = 0; d < D; ++d) { q.x[d] = src.x[d][p]; q.v[d] = src.v[d][p]; }
-        for (int k = 0; k < D * D; ++k) q.B[k] = apic ? src.B[k][p] : 0.0;
+        for (int k = 0; k < D * D; ++k) q.B[k] = apic ? src.B[k][p] : TF(0);
         for (int k = 0; k < SymPack<D>::n; ++k) q.t6[k] = src.tau[k][p];
         q.muI = MIBF ? src.muI[p] : 0.0;
     }
-    __device__ __forceinline__ bool add(const Part& q, int a, int half, double (&acc)[SS::NSH][NV], int* base) const {
+    __device__ __forceinline__ bool add(const Part& q, int a0, int half, TF (&acc)[SS::SPP][SS::NSH][NV], int* base) const {
         Stencil<D, SPL> st;
         if (!make_stencil<D, SPL>(st, q.x, org, nn, coord, h1, true)) return false;
         for (int d = 0; d < D; ++d) base[d] = st.base[d];
-        double tau[3][3];
+        TF tau[3][3];
         if (D == 3) { tau[0][0] = q.t6[0]; tau[1][1] = q.t6[1]; tau[2][2] = q.t6[2]; tau[0][1] = tau[1][0] = q.t6[3]; tau[0][2] = tau[2][0] = q.t6[4]; tau[1][2] = tau[2][1] = q.t6[5]; }
         else { tau[0][0] = q.t6[0]; tau[1][1] = q.t6[1]; tau[0][1] = tau[1][0] = q.t6[2]; }
-        for_slice<D, SPL, SS::OBN>(st, coord, q.x, h1, a, half, [&](int ol, double weight, const double* pd, const double* gr) __attribute__((always_inline)) {
-            const bool massive = weight > 1e-25;
-            double fr[3];
+        const TF aDt = (TF)aD, kft = (TF)kf;
+        WStencil<D, SPL, TF> ws;
+        to_wstencil<D, SPL, TF>(ws, st, q.x);
+        for_group<D, SPL, SS::SPP, SS::OBN, TF>(ws, st, coord, q.x, h1, a0, half, [&](int sl, int ol, TF w, const TF* pd, const TF* gr) __attribute__((always_inline)) {
+            const bool massive = w > TF(1e-25);
+            TF fr[3];
             for (int r = 0; r < D; ++r) {
-                double s = tau[r][0] * gr[0];
+                TF s = tau[r][0] * gr[0];
                 for (int k = 1; k < D; ++k) s += tau[r][k] * gr[k];
                 fr[r] = s;
             }
-            acc[ol][0] += massive ? weight : 0.0;
+            acc[sl][ol][0] += massive ? w : TF(0);
             for (int r = 0; r < D; ++r) {
-                double m = q.v[r] * weight;
+                TF m = q.v[r] * w;
                 if (apic) {
-                    double bd = q.B[r] * pd[0];
+                    TF bd = q.B[r] * pd[0];
                     for (int k = 1; k < D; ++k) bd += q.B[r + k * D] * pd[k];
-                    m += bd * aD * weight;
+                    m += bd * aDt * w;
                 }
-                m = massive ? m : 0.0;
-                if (SEP) { acc[ol][1 + r] += m; acc[ol][1 + D + r] += fr[r]; }
-                else acc[ol][1 + r] += m - kf * fr[r];
+                m = massive ? m : TF(0);
+                if (SEP) { acc[sl][ol][1 + r] += m; acc[sl][ol][1 + D + r] += fr[r]; }
+                else acc[sl][ol][1 + r] += m - kft * fr[r];
             }
-            if (MIBF) acc[ol][NV - 1] += massive ? q.muI * weight : 0.0;
+            if (MIBF) acc[sl][ol][NV - 1] += massive ? (TF)q.muI * w : TF(0);
         });
         return true;
     }
@@ -847,12 +913,12 @@ struct P2GContrib {
 // MUSL re-scatter contributions (musl.cpp:28-85): sum w (v_new + B_new (x_i - x_p) D^-1) over
 // weight > 1e-25, with the position of the step's start (source layout, slot p) and the new velocity
 // and affine matrix (destination layout, sorted slot i).
-template <int D, int SPL>
+template <int D, int SPL, class TF>
 struct MuslContrib {
     static constexpr int NV = D;
-    using SS = ScatterShape<D, SPL, NV>;
-    struct Part { double x[3], v[3], B[9]; };
-    PartState src, dst;
+    using SS = ScatterShape<D, SPL, NV, TF>;
+    struct Part { double x[3]; TF v[3], B[9]; };
+    PartStateT<TF> src, dst;
     double* coord[3];
     double org[3];
     uint32_t nn[3];
@@ -860,22 +926,25 @@ struct MuslContrib {
     bool apic;
     __device__ __forceinline__ void load(uint32_t p, uint32_t i, Part& q) const {
         for (int d = 0; d < D; ++d) { q.x[d] = src.x[d][p]; q.v[d] = dst.v[d][i]; }
-        for (int k = 0; k < D * D; ++k) q.B[k] = apic ? dst.B[k][i] : 0.0;
+        for (int k = 0; k < D * D; ++k) q.B[k] = apic ? dst.B[k][i] : TF(0);
     }
-    __device__ __forceinline__ bool add(const Part& q, int a, int half, double (&acc)[SS::NSH][NV], int* base) const {
+    __device__ __forceinline__ bool add(const Part& q, int a0, int half, TF (&acc)[SS::SPP][SS::NSH][NV], int* base) const {
         Stencil<D, SPL> st;
         if (!make_stencil<D, SPL>(st, q.x, org, nn, coord, h1, true)) return false;
         for (int d = 0; d < D; ++d) base[d] = st.base[d];
-        for_slice<D, SPL, SS::OBN>(st, coord, q.x, h1, a, half, [&](int ol, double weight, const double* pd, const double*) __attribute__((always_inline)) {
-            if (!(weight > 1e-25)) return;
+        const TF aDt = (TF)aD;
+        WStencil<D, SPL, TF> ws;
+        to_wstencil<D, SPL, TF>(ws, st, q.x);
+        for_group<D, SPL, SS::SPP, SS::OBN, TF>(ws, st, coord, q.x, h1, a0, half, [&](int sl, int ol, TF w, const TF* pd, const TF*) __attribute__((always_inline)) {
+            if (!(w > TF(1e-25))) return;
             for (int r = 0; r < D; ++r) {
-                double m = q.v[r] * weight;
+                TF m = q.v[r] * w;
                 if (apic) {
-                    double bd = q.B[r] * pd[0];
+                    TF bd = q.B[r] * pd[0];
                     for (int k = 1; k < D; ++k) bd += q.B[r + k * D] * pd[k];
-                    m += bd * aD * weight;
+                    m += bd * aDt * w;
                 }
-                acc[ol][r] += m;
+                acc[sl][ol][r] += m;
             }
         });
         return true;
@@ -887,11 +956,11 @@ struct MuslContrib {
 // (explicit_euler_update.cpp:15-168). The reference adds the force only where m_i > 0; nodes with
 // m_i = 0 are never updated or read, so the force is scattered unconditionally.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL, bool SEP, bool MIBF>
-__global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF>::NV>::TB))
-void k_p2g(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g, Staging stg) {
-    constexpr int NV = P2GContrib<D, SPL, SEP, MIBF>::NV;
-    __shared__ ScatterLDS<D, SPL, NV> L;
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+__global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB))
+void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+    constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
+    __shared__ ScatterLDS<D, SPL, NV, TF> L;
     if (S->stop) return;
     const uint32_t b = blockIdx.x;
     if (b >= S->nblocks) return;
@@ -900,7 +969,7 @@ void k_p2g(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, Gr
     const uint32_t start = sb.offset[b];
     int bc[3];
     block_coords<D>(b, S, bc);
-    P2GContrib<D, SPL, SEP, MIBF> cf;
+    P2GContrib<D, SPL, SEP, MIBF, TF> cf;
     cf.src = src;
     for (int d = 0; d < 3; ++d) { cf.coord[d] = g.coord[d]; cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
     cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
@@ -908,10 +977,10 @@ void k_p2g(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, Gr
     uint32_t err = 0;
     if (cnt > kMaxBP) {
         for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) dst.id[start + k] = src.id[sb.perm[start + k]];
-        err |= scatter_block_atomic<D, SPL, NV>(L, cf, stg, sb, b, start, cnt, bc);
+        err |= scatter_block_atomic<D, SPL, NV, TF>(L, cf, stg, sb, b, start, cnt, bc);
     } else {
-        err |= order_block<D, SPL, NV>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
-        scatter_block<D, SPL, NV>(L, cf, stg, b, start);
+        err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
+        scatter_block<D, SPL, NV, TF>(L, cf, stg, b, start);
     }
     if (S->check_conservation) {  // sum of m_p v_p (simulation.cpp:292-317)
         double pm[3] = {0, 0, 0};
@@ -927,11 +996,11 @@ void k_p2g(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, Gr
 }
 
 // k_musl_scatter: the MUSL re-scatter, in the particle order that k_p2g fixed for the step.
-template <int D, int SPL>
-__global__ __launch_bounds__((ScatterShape<D, SPL, D>::TB))
-void k_musl_scatter(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g, Staging stg) {
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__((ScatterShape<D, SPL, D, TF>::TB))
+void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
     using G = BG<D, SPL>;
-    __shared__ ScatterLDS<D, SPL, D> L;
+    __shared__ ScatterLDS<D, SPL, D, TF> L;
     if (S->stop) return;
     const uint32_t b = blockIdx.x;
     if (b >= S->nblocks) return;
@@ -940,19 +1009,19 @@ void k_musl_scatter(Params P, StepState* S, PartState src, PartState dst, SortBu
     const uint32_t start = sb.offset[b];
     int bc[3];
     block_coords<D>(b, S, bc);
-    MuslContrib<D, SPL> cf;
+    MuslContrib<D, SPL, TF> cf;
     cf.src = src; cf.dst = dst;
     for (int d = 0; d < 3; ++d) { cf.coord[d] = g.coord[d]; cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
     cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
     uint32_t err = 0;
     if (cnt > kMaxBP) {
-        err |= scatter_block_atomic<D, SPL, D>(L, cf, stg, sb, b, start, cnt, bc);
+        err |= scatter_block_atomic<D, SPL, D, TF>(L, cf, stg, sb, b, start, cnt, bc);
     } else {
         for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) L.p[k] = sb.perm[start + k];
         for (int c = threadIdx.x; c <= G::NC; c += blockDim.x) L.cstart[c] = sb.cell_off[(size_t)b * (G::NC + 1) + c];
         if (threadIdx.x == 0) L.cstart[G::NC + 1] = cnt;
         __syncthreads();
-        scatter_block<D, SPL, D>(L, cf, stg, b, start);
+        scatter_block<D, SPL, D, TF>(L, cf, stg, b, start);
     }
     err = wave_or_u32(err);
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
@@ -960,8 +1029,8 @@ void k_musl_scatter(Params P, StepState* S, PartState src, PartState dst, SortBu
 
 // Sum over the blocks that cover node gx (at most two per axis), in increasing block order, of the
 // staged tile values.
-template <int D, int SPL, int NV>
-__device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& sb, const Staging& stg, const int* gx, double (&s)[NV]) {
+template <int D, int SPL, int NV, class TF>
+__device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& sb, const StagingT<TF>& stg, const int* gx, double (&s)[NV]) {
     using G = BG<D, SPL>;
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) {
@@ -979,7 +1048,7 @@ __device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& 
                 const int q = tile_index<D, SPL>(gx[0] - G::E * b0, gx[1] - G::E * b1, D == 3 ? gx[2] - G::E * b2 : 0);
                 const size_t at = (size_t)bl * G::NT + q;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) s[v] += stg.base[(size_t)v * stg.stride + at];
+                for (int v = 0; v < NV; ++v) s[v] += (double)stg.base[(size_t)v * stg.stride + at];
             }
 }
 
@@ -987,9 +1056,9 @@ __device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& 
 // k_grid_update: node sums, normalisation (p2g.cpp:84-98), explicit update and boundary conditions
 // (explicit_euler_update.cpp:183-232, boundary_collision.cpp:5-394), flip = v_new - v_old.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL, bool SEP, bool MIBF>
-__global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, SortBufs sb, GridPtrs g, Staging stg) {
-    constexpr int NV = P2GContrib<D, SPL, SEP, MIBF>::NV;
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+__global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+    constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
     if (S->stop) return;
     const uint32_t nn = S->grid_nodes;
     const uint32_t n[3] = {S->n[0], S->n[1], S->n[2]};
@@ -1003,7 +1072,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, 
         if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
         else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
         double s[NV];
-        sum_staging<D, SPL, NV>(S, sb, stg, gx, s);
+        sum_staging<D, SPL, NV, TF>(S, sb, stg, gx, s);
         const double sw = s[0];
         V<D> v;
         if (sw > 0) { for (int d = 0; d < D; ++d) v[d] = s[1 + d] / sw; }
@@ -1022,10 +1091,10 @@ __global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, 
             vn = v;
             for (int d = 0; d < D; ++d) flip[d] = 0.0;
         }
-        g.mass[i] = mi;
-        for (int d = 0; d < D; ++d) g.v[d][i] = vn[d];
-        if (SEP) for (int d = 0; d < D; ++d) g.flip[d][i] = flip[d];
-        if (MIBF) g.fric[i] = fr;
+        g.mass[i] = (TF)mi;
+        for (int d = 0; d < D; ++d) g.v[d][i] = (TF)vn[d];
+        if (SEP) for (int d = 0; d < D; ++d) g.flip[d][i] = (TF)flip[d];
+        if (MIBF) g.fric[i] = (TF)fr;
     }
     if (cons) {
         cm = wave_sum_f64(cm);
@@ -1038,8 +1107,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, 
 }
 
 // k_musl_update: MUSL normalisation, musl.cpp:182-188: v_i = (sum) / (m_i / m_p), zero where m_i == 0.
-template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_musl_update(Params P, const StepState* S, SortBufs sb, GridPtrs g, Staging stg) {
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_musl_update(Params P, const StepState* S, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
     if (S->stop) return;
     const uint32_t nn = S->grid_nodes;
     const uint32_t n[3] = {S->n[0], S->n[1], S->n[2]};
@@ -1050,11 +1119,11 @@ __global__ __launch_bounds__(kBlock) void k_musl_update(Params P, const StepStat
         const double mi = g.mass[i];
         if (mi > 0) {
             double s[D];
-            sum_staging<D, SPL, D>(S, sb, stg, gx, s);
+            sum_staging<D, SPL, D, TF>(S, sb, stg, gx, s);
             const double q = mi / P.particle_mass;
-            for (int d = 0; d < D; ++d) g.v[d][i] = s[d] / q;
+            for (int d = 0; d < D; ++d) g.v[d][i] = (TF)(s[d] / q);
         } else {
-            for (int d = 0; d < D; ++d) g.v[d][i] = 0.0;
+            for (int d = 0; d < D; ++d) g.v[d][i] = TF(0);
         }
     }
 }
@@ -1068,9 +1137,9 @@ __device__ __forceinline__ double blend_velocity(double fr, double v, double fli
 
 // F <- F + (dt*grad v) * F, return map, Kirchhoff stress for the next step (deformation_update.cpp:13-46,
 // plasticity.cpp:11-665). Reads particle p of the source layout, writes slot i of the destination layout.
-template <int D>
-__device__ __forceinline__ void deform_and_project(const Params& P, double dt, const PartState& src, uint32_t p,
-                                                   const PartState& dst, uint32_t i, const double (&gv)[3][3],
+template <int D, class TF>
+__device__ __forceinline__ void deform_and_project(const Params& P, double dt, const PartStateT<TF>& src, uint32_t p,
+                                                   const PartStateT<TF>& dst, uint32_t i, const double (&gv)[3][3],
                                                    uint32_t& err, uint32_t& pcount) {
     M<D> F, A;
     for (int k = 0; k < D * D; ++k) F.m[k] = src.F[k][p];
@@ -1105,7 +1174,7 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
         hencky_tau<D>(U, ls, P.mu, P.lambda, tau);
     }
     for (int k = 0; k < D * D; ++k) dst.F[k][i] = F.m[k];
-    for (int k = 0; k < SymPack<D>::n; ++k) dst.tau[k][i] = tau[k];
+    for (int k = 0; k < SymPack<D>::n; ++k) dst.tau[k][i] = (TF)tau[k];
     dst.eps_dev[i] = ps.eps_dev; dst.eps_vol[i] = ps.eps_vol; dst.eps_prad[i] = ps.eps_prad;
     dst.dgamma[i] = ps.dgamma; dst.visc[i] = ps.visc; dst.muI[i] = ps.muI;
 }
@@ -1120,8 +1189,8 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
 
 // Copies ncomp node arrays over the block's tile into LDS ([component][node]); nodes outside the grid
 // read as zero (no particle window reaches them).
-template <int D, int SPL>
-__device__ __forceinline__ void load_tile(double* s, double* const* src, int ncomp, const int* bc, const uint32_t* n) {
+template <int D, int SPL, class TF>
+__device__ __forceinline__ void load_tile(TF* s, TF* const* src, int ncomp, const int* bc, const uint32_t* n) {
     using G = BG<D, SPL>;
     for (int q = threadIdx.x; q < G::NT; q += blockDim.x) {
         int t[3] = {0, 0, 0};
@@ -1131,7 +1200,7 @@ __device__ __forceinline__ void load_tile(double* s, double* const* src, int nco
         int gx[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) { gx[d] = G::E * bc[d] + t[d]; in = in && gx[d] < (int)n[d]; }
         const uint32_t node = in ? node_index<D>(gx[0], gx[1], gx[2], n) : 0u;
-        for (int c = 0; c < ncomp; ++c) s[c * G::NT + q] = in ? src[c][node] : 0.0;
+        for (int c = 0; c < ncomp; ++c) s[c * G::NT + q] = in ? src[c][node] : TF(0);
     }
 }
 
@@ -1149,9 +1218,9 @@ __device__ __forceinline__ bool gather_stencil(Stencil<D, SPL>& st, int* tl, con
 
 // Gather of G2P (g2p.cpp:35-67) and of the velocity gradient (deformation_update.cpp:217-230) over the
 // non-zero nodes of the window, in the reference's (i,j,k) order, from the LDS tile.
-template <int D, int SPL, bool PIC, bool GRAD>
-__device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const int* tl, const double* xp, const double* s_v, const double* s_flip,
-                                       double h1, bool apic, bool useflip, double* pic, double* fl, double (&Bp)[3][3], double (&gv)[3][3]) {
+template <int D, int SPL, bool PIC, bool GRAD, class TF>
+__device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const int* tl, const double* xp, const TF* s_v, const TF* s_flip,
+                                       double h1, bool apic, bool useflip, TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
     using G = BG<D, SPL>;
     const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
 #pragma unroll
@@ -1162,17 +1231,18 @@ __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const int* tl,
             for (int ic = 0; ic < c2; ++ic) {
                 const int c = (D == 3) ? ic : 0;
                 const int q = tile_index<D, SPL>(tl[0] + a, tl[1] + b, tl[2] + c);
-                double vi[3];
+                TF vi[3];
                 for (int d = 0; d < D; ++d) vi[d] = s_v[d * G::NT + q];
                 if (PIC) {
                     double weight = st.w[0][a] * st.w[1][b];
                     if (D == 3) weight = weight * st.w[2][c];
-                    const double pd[3] = {st.xi[0][a] - xp[0], st.xi[1][b] - xp[1], D == 3 ? st.xi[2][c] - xp[2] : 0.0};
+                    const TF w = (TF)weight;
+                    const TF pd[3] = {(TF)(st.xi[0][a] - xp[0]), (TF)(st.xi[1][b] - xp[1]), D == 3 ? (TF)(st.xi[2][c] - xp[2]) : TF(0)};
                     for (int r = 0; r < D; ++r) {
-                        pic[r] += vi[r] * weight;
-                        if (apic) for (int k = 0; k < D; ++k) Bp[r][k] += vi[r] * pd[k] * weight;
+                        pic[r] += vi[r] * w;
+                        if (apic) for (int k = 0; k < D; ++k) Bp[r][k] += vi[r] * pd[k] * w;
                     }
-                    if (useflip) for (int r = 0; r < D; ++r) fl[r] += s_flip[r * G::NT + q] * weight;
+                    if (useflip) for (int r = 0; r < D; ++r) fl[r] += s_flip[r * G::NT + q] * w;
                 }
                 if (GRAD) {
                     double gr[3];
@@ -1184,7 +1254,7 @@ __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const int* tl,
                         gr[0] = st.dw[0][a] * st.w[1][b] * h1;
                         gr[1] = st.dw[1][b] * st.w[0][a] * h1;
                     }
-                    for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) gv[r][k] += vi[r] * gr[k];
+                    for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) gv[r][k] += vi[r] * (TF)gr[k];
                 }
             }
         }
@@ -1208,18 +1278,18 @@ __device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& 
 // ----------------------------------------------------------------------------------------------
 // k_g2p_usl (USL): G2P (g2p.cpp:17-77), deformationUpdate + plasticity, positionUpdate.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g) {
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
     using G = BG<D, SPL>;
-    __shared__ double s_v[3 * G::NT], s_f[3 * G::NT];
+    __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
     BlockRange br;
     if (!block_range<D>(S, sb, br)) return;
     const double h1 = P.one_over_dx, dt = S->dt;
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
     const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
-    load_tile<D, SPL>(s_v, g.v, D, br.bc, nn);
-    if (useflip) load_tile<D, SPL>(s_f, g.flip, D, br.bc, nn);
+    load_tile<D, SPL, TF>(s_v, g.v, D, br.bc, nn);
+    if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, D, br.bc, nn);
     __syncthreads();
     uint32_t err = 0, pcount = 0;
     BoundsAcc ba;
@@ -1230,18 +1300,18 @@ __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, Part
         Stencil<D, SPL> st;
         int tl[3];
         if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, g.coord, h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
-        double pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3] = {{0}};
-        gather<D, SPL, true, true>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gv);
-        double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
+        TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gvt[3][3] = {{0}};
+        gather<D, SPL, true, true, TF>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gvt);
+        double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0}, gv[3][3];
         for (int d = 0; d < D; ++d) {
-            const double vo = useflip ? src.v[d][p] : 0.0;
-            vnew[d] = blend_velocity(P.flip_ratio, vo, fl[d], pic[d]);
+            const double vo = useflip ? (double)src.v[d][p] : 0.0;
+            vnew[d] = (double)(TF)blend_velocity(P.flip_ratio, vo, (double)fl[d], (double)pic[d]);
         }
-        for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : 0.0;
-        deform_and_project<D>(P, dt, src, p, dst, i, gv, err, pcount);
-        for (int d = 0; d < D; ++d) xnew[d] = xp[d] + dt * pic[d];
+        for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { dst.B[r + c * D][i] = apic ? Bp[r][c] : TF(0); gv[r][c] = (double)gvt[r][c]; }
+        deform_and_project<D, TF>(P, dt, src, p, dst, i, gv, err, pcount);
+        for (int d = 0; d < D; ++d) xnew[d] = xp[d] + dt * (double)pic[d];
         if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
-        for (int d = 0; d < D; ++d) { dst.x[d][i] = xnew[d]; dst.v[d][i] = vnew[d]; }
+        for (int d = 0; d < D; ++d) { dst.x[d][i] = xnew[d]; dst.v[d][i] = (TF)vnew[d]; }
         ba.add<D>(xnew, vnew);
     }
     reduce_bounds<D>(S, ba);
@@ -1257,18 +1327,18 @@ __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, Part
 // k_g2p_musl (MUSL): G2P (g2p.cpp:17-77), the MUSL velocity update (musl.cpp:13-25) and the position
 // update of positionUpdate (position_update.cpp:12).
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g) {
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
     using G = BG<D, SPL>;
-    __shared__ double s_v[3 * G::NT], s_f[3 * G::NT];
+    __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
     BlockRange br;
     if (!block_range<D>(S, sb, br)) return;
     const double h1 = P.one_over_dx, dt = S->dt;
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
     const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
-    load_tile<D, SPL>(s_v, g.v, D, br.bc, nn);
-    if (useflip) load_tile<D, SPL>(s_f, g.flip, D, br.bc, nn);
+    load_tile<D, SPL, TF>(s_v, g.v, D, br.bc, nn);
+    if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, D, br.bc, nn);
     __syncthreads();
     uint32_t err = 0;
     BoundsAcc ba;
@@ -1279,17 +1349,17 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
         Stencil<D, SPL> st;
         int tl[3];
         if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, g.coord, h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
-        double pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3];
-        gather<D, SPL, true, false>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gv);
+        TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3];
+        gather<D, SPL, true, false, TF>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gv);
         double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) {
-            const double vo = useflip ? src.v[d][p] : 0.0;
-            vnew[d] = blend_velocity(P.flip_ratio, vo, fl[d], pic[d]);
-            xnew[d] = xp[d] + dt * pic[d];
+            const double vo = useflip ? (double)src.v[d][p] : 0.0;
+            vnew[d] = (double)(TF)blend_velocity(P.flip_ratio, vo, (double)fl[d], (double)pic[d]);
+            xnew[d] = xp[d] + dt * (double)pic[d];
         }
         if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
-        for (int d = 0; d < D; ++d) { dst.v[d][i] = vnew[d]; dst.x[d][i] = xnew[d]; }
-        for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : 0.0;
+        for (int d = 0; d < D; ++d) { dst.v[d][i] = (TF)vnew[d]; dst.x[d][i] = xnew[d]; }
+        for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : TF(0);
         ba.add<D>(xnew, vnew);
     }
     reduce_bounds<D>(S, ba);
@@ -1302,9 +1372,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
 // One thread per sorted particle slot (the return map is long and register-heavy, so this kernel is
 // laid out for occupancy rather than LDS reuse); the velocity gradient gathers from the grid arrays.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL>
-__device__ __forceinline__ void gather_grad_global(const Stencil<D, SPL>& st, const GridPtrs& g, const uint32_t* n, double h1,
-                                                   double (&gv)[3][3]) {
+template <int D, int SPL, class TF>
+__device__ __forceinline__ void gather_grad_global(const Stencil<D, SPL>& st, const GridPtrsT<TF>& g, const uint32_t* n, double h1,
+                                                   TF (&gv)[3][3]) {
     const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
 #pragma unroll
     for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) {
@@ -1314,7 +1384,7 @@ __device__ __forceinline__ void gather_grad_global(const Stencil<D, SPL>& st, co
             for (int ic = 0; ic < c2; ++ic) {
                 const int c = (D == 3) ? ic : 0;
                 const uint32_t node = node_index<D>(st.base[0] + a, st.base[1] + b, (D == 3) ? st.base[2] + c : 0, n);
-                double vi[3];
+                TF vi[3];
                 for (int d = 0; d < D; ++d) vi[d] = g.v[d][node];
                 double gr[3];
                 if (D == 3) {
@@ -1325,14 +1395,14 @@ __device__ __forceinline__ void gather_grad_global(const Stencil<D, SPL>& st, co
                     gr[0] = st.dw[0][a] * st.w[1][b] * h1;
                     gr[1] = st.dw[1][b] * st.w[0][a] * h1;
                 }
-                for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) gv[r][k] += vi[r] * gr[k];
+                for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) gv[r][k] += vi[r] * (TF)gr[k];
             }
         }
     }
 }
 
-template <int D, int SPL>
-__global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartState src, PartState dst, SortBufs sb, GridPtrs g, uint32_t n) {
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
     if (S->stop) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t err = 0, pcount = 0;
@@ -1347,9 +1417,11 @@ __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, Part
         if (!make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, false)) {
             err |= MPM_ERR_GRID_BOUNDS;
         } else {
-            double gv[3][3] = {{0}};
-            gather_grad_global<D, SPL>(st, g, nn, h1, gv);
-            deform_and_project<D>(P, dt, src, p, dst, i, gv, err, pcount);
+            TF gvt[3][3] = {{0}};
+            gather_grad_global<D, SPL, TF>(st, g, nn, h1, gvt);
+            double gv[3][3];
+            for (int r = 0; r < 3; ++r) for (int k = 0; k < 3; ++k) gv[r][k] = (double)gvt[r][k];
+            deform_and_project<D, TF>(P, dt, src, p, dst, i, gv, err, pcount);
         }
     }
     pcount = wave_sum_u32(pcount);
