@@ -129,6 +129,10 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) { const uint32_t t = __shfl_xor(v, o, 64); v = t > v ? t : v; }
+    return v;
+}
 __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
     return v;
@@ -244,22 +248,50 @@ __device__ __forceinline__ bool stencil_base(int* b, const double* xp, const dou
     return ok;
 }
 
+// Node coordinates read from the grid's coordinate arrays.
+struct GlobalCoords {
+    double* const* c;
+    const uint32_t* n;
+    __device__ __forceinline__ double operator()(int d, int i) const { return c[d][i]; }
+    __device__ __forceinline__ int clamp4(int d, int b) const { return max(min(b, (int)n[d] - 4), 0); }
+};
+
+// Node coordinates around one block staged in LDS: entries lo_d .. lo_d + LC - 1 of axis d (separate
+// members rather than arrays, so that the view stays in registers).
+template <int LC> struct LdsCoords {
+    const double* c0; const double* c1; const double* c2;
+    int lo0, lo1, lo2;
+    __device__ __forceinline__ double operator()(int d, int i) const {
+        return d == 0 ? c0[i - lo0] : (d == 1 ? c1[i - lo1] : c2[i - lo2]);
+    }
+    __device__ __forceinline__ int clamp4(int d, int b) const {
+        const int lo = d == 0 ? lo0 : (d == 1 ? lo1 : lo2);
+        return max(min(b, lo + LC - 4), lo);
+    }
+};
+
 // Evaluates the 4-node window of the reference (weights are those of wip/grad_wip, tools.hpp:222-252) and
 // keeps its non-zero part: for the quadratic spline exactly one end node has zero weight (both at
-// f = 0.5), and skipping an exactly-zero term leaves every sum of the reference unchanged.
-template <int D, int SPL>
+// f = 0.5), and skipping an exactly-zero term leaves every sum of the reference unchanged. The twelve node
+// coordinates are fetched unconditionally (from a clamped index) so that their loads issue together.
+template <int D, int SPL, class CS>
 __device__ __forceinline__ bool make_stencil(Stencil<D, SPL>& st, const double* xp, const double* org, const uint32_t* n,
-                                             double* const* coord, double h1, bool clamp) {
+                                             const CS& cs, double h1, bool clamp) {
     int b[3] = {0, 0, 0};
-    if (!stencil_base<D>(b, xp, org, n, h1, clamp)) return false;
+    const bool ok = stencil_base<D>(b, xp, org, n, h1, clamp);
+    double c4[3][4];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        double w4[4], dw4[4], xi4[4];
+        const int bb = cs.clamp4(d, b[d]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) c4[d][a] = cs(d, bb + a);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        double w4[4], dw4[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const double c = coord[d][b[d] + a];
-            const double t = (xp[d] - c) * h1;
-            xi4[a] = c;
+            const double t = (xp[d] - c4[d][a]) * h1;
             w4[a] = BSpline<SPL>::N(t);
             dw4[a] = BSpline<SPL>::dN(t);
         }
@@ -268,33 +300,33 @@ __device__ __forceinline__ bool make_stencil(Stencil<D, SPL>& st, const double* 
 #pragma unroll
         for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) {
             if (SPL == 2) {
-                st.w[d][a] = shift ? w4[a + 1] : w4[a];
-                st.dw[d][a] = shift ? dw4[a + 1] : dw4[a];
-                st.xi[d][a] = shift ? xi4[a + 1] : xi4[a];
+                st.w[d][a] = MPM_SEL(shift) ? w4[a + 1] : w4[a];
+                st.dw[d][a] = MPM_SEL(shift) ? dw4[a + 1] : dw4[a];
+                st.xi[d][a] = MPM_SEL(shift) ? c4[d][a + 1] : c4[d][a];
             } else {
-                st.w[d][a] = w4[a]; st.dw[d][a] = dw4[a]; st.xi[d][a] = xi4[a];
+                st.w[d][a] = w4[a]; st.dw[d][a] = dw4[a]; st.xi[d][a] = c4[d][a];
             }
         }
     }
-    return true;
+    return ok;
 }
 
 // First node of the non-zero window per axis, as make_stencil chooses it (for the quadratic spline the
 // base node is skipped when its weight is exactly zero); false if the 4-node window leaves the grid.
-template <int D, int SPL>
-__device__ __forceinline__ bool window_base(int* wb, const double* xp, const double* org, const uint32_t* n, double* const* coord,
+template <int D, int SPL, class CS>
+__device__ __forceinline__ bool window_base(int* wb, const double* xp, const double* org, const uint32_t* n, const CS& cs,
                                             double h1, bool clamp) {
     int b[3] = {0, 0, 0};
-    if (!stencil_base<D>(b, xp, org, n, h1, clamp)) return false;
+    const bool ok = stencil_base<D>(b, xp, org, n, h1, clamp);
     for (int d = 0; d < D; ++d) {
         int s = 0;
         if (SPL == 2) {
-            const double t = (xp[d] - coord[d][b[d]]) * h1;
+            const double t = (xp[d] - cs(d, cs.clamp4(d, b[d]))) * h1;
             s = BSpline<2>::N(t) == 0.0 ? 1 : 0;
         }
         wb[d] = b[d] + s;
     }
-    return true;
+    return ok;
 }
 
 __device__ __forceinline__ uint32_t node_index3(int i, int j, int k, const uint32_t* n) { return ((uint32_t)i * n[1] + (uint32_t)j) * n[2] + (uint32_t)k; }
@@ -462,7 +494,7 @@ __device__ __forceinline__ uint32_t block_key(const double* xp, const StepState*
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
     int wb[3];
-    if (!window_base<D, SPL>(wb, xp, org, nn, g.coord, h1, true)) return 0u;
+    if (!window_base<D, SPL>(wb, xp, org, nn, GlobalCoords{g.coord, nn}, h1, true)) return 0u;
     uint32_t c[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) {
         const uint32_t b = (uint32_t)wb[d] / BlockEdge<D>::v;
@@ -542,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_binning(Params P, const StepState* S
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, int NV, class TF> struct ScatterShape {
     using G = BG<D, SPL>;
-    static constexpr int REGS = 80;                                    // 32-bit registers for partial sums
+    static constexpr int REGS = 40;                                    // 32-bit registers for partial sums
     static constexpr int WORDS = NV * (int)(sizeof(TF) / 4);
     static constexpr int SPP0 = REGS / (G::NS * WORDS);
     static constexpr int SPP = SPP0 < 1 ? 1 : (SPP0 > G::W ? G::W : SPP0);   // slices per wavefront
@@ -554,8 +586,32 @@ template <int D, int SPL, int NV, class TF> struct ScatterShape {
     static constexpr int TB = 64 * WAVES;                               // workgroup size
 };
 
+// Node coordinates around a block: axis d, nodes E*bc[d] - 1 .. E*bc[d] + E + 2 (every 4-node window of the
+// block's particles), zero outside the grid.
+template <int D, int SPL> struct BlockCoordLDS {
+    static constexpr int LC = BG<D, SPL>::E + 4;
+    double c[3][LC];
+    __device__ __forceinline__ void load(double* const* coord, const int* bc, const uint32_t* n) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int e = BG<D, SPL>::E * bc[d] - 1, nd = (int)n[d];
+            for (int i = threadIdx.x; i < LC; i += blockDim.x) {
+                const int gi = e + i;
+                c[d][i] = (gi >= 0 && gi < nd) ? coord[d][gi] : 0.0;
+            }
+        }
+    }
+    __device__ __forceinline__ LdsCoords<LC> view(const int* bc) const {
+        LdsCoords<LC> v;
+        v.c0 = c[0]; v.c1 = c[1]; v.c2 = c[2];
+        v.lo0 = BG<D, SPL>::E * bc[0] - 1; v.lo1 = BG<D, SPL>::E * bc[1] - 1; v.lo2 = BG<D, SPL>::E * bc[2] - 1;
+        return v;
+    }
+};
+
 template <int D, int SPL, int NV, class TF> struct ScatterLDS {
     using G = BG<D, SPL>;
+    BlockCoordLDS<D, SPL> coords;
     uint32_t p[kMaxBP];              // source slots of the block's particles, ordered by (cell, slot)
     uint32_t cstart[G::NC + 2];      // bucket c = cell c occupies p[cstart[c] .. cstart[c+1]); bucket NC: outside
     union {
@@ -571,11 +627,14 @@ __device__ __forceinline__ int tile_index(int t0, int t1, int t2) {
     return D == 3 ? (t0 * G::T + t1) * G::T + t2 : t0 * G::T + t1;
 }
 
-// Orders a block's particles by (dual cell, source slot) into L.p / L.cstart and writes the sorted
-// permutation, the destination ids and the cell offsets. Particles whose window is outside the block
-// (their window leaves the grid) go to the last bucket and are reported.
+// Orders a block's particles and writes the sorted permutation, the destination ids and the cell
+// offsets. Within a dual cell the particles are ranked by source slot; the block's slots then hold rank 0
+// of every non-empty cell in cell order, then rank 1 of every cell with at least two particles, and so
+// on, so that the 64 lanes of a scatter wavefront (one per cell) read consecutive slots at every step and
+// the next step's reads of this layout are coalesced. Particles whose window is outside the block (their
+// window leaves the grid) go last and are reported. L.cstart keeps the cell-major offsets (cell counts).
 template <int D, int SPL, int NV, class TF>
-__device__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, const Params& P, const StepState* S, const PartStateT<TF>& src,
+__device__ __forceinline__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, const Params& P, const StepState* S, const PartStateT<TF>& src,
                                 const PartStateT<TF>& dst, const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b, uint32_t start,
                                 uint32_t cnt, const int* bc) {
     using G = BG<D, SPL>;
@@ -592,7 +651,7 @@ __device__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, const Params& P, 
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
         int wb[3];
         int cell = G::NC;
-        if (window_base<D, SPL>(wb, xp, org, nn, g.coord, h1, true)) {
+        if (window_base<D, SPL>(wb, xp, org, nn, L.coords.view(bc), h1, true)) {
             int l[3] = {0, 0, 0};
             bool in = true;
             for (int d = 0; d < D; ++d) { l[d] = wb[d] - G::E * bc[d]; in = in && l[d] >= 0 && l[d] < G::E; }
@@ -631,6 +690,22 @@ __device__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, const Params& P, 
             L.p[j] = key;
         }
     }
+    __syncthreads();
+    for (uint32_t k = tid; k < cnt; k += blockDim.x) L.u.sort.tmp[k] = L.p[k];
+    __syncthreads();
+    if (tid < 64) {  // rank-major layout: one wavefront, lane = cell
+        const uint32_t c0 = L.cstart[tid], n_c = L.cstart[tid + 1] - c0;
+        const unsigned long long below = (1ull << tid) - 1ull;
+        uint32_t base = 0;
+        for (uint32_t j = 0;; ++j) {
+            const bool has = j < n_c;
+            const unsigned long long m = __ballot(has);
+            if (m == 0ull) break;
+            if (has) L.p[base + (uint32_t)__popcll(m & below)] = L.u.sort.tmp[c0 + j];
+            base += (uint32_t)__popcll(m);
+        }
+    }
+    for (uint32_t k = L.cstart[G::NC] + tid; k < cnt; k += blockDim.x) L.p[k] = L.u.sort.tmp[k];  // outside bucket last
     __syncthreads();
     for (uint32_t k = tid; k < cnt; k += blockDim.x) {
         const uint32_t p = L.p[k];
@@ -694,7 +769,7 @@ __device__ __forceinline__ void add_partials(ScatterLDS<D, SPL, NV, TF>& L, int 
 // contrib.load(p, i, part) fetches source particle p (sorted slot i); contrib.add(part, a0, half, acc, base)
 // adds it to acc for slices a0.. and second-axis half `half` and returns false if its window leaves the grid.
 template <int D, int SPL, int NV, class TF, class Contrib>
-__device__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, uint32_t b,
+__device__ __forceinline__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, uint32_t b,
                               uint32_t start) {
     using SS = ScatterShape<D, SPL, NV, TF>;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -706,13 +781,22 @@ __device__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& cont
         for (int o = 0; o < SS::NSH; ++o)
 #pragma unroll
             for (int v = 0; v < NV; ++v) acc[s][o][v] = TF(0);
-    {
-        const uint32_t k0 = L.cstart[lane], k1 = L.cstart[lane + 1];
+    {  // rank-major layout (see order_block): at rank j the cells with more than j particles are consecutive
+        const uint32_t n_c = L.cstart[lane + 1] - L.cstart[lane];
+        const uint32_t n_max = wave_max_u32(n_c);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        uint32_t kb = 0;
         int base[3];
-        for (uint32_t k = k0; k < k1; ++k) {
-            typename Contrib::Part q;
-            contrib.load(L.p[k], start + k, q);
-            (void)contrib.add(q, a0, half, acc, base);
+        for (uint32_t j = 0; j < n_max; ++j) {
+            const bool has = j < n_c;
+            const unsigned long long m = __ballot(has);
+            const uint32_t k = kb + (uint32_t)__popcll(m & below);
+            kb += (uint32_t)__popcll(m);
+            if (has) {
+                typename Contrib::Part q;
+                contrib.load(L.p[k], start + k, q);
+                (void)contrib.add(q, a0, half, acc, base);
+            }
         }
     }
     zero_tile<D, SPL, NV, TF>(L);
@@ -726,7 +810,7 @@ __device__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& cont
 
 // Tile of a block with more than kMaxBP particles: every particle adds its window with LDS atomics.
 template <int D, int SPL, int NV, class TF, class Contrib>
-__device__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, const SortBufs& sb,
+__device__ __forceinline__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, const SortBufs& sb,
                                          uint32_t b, uint32_t start, uint32_t cnt, const int* bc) {
     using G = BG<D, SPL>;
     using SS = ScatterShape<D, SPL, NV, TF>;
@@ -785,9 +869,9 @@ __device__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV, TF>& L, const Co
 
 // Weight, derivative and coordinate of window node `off` on axis d, evaluated as make_stencil does
 // (the same expression on the same operands gives the same bits) but for a run-time offset.
-template <int SPL>
-__device__ __forceinline__ void axis_node(double* const* coord, int d, int base, int off, double x, double h1, double& w, double& dw, double& xi) {
-    xi = coord[d][base + off];
+template <int SPL, class CS>
+__device__ __forceinline__ void axis_node(const CS& cs, int d, int base, int off, double x, double h1, double& w, double& dw, double& xi) {
+    xi = cs(d, base + off);
     const double t = (x - xi) * h1;
     w = BSpline<SPL>::N(t);
     dw = BSpline<SPL>::dN(t);
@@ -815,8 +899,8 @@ __device__ __forceinline__ void to_wstencil(WStencil<D, SPL, TF>& ws, const Sten
 // particle: weights, offsets to the particle and weight gradients in TF. f(s, ol, weight, pd, gr) is called
 // per node with the slice index s and the wavefront-local offset index ol. (For TF = double the values and
 // the order of the products are those of make_stencil and grad_wip.)
-template <int D, int SPL, int SPP, int OBN, class TF, class F>
-__device__ __forceinline__ void for_group(const WStencil<D, SPL, TF>& ws, const Stencil<D, SPL>& st, double* const* coord, const double* xp,
+template <int D, int SPL, int SPP, int OBN, class TF, class CS, class F>
+__device__ __forceinline__ void for_group(const WStencil<D, SPL, TF>& ws, const Stencil<D, SPL>& st, const CS& cs, const double* xp,
                                           double h1, int a0, int half, F&& f) {
     constexpr int W = BG<D, SPL>::W;
     const TF h1t = (TF)h1;
@@ -826,12 +910,12 @@ __device__ __forceinline__ void for_group(const WStencil<D, SPL, TF>& ws, const 
         if (a >= W) break;
         TF w0, dw0, pd0;
         if (SPP == W) { w0 = ws.w[0][s]; dw0 = ws.dw[0][s]; pd0 = ws.pd[0][s]; }
-        else { double w, dw, xi; axis_node<SPL>(coord, 0, st.base[0], a, xp[0], h1, w, dw, xi); w0 = (TF)w; dw0 = (TF)dw; pd0 = (TF)(xi - xp[0]); }
+        else { double w, dw, xi; axis_node<SPL>(cs, 0, st.base[0], a, xp[0], h1, w, dw, xi); w0 = (TF)w; dw0 = (TF)dw; pd0 = (TF)(xi - xp[0]); }
 #pragma unroll
         for (int obl = 0; obl < OBN; ++obl) {
             TF w1, dw1, pd1;
             if (OBN == W) { w1 = ws.w[1][obl]; dw1 = ws.dw[1][obl]; pd1 = ws.pd[1][obl]; }
-            else { double w, dw, xi; axis_node<SPL>(coord, 1, st.base[1], half * OBN + obl, xp[1], h1, w, dw, xi); w1 = (TF)w; dw1 = (TF)dw; pd1 = (TF)(xi - xp[1]); }
+            else { double w, dw, xi; axis_node<SPL>(cs, 1, st.base[1], half * OBN + obl, xp[1], h1, w, dw, xi); w1 = (TF)w; dw1 = (TF)dw; pd1 = (TF)(xi - xp[1]); }
             if (D == 3) {
 #pragma unroll
                 for (int oc = 0; oc < W; ++oc) {
@@ -863,7 +947,7 @@ struct P2GContrib {
     using SS = ScatterShape<D, SPL, NV, TF>;
     struct Part { double x[3]; TF v[3], B[9], t6[6]; double muI; };
     PartStateT<TF> src;
-    double* coord[3];
+    LdsCoords<BG<D, SPL>::E + 4> cs;
     double org[3];
     uint32_t nn[3];
     double h1, aD, kf;
@@ -876,7 +960,7 @@ struct P2GContrib {
     }
     __device__ __forceinline__ bool add(const Part& q, int a0, int half, TF (&acc)[SS::SPP][SS::NSH][NV], int* base) const {
         Stencil<D, SPL> st;
-        if (!make_stencil<D, SPL>(st, q.x, org, nn, coord, h1, true)) return false;
+        if (!make_stencil<D, SPL>(st, q.x, org, nn, cs, h1, true)) return false;
         for (int d = 0; d < D; ++d) base[d] = st.base[d];
         TF tau[3][3];
         if (D == 3) { tau[0][0] = q.t6[0]; tau[1][1] = q.t6[1]; tau[2][2] = q.t6[2]; tau[0][1] = tau[1][0] = q.t6[3]; tau[0][2] = tau[2][0] = q.t6[4]; tau[1][2] = tau[2][1] = q.t6[5]; }
@@ -884,7 +968,7 @@ struct P2GContrib {
         const TF aDt = (TF)aD, kft = (TF)kf;
         WStencil<D, SPL, TF> ws;
         to_wstencil<D, SPL, TF>(ws, st, q.x);
-        for_group<D, SPL, SS::SPP, SS::OBN, TF>(ws, st, coord, q.x, h1, a0, half, [&](int sl, int ol, TF w, const TF* pd, const TF* gr) __attribute__((always_inline)) {
+        for_group<D, SPL, SS::SPP, SS::OBN, TF>(ws, st, cs, q.x, h1, a0, half, [&](int sl, int ol, TF w, const TF* pd, const TF* gr) __attribute__((always_inline)) {
             const bool massive = w > TF(1e-25);
             TF fr[3];
             for (int r = 0; r < D; ++r) {
@@ -919,7 +1003,7 @@ struct MuslContrib {
     using SS = ScatterShape<D, SPL, NV, TF>;
     struct Part { double x[3]; TF v[3], B[9]; };
     PartStateT<TF> src, dst;
-    double* coord[3];
+    LdsCoords<BG<D, SPL>::E + 4> cs;
     double org[3];
     uint32_t nn[3];
     double h1, aD;
@@ -930,12 +1014,12 @@ struct MuslContrib {
     }
     __device__ __forceinline__ bool add(const Part& q, int a0, int half, TF (&acc)[SS::SPP][SS::NSH][NV], int* base) const {
         Stencil<D, SPL> st;
-        if (!make_stencil<D, SPL>(st, q.x, org, nn, coord, h1, true)) return false;
+        if (!make_stencil<D, SPL>(st, q.x, org, nn, cs, h1, true)) return false;
         for (int d = 0; d < D; ++d) base[d] = st.base[d];
         const TF aDt = (TF)aD;
         WStencil<D, SPL, TF> ws;
         to_wstencil<D, SPL, TF>(ws, st, q.x);
-        for_group<D, SPL, SS::SPP, SS::OBN, TF>(ws, st, coord, q.x, h1, a0, half, [&](int sl, int ol, TF w, const TF* pd, const TF*) __attribute__((always_inline)) {
+        for_group<D, SPL, SS::SPP, SS::OBN, TF>(ws, st, cs, q.x, h1, a0, half, [&](int sl, int ol, TF w, const TF* pd, const TF*) __attribute__((always_inline)) {
             if (!(w > TF(1e-25))) return;
             for (int r = 0; r < D; ++r) {
                 TF m = q.v[r] * w;
@@ -971,7 +1055,10 @@ void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortB
     block_coords<D>(b, S, bc);
     P2GContrib<D, SPL, SEP, MIBF, TF> cf;
     cf.src = src;
-    for (int d = 0; d < 3; ++d) { cf.coord[d] = g.coord[d]; cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
+    for (int d = 0; d < 3; ++d) { cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
+    L.coords.load(g.coord, bc, cf.nn);
+    cf.cs = L.coords.view(bc);
+    __syncthreads();
     cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
     cf.kf = S->dt * P.particle_volume / P.particle_mass;
     uint32_t err = 0;
@@ -1011,7 +1098,9 @@ void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> d
     block_coords<D>(b, S, bc);
     MuslContrib<D, SPL, TF> cf;
     cf.src = src; cf.dst = dst;
-    for (int d = 0; d < 3; ++d) { cf.coord[d] = g.coord[d]; cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
+    for (int d = 0; d < 3; ++d) { cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
+    L.coords.load(g.coord, bc, cf.nn);
+    cf.cs = L.coords.view(bc);
     cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
     uint32_t err = 0;
     if (cnt > kMaxBP) {
@@ -1205,11 +1294,11 @@ __device__ __forceinline__ void load_tile(TF* s, TF* const* src, int ncomp, cons
 }
 
 // Stencil of a gathering particle (G2P does not clamp, g2p.cpp:29-32) and its tile-local base.
-template <int D, int SPL>
+template <int D, int SPL, class CS>
 __device__ __forceinline__ bool gather_stencil(Stencil<D, SPL>& st, int* tl, const double* xp, const double* org, const uint32_t* n,
-                                               double* const* coord, double h1, const int* bc) {
+                                               const CS& cs, double h1, const int* bc) {
     using G = BG<D, SPL>;
-    if (!make_stencil<D, SPL>(st, xp, org, n, coord, h1, false)) return false;
+    if (!make_stencil<D, SPL>(st, xp, org, n, cs, h1, false)) return false;
     bool in = true;
     tl[2] = 0;
     for (int d = 0; d < D; ++d) { tl[d] = st.base[d] - G::E * bc[d]; in = in && tl[d] >= 0 && tl[d] < G::E; }
@@ -1282,6 +1371,7 @@ template <int D, int SPL, class TF>
 __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
     using G = BG<D, SPL>;
     __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
+    __shared__ BlockCoordLDS<D, SPL> s_c;
     BlockRange br;
     if (!block_range<D>(S, sb, br)) return;
     const double h1 = P.one_over_dx, dt = S->dt;
@@ -1290,6 +1380,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, Part
     const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
     load_tile<D, SPL, TF>(s_v, g.v, D, br.bc, nn);
     if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, D, br.bc, nn);
+    s_c.load(g.coord, br.bc, nn);
     __syncthreads();
     uint32_t err = 0, pcount = 0;
     BoundsAcc ba;
@@ -1299,7 +1390,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, Part
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
         Stencil<D, SPL> st;
         int tl[3];
-        if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, g.coord, h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
+        if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, s_c.view(br.bc), h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
         TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gvt[3][3] = {{0}};
         gather<D, SPL, true, true, TF>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gvt);
         double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0}, gv[3][3];
@@ -1331,6 +1422,7 @@ template <int D, int SPL, class TF>
 __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
     using G = BG<D, SPL>;
     __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
+    __shared__ BlockCoordLDS<D, SPL> s_c;
     BlockRange br;
     if (!block_range<D>(S, sb, br)) return;
     const double h1 = P.one_over_dx, dt = S->dt;
@@ -1339,6 +1431,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
     const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
     load_tile<D, SPL, TF>(s_v, g.v, D, br.bc, nn);
     if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, D, br.bc, nn);
+    s_c.load(g.coord, br.bc, nn);
     __syncthreads();
     uint32_t err = 0;
     BoundsAcc ba;
@@ -1348,7 +1441,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
         Stencil<D, SPL> st;
         int tl[3];
-        if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, g.coord, h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
+        if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, s_c.view(br.bc), h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
         TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3];
         gather<D, SPL, true, false, TF>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gv);
         double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
@@ -1414,7 +1507,7 @@ __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, Part
         double xp[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
         Stencil<D, SPL> st;
-        if (!make_stencil<D, SPL>(st, xp, org, nn, g.coord, h1, false)) {
+        if (!make_stencil<D, SPL>(st, xp, org, nn, GlobalCoords{g.coord, nn}, h1, false)) {
             err |= MPM_ERR_GRID_BOUNDS;
         } else {
             TF gvt[3][3] = {{0}};

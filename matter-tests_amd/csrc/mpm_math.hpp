@@ -25,34 +25,48 @@ namespace mpm {
 MPM_HD int sgn_d(double v) { return (0.0 < v) - (v < 0.0); }
 
 template <int SPL> struct BSpline;
+// The piecewise formulas are evaluated unconditionally and selected (the selects are marked unpredictable
+// so that the compiler keeps them as selects rather than divergent branches); each piece is the same
+// expression, with the same operations in the same order, as in the reference.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MPM_SEL(c) __builtin_unpredictable(c)
+#else
+#define MPM_SEL(c) (c)
+#endif
 template <> struct BSpline<2> {
     static constexpr double apic_factor = 4.0;  // apicDinverse = 4/dx^2 (simulation.cpp:98)
     MPM_HD static double N(double u) {
-        double a = fabs(u);
-        if (a < 0.5) return 0.75 - a * a;
-        if (a < 1.5) return 0.5 * (1.5 - a) * (1.5 - a);
-        return 0.0;
+        const double a = fabs(u);
+        const double inner = 0.75 - a * a;
+        const double outer = 0.5 * (1.5 - a) * (1.5 - a);
+        const double r = MPM_SEL(a < 1.5) ? outer : 0.0;
+        return MPM_SEL(a < 0.5) ? inner : r;
     }
     MPM_HD static double dN(double u) {
-        double a = fabs(u);
-        if (a < 0.5) return -2 * u;
-        if (a < 1.5) return u - 1.5 * sgn_d(u);
-        return 0.0;
+        const double a = fabs(u);
+        const double inner = -2 * u;
+        const double sg = MPM_SEL(u > 0.0) ? 1.0 : (MPM_SEL(u < 0.0) ? -1.0 : 0.0);
+        const double outer = u - 1.5 * sg;
+        const double r = MPM_SEL(a < 1.5) ? outer : 0.0;
+        return MPM_SEL(a < 0.5) ? inner : r;
     }
 };
 template <> struct BSpline<3> {
     static constexpr double apic_factor = 3.0;  // apicDinverse = 3/dx^2 (simulation.cpp:95)
     MPM_HD static double N(double u) {
-        double a = fabs(u);
-        if (a < 1.0) return 0.5 * a * a * a - a * a + 0.6666666666666666666666666666666666666666666666666;
-        if (a < 2.0) return 0.1666666666666666666666666666666666666666666 * (2.0 - a) * (2.0 - a) * (2.0 - a);
-        return 0.0;
+        const double a = fabs(u);
+        const double inner = 0.5 * a * a * a - a * a + 0.6666666666666666666666666666666666666666666666666;
+        const double outer = 0.1666666666666666666666666666666666666666666 * (2.0 - a) * (2.0 - a) * (2.0 - a);
+        const double r = MPM_SEL(a < 2.0) ? outer : 0.0;
+        return MPM_SEL(a < 1.0) ? inner : r;
     }
     MPM_HD static double dN(double u) {
-        double a = fabs(u);
-        if (a < 1.0) return u * (1.5 * a - 2.0);
-        if (a < 2.0) return -0.5 * sgn_d(u) * (2.0 - a) * (2.0 - a);
-        return 0.0;
+        const double a = fabs(u);
+        const double inner = u * (1.5 * a - 2.0);
+        const double sg = MPM_SEL(u > 0.0) ? 1.0 : (MPM_SEL(u < 0.0) ? -1.0 : 0.0);
+        const double outer = -0.5 * sg * (2.0 - a) * (2.0 - a);
+        const double r = MPM_SEL(a < 2.0) ? outer : 0.0;
+        return MPM_SEL(a < 1.0) ? inner : r;
     }
 };
 
