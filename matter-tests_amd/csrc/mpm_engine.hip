@@ -104,6 +104,9 @@ void collect_timings(mpm_context* c) {
 namespace {
 
 constexpr uint32_t kBatch = 32;
+// Arrays are padded so that consecutive ones start 4 KiB (floats) or 8 KiB (doubles) apart modulo 64 KiB:
+// equal indices of different arrays then fall into different memory channels.
+constexpr size_t kArrayPad = 1024;
 
 #define HIP_TRY(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return MPM_ERR_HIP; } while (0)
 
@@ -139,6 +142,9 @@ void derive_params(const mpm_config& c, Params& P) {
     P.q_max = c.q_max; P.q_min = c.q_min; P.p_min = c.p_min; P.M = c.M; P.q_cohesion = c.q_cohesion;
     P.perzyna_exp = c.perzyna_exp; P.perzyna_visc = c.perzyna_visc; P.beta = c.beta; P.p0 = c.p0;
     P.rho_s = c.rho_s; P.grain_diameter = c.grain_diameter; P.I_ref = c.I_ref; P.mu_1 = c.mu_1; P.mu_2 = c.mu_2;
+    // The split form's transfers carry about 1e-7 relative error; rotations below 1e-12 of the largest
+    // diagonal entry change its stresses far less than that, and skipping them saves sweeps.
+    P.svd_tol = c.precision == MPM_PRECISION_SPLIT ? 1e-12 : kSvdTolEigen;
 }
 
 size_t tile_nodes(int D, int spl) {
@@ -153,7 +159,7 @@ size_t layout_bytes(int D, bool split) {
 
 int alloc_particles(mpm_context* c, uint32_t n) {
     const int D = c->cfg.dim;
-    const size_t need = 2 * layout_bytes(D, c->split) * n + 64;
+    const size_t need = 2 * layout_bytes(D, c->split) * ((size_t)n + kArrayPad) + 64;
     const size_t ineed = (size_t)4 * n;  // id x2, key, perm
     if (!(c->pbuf && c->pbuf_bytes >= need)) {
         if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
@@ -172,8 +178,9 @@ template <class TF>
 void layout_particles(mpm_context* c, uint32_t n) {
     const int D = c->cfg.dim;
     char* q = c->pbuf;
-    auto taked = [&]() { double* r = (double*)q; q += sizeof(double) * n; return r; };
-    auto taket = [&]() { TF* r = (TF*)q; q += sizeof(TF) * n; return r; };
+    const size_t np = (size_t)n + kArrayPad;
+    auto taked = [&]() { double* r = (double*)q; q += sizeof(double) * np; return r; };
+    auto taket = [&]() { TF* r = (TF*)q; q += sizeof(TF) * np; return r; };
     for (int s = 0; s < 2; ++s) {
         PartStateT<TF> ps{};
         for (int d = 0; d < D; ++d) ps.x[d] = taked();
@@ -287,12 +294,16 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     else enqueue_p2g<D, SPL, false, false, TF>(c, src, dst, gblocks);
     if (P.use_musl) {
         constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
-        { LaunchTimer t(c, KS_G2P_MUSL); k_g2p_musl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g); }
+        {
+            LaunchTimer t(c, KS_G2P_MUSL);
+            k_g2p_musl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
+        }
         { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->cap_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
         { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL, TF><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, g, stg); }
         { LaunchTimer t(c, KS_DEFGRAD); k_defgrad<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g, n); }
     } else {
-        LaunchTimer t(c, KS_G2P_USL); k_g2p_usl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
+        LaunchTimer t(c, KS_G2P_USL);
+        k_g2p_usl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
     }
     { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
 }

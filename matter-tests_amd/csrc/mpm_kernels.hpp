@@ -877,70 +877,69 @@ __device__ __forceinline__ void axis_node(const CS& cs, int d, int base, int off
     dw = BSpline<SPL>::dN(t);
 }
 
-// Stencil values in the transfer type: weight, derivative and node-minus-particle offset per axis node.
-template <int D, int SPL, class TF> struct WStencil {
+// Multiply-add used by the scatter arithmetic (a fused multiply-add in both forms).
+template <class TF> __device__ __forceinline__ TF madd(TF a, TF b, TF c) { return fma(a, b, c); }
+
+// The window of one particle as a scatter wavefront needs it: the first non-zero node wb per axis
+// (make_stencil's base, with the same shift test), and weight, derivative and node-minus-particle offset of
+// the first-axis nodes a0 .. a0+SPP-1, the second-axis nodes of `half` and all third-axis nodes. Every
+// value is the expression make_stencil evaluates on the same operands; they are rounded to TF.
+template <int D, int SPL, int SPP, int OBN, class TF>
+struct SliceStencil {
     static constexpr int W = BG<D, SPL>::W;
-    TF w[3][W], dw[3][W], pd[3][W];
+    int wb[3];
+    TF w0[SPP], dw0[SPP], pd0[SPP];
+    TF w1[OBN], dw1[OBN], pd1[OBN];
+    TF w2[W], dw2[W], pd2[W];
 };
 
-template <int D, int SPL, class TF>
-__device__ __forceinline__ void to_wstencil(WStencil<D, SPL, TF>& ws, const Stencil<D, SPL>& st, const double* xp) {
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-#pragma unroll
-        for (int a = 0; a < BG<D, SPL>::W; ++a) {
-            ws.w[d][a] = (TF)st.w[d][a];
-            ws.dw[d][a] = (TF)st.dw[d][a];
-            ws.pd[d][a] = (TF)(st.xi[d][a] - xp[d]);
-        }
+template <int SPL, class CS, class TF>
+__device__ __forceinline__ void node_vals(const CS& cs, int d, int idx, double x, double h1, TF& w, TF& dw, TF& pd) {
+    const double xi = cs(d, idx);
+    const double t = (x - xi) * h1;
+    w = (TF)BSpline<SPL>::N(t);
+    dw = (TF)BSpline<SPL>::dN(t);
+    pd = (TF)(xi - x);
 }
 
-// The window nodes of slices a0 .. a0+SPP-1 (second-axis offsets of `half`, all third-axis offsets) of one
-// particle: weights, offsets to the particle and weight gradients in TF. f(s, ol, weight, pd, gr) is called
-// per node with the slice index s and the wavefront-local offset index ol. (For TF = double the values and
-// the order of the products are those of make_stencil and grad_wip.)
-template <int D, int SPL, int SPP, int OBN, class TF, class CS, class F>
-__device__ __forceinline__ void for_group(const WStencil<D, SPL, TF>& ws, const Stencil<D, SPL>& st, const CS& cs, const double* xp,
-                                          double h1, int a0, int half, F&& f) {
+template <int D, int SPL, int SPP, int OBN, class TF, class CS>
+__device__ __forceinline__ bool slice_stencil(SliceStencil<D, SPL, SPP, OBN, TF>& ss, const double* xp, const double* org,
+                                              const uint32_t* n, const CS& cs, double h1, bool clamp, int a0, int half) {
     constexpr int W = BG<D, SPL>::W;
-    const TF h1t = (TF)h1;
+    int b[3] = {0, 0, 0};
+    const bool ok = stencil_base<D>(b, xp, org, n, h1, clamp);
+    int i0[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int bb = cs.clamp4(d, b[d]);
+        int sh = 0;
+        if (SPL == 2) {  // make_stencil skips the base node when its weight is exactly zero, i.e. |t| >= 1.5
+            const double t = (xp[d] - cs(d, bb)) * h1;
+            sh = MPM_SEL(fabs(t) < 1.5) ? 0 : 1;
+        }
+        ss.wb[d] = b[d] + sh;
+        i0[d] = bb + sh;
+    }
 #pragma unroll
     for (int s = 0; s < SPP; ++s) {
-        const int a = a0 + s;
-        if (a >= W) break;
-        TF w0, dw0, pd0;
-        if (SPP == W) { w0 = ws.w[0][s]; dw0 = ws.dw[0][s]; pd0 = ws.pd[0][s]; }
-        else { double w, dw, xi; axis_node<SPL>(cs, 0, st.base[0], a, xp[0], h1, w, dw, xi); w0 = (TF)w; dw0 = (TF)dw; pd0 = (TF)(xi - xp[0]); }
-#pragma unroll
-        for (int obl = 0; obl < OBN; ++obl) {
-            TF w1, dw1, pd1;
-            if (OBN == W) { w1 = ws.w[1][obl]; dw1 = ws.dw[1][obl]; pd1 = ws.pd[1][obl]; }
-            else { double w, dw, xi; axis_node<SPL>(cs, 1, st.base[1], half * OBN + obl, xp[1], h1, w, dw, xi); w1 = (TF)w; dw1 = (TF)dw; pd1 = (TF)(xi - xp[1]); }
-            if (D == 3) {
-#pragma unroll
-                for (int oc = 0; oc < W; ++oc) {
-                    const TF w2 = ws.w[2][oc], dw2 = ws.dw[2][oc];
-                    TF weight = w0 * w1;
-                    weight = weight * w2;
-                    const TF pd[3] = {pd0, pd1, ws.pd[2][oc]};
-                    const TF gr[3] = {dw0 * w1 * w2 * h1t, dw1 * w0 * w2 * h1t, dw2 * w0 * w1 * h1t};
-                    f(s, obl * W + oc, weight, pd, gr);
-                }
-            } else {
-                const TF weight = w0 * w1;
-                const TF pd[3] = {pd0, pd1, TF(0)};
-                const TF gr[3] = {dw0 * w1 * h1t, dw1 * w0 * h1t, TF(0)};
-                f(s, obl, weight, pd, gr);
-            }
-        }
+        const int a = min(a0 + s, W - 1);
+        node_vals<SPL>(cs, 0, i0[0] + a, xp[0], h1, ss.w0[s], ss.dw0[s], ss.pd0[s]);
     }
+#pragma unroll
+    for (int o = 0; o < OBN; ++o) node_vals<SPL>(cs, 1, i0[1] + half * OBN + o, xp[1], h1, ss.w1[o], ss.dw1[o], ss.pd1[o]);
+    if (D == 3) {
+#pragma unroll
+        for (int o = 0; o < W; ++o) node_vals<SPL>(cs, 2, i0[2] + o, xp[2], h1, ss.w2[o], ss.dw2[o], ss.pd2[o]);
+    }
+    return ok;
 }
 
 // P2G contributions (p2g.cpp:12-66 and the force of explicit_euler_update.cpp:15-168): value 0 sum of
 // weights, 1..D momentum sum w (v + B (x_i - x_p) D^-1) (weight > 1e-25 only), then either the force sum
 // tau grad w in separate values (SEP) or the force folded into the momentum as - dt V0 / m_p tau grad w,
-// and last the MIBF friction sum w muI. load() fetches a particle, add() computes its stencil and adds
-// it, so that a worker can fetch the next particle while it adds the current one.
+// and last the MIBF friction sum w muI. The products are arranged separably: B (x_i - x_p) D^-1 =
+// sum_k D^-1 B[:,k] (x_i - x_p)_k with one term per axis node, and grad w = h1 (dw0 w1 w2, w0 dw1 w2,
+// w0 w1 dw2) with h1 folded into the stress.
 template <int D, int SPL, bool SEP, bool MIBF, class TF>
 struct P2GContrib {
     static constexpr int NV = 1 + D + (SEP ? D : 0) + (MIBF ? 1 : 0);
@@ -959,44 +958,72 @@ struct P2GContrib {
         q.muI = MIBF ? src.muI[p] : 0.0;
     }
     __device__ __forceinline__ bool add(const Part& q, int a0, int half, TF (&acc)[SS::SPP][SS::NSH][NV], int* base) const {
-        Stencil<D, SPL> st;
-        if (!make_stencil<D, SPL>(st, q.x, org, nn, cs, h1, true)) return false;
-        for (int d = 0; d < D; ++d) base[d] = st.base[d];
+        constexpr int W = BG<D, SPL>::W;
+        SliceStencil<D, SPL, SS::SPP, SS::OBN, TF> ss;
+        const bool ok = slice_stencil(ss, q.x, org, nn, cs, h1, true, a0, half);
+        for (int d = 0; d < D; ++d) base[d] = ss.wb[d];
+        // stress columns scaled by h1 (force) or by -dt V0 / m_p h1 (folded into the momentum)
         TF tau[3][3];
         if (D == 3) { tau[0][0] = q.t6[0]; tau[1][1] = q.t6[1]; tau[2][2] = q.t6[2]; tau[0][1] = tau[1][0] = q.t6[3]; tau[0][2] = tau[2][0] = q.t6[4]; tau[1][2] = tau[2][1] = q.t6[5]; }
         else { tau[0][0] = q.t6[0]; tau[1][1] = q.t6[1]; tau[0][1] = tau[1][0] = q.t6[2]; }
-        const TF aDt = (TF)aD, kft = (TF)kf;
-        WStencil<D, SPL, TF> ws;
-        to_wstencil<D, SPL, TF>(ws, st, q.x);
-        for_group<D, SPL, SS::SPP, SS::OBN, TF>(ws, st, cs, q.x, h1, a0, half, [&](int sl, int ol, TF w, const TF* pd, const TF* gr) __attribute__((always_inline)) {
-            const bool massive = w > TF(1e-25);
-            TF fr[3];
-            for (int r = 0; r < D; ++r) {
-                TF s = tau[r][0] * gr[0];
-                for (int k = 1; k < D; ++k) s += tau[r][k] * gr[k];
-                fr[r] = s;
-            }
-            acc[sl][ol][0] += massive ? w : TF(0);
-            for (int r = 0; r < D; ++r) {
-                TF m = q.v[r] * w;
-                if (apic) {
-                    TF bd = q.B[r] * pd[0];
-                    for (int k = 1; k < D; ++k) bd += q.B[r + k * D] * pd[k];
-                    m += bd * aDt * w;
+        const TF fs = SEP ? (TF)h1 : (TF)(-kf * h1);
+        TF T[3][3];
+        for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) T[r][k] = tau[r][k] * fs;
+        const TF aDt = (TF)aD;
+        TF Bc[3][3];  // D^-1 B[:,k]
+        for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) Bc[r][k] = q.B[r + k * D] * aDt;
+        const TF muI = (TF)q.muI;
+#pragma unroll
+        for (int s = 0; s < SS::SPP; ++s) {
+            if (a0 + s >= W) break;
+            TF u0[3];
+            for (int r = 0; r < D; ++r) u0[r] = madd(Bc[r][0], ss.pd0[s], q.v[r]);
+#pragma unroll
+            for (int o1 = 0; o1 < SS::OBN; ++o1) {
+                const TF w01 = ss.w0[s] * ss.w1[o1], dw0w1 = ss.dw0[s] * ss.w1[o1], w0dw1 = ss.w0[s] * ss.dw1[o1];
+                TF u01[3];
+                for (int r = 0; r < D; ++r) u01[r] = madd(Bc[r][1], ss.pd1[o1], u0[r]);
+                if (D == 3) {
+#pragma unroll
+                    for (int o2 = 0; o2 < W; ++o2) {
+                        const TF wt = w01 * ss.w2[o2];
+                        const TF g[3] = {dw0w1 * ss.w2[o2], w0dw1 * ss.w2[o2], w01 * ss.dw2[o2]};
+                        const TF wm = MPM_SEL(wt > TF(1e-25)) ? wt : TF(0);
+                        TF* a = acc[s][o1 * W + o2];
+                        a[0] += wm;
+                        for (int r = 0; r < D; ++r) {
+                            const TF u = madd(Bc[r][2], ss.pd2[o2], u01[r]);
+                            TF f = T[r][0] * g[0];
+                            f = madd(T[r][1], g[1], f);
+                            f = madd(T[r][2], g[2], f);
+                            if (SEP) { a[1 + r] = madd(wm, u, a[1 + r]); a[1 + D + r] += f; }
+                            else a[1 + r] = madd(wm, u, a[1 + r] + f);
+                        }
+                        if (MIBF) a[NV - 1] = madd(wm, muI, a[NV - 1]);
+                    }
+                } else {
+                    const TF wt = w01;
+                    const TF g[2] = {dw0w1, w0dw1};
+                    const TF wm = MPM_SEL(wt > TF(1e-25)) ? wt : TF(0);
+                    TF* a = acc[s][o1];
+                    a[0] += wm;
+                    for (int r = 0; r < D; ++r) {
+                        TF f = T[r][0] * g[0];
+                        f = madd(T[r][1], g[1], f);
+                        if (SEP) { a[1 + r] = madd(wm, u01[r], a[1 + r]); a[1 + D + r] += f; }
+                        else a[1 + r] = madd(wm, u01[r], a[1 + r] + f);
+                    }
+                    if (MIBF) a[NV - 1] = madd(wm, muI, a[NV - 1]);
                 }
-                m = massive ? m : TF(0);
-                if (SEP) { acc[sl][ol][1 + r] += m; acc[sl][ol][1 + D + r] += fr[r]; }
-                else acc[sl][ol][1 + r] += m - kft * fr[r];
             }
-            if (MIBF) acc[sl][ol][NV - 1] += massive ? (TF)q.muI * w : TF(0);
-        });
-        return true;
+        }
+        return ok;
     }
 };
 
 // MUSL re-scatter contributions (musl.cpp:28-85): sum w (v_new + B_new (x_i - x_p) D^-1) over
 // weight > 1e-25, with the position of the step's start (source layout, slot p) and the new velocity
-// and affine matrix (destination layout, sorted slot i).
+// and affine matrix (destination layout, sorted slot i), arranged separably as in P2GContrib.
 template <int D, int SPL, class TF>
 struct MuslContrib {
     static constexpr int NV = D;
@@ -1013,25 +1040,39 @@ struct MuslContrib {
         for (int k = 0; k < D * D; ++k) q.B[k] = apic ? dst.B[k][i] : TF(0);
     }
     __device__ __forceinline__ bool add(const Part& q, int a0, int half, TF (&acc)[SS::SPP][SS::NSH][NV], int* base) const {
-        Stencil<D, SPL> st;
-        if (!make_stencil<D, SPL>(st, q.x, org, nn, cs, h1, true)) return false;
-        for (int d = 0; d < D; ++d) base[d] = st.base[d];
+        constexpr int W = BG<D, SPL>::W;
+        SliceStencil<D, SPL, SS::SPP, SS::OBN, TF> ss;
+        const bool ok = slice_stencil(ss, q.x, org, nn, cs, h1, true, a0, half);
+        for (int d = 0; d < D; ++d) base[d] = ss.wb[d];
         const TF aDt = (TF)aD;
-        WStencil<D, SPL, TF> ws;
-        to_wstencil<D, SPL, TF>(ws, st, q.x);
-        for_group<D, SPL, SS::SPP, SS::OBN, TF>(ws, st, cs, q.x, h1, a0, half, [&](int sl, int ol, TF w, const TF* pd, const TF*) __attribute__((always_inline)) {
-            if (!(w > TF(1e-25))) return;
-            for (int r = 0; r < D; ++r) {
-                TF m = q.v[r] * w;
-                if (apic) {
-                    TF bd = q.B[r] * pd[0];
-                    for (int k = 1; k < D; ++k) bd += q.B[r + k * D] * pd[k];
-                    m += bd * aDt * w;
+        TF Bc[3][3];
+        for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) Bc[r][k] = q.B[r + k * D] * aDt;
+#pragma unroll
+        for (int s = 0; s < SS::SPP; ++s) {
+            if (a0 + s >= W) break;
+            TF u0[3];
+            for (int r = 0; r < D; ++r) u0[r] = madd(Bc[r][0], ss.pd0[s], q.v[r]);
+#pragma unroll
+            for (int o1 = 0; o1 < SS::OBN; ++o1) {
+                const TF w01 = ss.w0[s] * ss.w1[o1];
+                TF u01[3];
+                for (int r = 0; r < D; ++r) u01[r] = madd(Bc[r][1], ss.pd1[o1], u0[r]);
+                if (D == 3) {
+#pragma unroll
+                    for (int o2 = 0; o2 < W; ++o2) {
+                        const TF wt = w01 * ss.w2[o2];
+                        const TF wm = MPM_SEL(wt > TF(1e-25)) ? wt : TF(0);
+                        TF* a = acc[s][o1 * W + o2];
+                        for (int r = 0; r < D; ++r) a[r] = madd(wm, madd(Bc[r][2], ss.pd2[o2], u01[r]), a[r]);
+                    }
+                } else {
+                    const TF wm = MPM_SEL(w01 > TF(1e-25)) ? w01 : TF(0);
+                    TF* a = acc[s][o1];
+                    for (int r = 0; r < D; ++r) a[r] = madd(wm, u01[r], a[r]);
                 }
-                acc[sl][ol][r] += m;
             }
-        });
-        return true;
+        }
+        return ok;
     }
 };
 
@@ -1243,17 +1284,17 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
         neohookean_tau<D>(F, P.mu, P.lambda, tau);
     } else {
         M<D> U, W; V<D> s;
-        svd<D>(F, U, s, W);
+        svd<D>(F, U, s, W, P.svd_tol);
         V<D> ls;
         if (P.plastic_model != MPM_PLASTIC_NONE) {
             V<D> h; for (int k = 0; k < D; ++k) h[k] = log(fmax(fabs(s[k]), 1e-4));
             RMResult r = return_map<D>(P, dt, h, ps);
             err |= r.err;
             pcount += r.plastic_incr;
-            if (r.recompose) {
+            if (r.recompose) {  // the new F = U exp(h) W^T has singular values exp(h): its Hencky strain is h
                 V<D> e; for (int k = 0; k < D; ++k) e[k] = exp(h[k]);
                 F = udvt<D>(U, e, W);
-                for (int k = 0; k < D; ++k) ls[k] = log(e[k]);
+                for (int k = 0; k < D; ++k) ls[k] = h[k];
             } else {
                 for (int k = 0; k < D; ++k) ls[k] = log(fabs(s[k]));
             }
@@ -1293,24 +1334,46 @@ __device__ __forceinline__ void load_tile(TF* s, TF* const* src, int ncomp, cons
     }
 }
 
-// Stencil of a gathering particle (G2P does not clamp, g2p.cpp:29-32) and its tile-local base.
+// Full window of a gathering particle (G2P does not clamp, g2p.cpp:29-32) and its tile-local base.
 template <int D, int SPL, class CS>
 __device__ __forceinline__ bool gather_stencil(Stencil<D, SPL>& st, int* tl, const double* xp, const double* org, const uint32_t* n,
                                                const CS& cs, double h1, const int* bc) {
     using G = BG<D, SPL>;
-    if (!make_stencil<D, SPL>(st, xp, org, n, cs, h1, false)) return false;
+    const bool ok = make_stencil<D, SPL>(st, xp, org, n, cs, h1, false);
     bool in = true;
     tl[2] = 0;
     for (int d = 0; d < D; ++d) { tl[d] = st.base[d] - G::E * bc[d]; in = in && tl[d] >= 0 && tl[d] < G::E; }
-    return in;
+    return ok && in;
 }
 
+// Node values for gathers: from the LDS tile of a block (tile-local window base t0..t2) or from the grid.
+template <int D, int SPL, class TF> struct TileFetch {
+    const TF* v; const TF* f;
+    int t0, t1, t2;
+    __device__ __forceinline__ void get(int a, int b, int c, TF* vi) const {
+        const int q = tile_index<D, SPL>(t0 + a, t1 + b, t2 + c);
+        for (int d = 0; d < D; ++d) vi[d] = v[d * BG<D, SPL>::NT + q];
+    }
+    __device__ __forceinline__ TF flip(int a, int b, int c, int d) const {
+        return f[d * BG<D, SPL>::NT + tile_index<D, SPL>(t0 + a, t1 + b, t2 + c)];
+    }
+};
+template <int D, class TF> struct GridFetch {
+    const GridPtrsT<TF>& g;
+    const uint32_t* n;
+    int b0, b1, b2;
+    __device__ __forceinline__ void get(int a, int b, int c, TF* vi) const {
+        const uint32_t node = node_index<D>(b0 + a, b1 + b, D == 3 ? b2 + c : 0, n);
+        for (int d = 0; d < D; ++d) vi[d] = g.v[d][node];
+    }
+    __device__ __forceinline__ TF flip(int, int, int, int) const { return TF(0); }
+};
+
 // Gather of G2P (g2p.cpp:35-67) and of the velocity gradient (deformation_update.cpp:217-230) over the
-// non-zero nodes of the window, in the reference's (i,j,k) order, from the LDS tile.
-template <int D, int SPL, bool PIC, bool GRAD, class TF>
-__device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const int* tl, const double* xp, const TF* s_v, const TF* s_flip,
-                                       double h1, bool apic, bool useflip, TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
-    using G = BG<D, SPL>;
+// non-zero nodes of the window, in the reference's (i,j,k) order.
+template <int D, int SPL, bool PIC, bool GRAD, class TF, class Fetch>
+__device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* xp, double h1, const Fetch& fetch, bool apic,
+                                       bool useflip, TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
     const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
 #pragma unroll
     for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) {
@@ -1319,9 +1382,8 @@ __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const int* tl,
 #pragma unroll
             for (int ic = 0; ic < c2; ++ic) {
                 const int c = (D == 3) ? ic : 0;
-                const int q = tile_index<D, SPL>(tl[0] + a, tl[1] + b, tl[2] + c);
                 TF vi[3];
-                for (int d = 0; d < D; ++d) vi[d] = s_v[d * G::NT + q];
+                fetch.get(a, b, c, vi);
                 if (PIC) {
                     double weight = st.w[0][a] * st.w[1][b];
                     if (D == 3) weight = weight * st.w[2][c];
@@ -1331,7 +1393,7 @@ __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const int* tl,
                         pic[r] += vi[r] * w;
                         if (apic) for (int k = 0; k < D; ++k) Bp[r][k] += vi[r] * pd[k] * w;
                     }
-                    if (useflip) for (int r = 0; r < D; ++r) fl[r] += s_flip[r * G::NT + q] * w;
+                    if (useflip) for (int r = 0; r < D; ++r) fl[r] += fetch.flip(a, b, c, r) * w;
                 }
                 if (GRAD) {
                     double gr[3];
@@ -1384,6 +1446,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, Part
     __syncthreads();
     uint32_t err = 0, pcount = 0;
     BoundsAcc ba;
+#pragma unroll 1
     for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) {
         const uint32_t i = br.start + k, p = sb.perm[i];
         double xp[3] = {0, 0, 0};
@@ -1392,7 +1455,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, Part
         int tl[3];
         if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, s_c.view(br.bc), h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
         TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gvt[3][3] = {{0}};
-        gather<D, SPL, true, true, TF>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gvt);
+        gather<D, SPL, true, true, TF>(st, xp, h1, TileFetch<D, SPL, TF>{s_v, s_f, tl[0], tl[1], tl[2]}, apic, useflip, pic, fl, Bp, gvt);
         double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0}, gv[3][3];
         for (int d = 0; d < D; ++d) {
             const double vo = useflip ? (double)src.v[d][p] : 0.0;
@@ -1435,6 +1498,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
     __syncthreads();
     uint32_t err = 0;
     BoundsAcc ba;
+#pragma unroll 1
     for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) {
         const uint32_t i = br.start + k, p = sb.perm[i];
         double xp[3] = {0, 0, 0};
@@ -1443,7 +1507,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
         int tl[3];
         if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, s_c.view(br.bc), h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
         TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3];
-        gather<D, SPL, true, false, TF>(st, tl, xp, s_v, s_f, h1, apic, useflip, pic, fl, Bp, gv);
+        gather<D, SPL, true, false, TF>(st, xp, h1, TileFetch<D, SPL, TF>{s_v, s_f, tl[0], tl[1], tl[2]}, apic, useflip, pic, fl, Bp, gv);
         double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) {
             const double vo = useflip ? (double)src.v[d][p] : 0.0;
@@ -1466,35 +1530,6 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
 // laid out for occupancy rather than LDS reuse); the velocity gradient gathers from the grid arrays.
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, class TF>
-__device__ __forceinline__ void gather_grad_global(const Stencil<D, SPL>& st, const GridPtrsT<TF>& g, const uint32_t* n, double h1,
-                                                   TF (&gv)[3][3]) {
-    const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
-#pragma unroll
-    for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) {
-#pragma unroll
-        for (int b = 0; b < Stencil<D, SPL>::cnt; ++b) {
-#pragma unroll
-            for (int ic = 0; ic < c2; ++ic) {
-                const int c = (D == 3) ? ic : 0;
-                const uint32_t node = node_index<D>(st.base[0] + a, st.base[1] + b, (D == 3) ? st.base[2] + c : 0, n);
-                TF vi[3];
-                for (int d = 0; d < D; ++d) vi[d] = g.v[d][node];
-                double gr[3];
-                if (D == 3) {
-                    gr[0] = st.dw[0][a] * st.w[1][b] * st.w[2][c] * h1;
-                    gr[1] = st.dw[1][b] * st.w[0][a] * st.w[2][c] * h1;
-                    gr[2] = st.dw[2][c] * st.w[0][a] * st.w[1][b] * h1;
-                } else {
-                    gr[0] = st.dw[0][a] * st.w[1][b] * h1;
-                    gr[1] = st.dw[1][b] * st.w[0][a] * h1;
-                }
-                for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) gv[r][k] += vi[r] * (TF)gr[k];
-            }
-        }
-    }
-}
-
-template <int D, int SPL, class TF>
 __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
     if (S->stop) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1510,8 +1545,8 @@ __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, Part
         if (!make_stencil<D, SPL>(st, xp, org, nn, GlobalCoords{g.coord, nn}, h1, false)) {
             err |= MPM_ERR_GRID_BOUNDS;
         } else {
-            TF gvt[3][3] = {{0}};
-            gather_grad_global<D, SPL, TF>(st, g, nn, h1, gvt);
+            TF gvt[3][3] = {{0}}, pic[3], fl[3], Bp[3][3];
+            gather<D, SPL, false, true, TF>(st, xp, h1, GridFetch<D, TF>{g, nn, st.base[0], st.base[1], st.base[2]}, false, false, pic, fl, Bp, gvt);
             double gv[3][3];
             for (int r = 0; r < 3; ++r) for (int k = 0; k < 3; ++k) gv[r][k] = (double)gvt[r][k];
             deform_and_project<D, TF>(P, dt, src, p, dst, i, gv, err, pcount);

@@ -111,25 +111,25 @@ MPM_HD PlaneRot sym_jacobi(double x, double y, double z) {  // annihilates y in 
     if (2.0 * ay < 2.2250738585072014e-308) return r;
     double tau = (x - z) / (2.0 * ay);
     double w = sqrt(tau * tau + 1.0);
-    double t = (tau > 0) ? 1.0 / (tau + w) : 1.0 / (tau - w);
+    double t = 1.0 / (tau > 0 ? tau + w : tau - w);
     double n = 1.0 / sqrt(t * t + 1.0);
     double st = t > 0 ? 1.0 : -1.0;
-    r.s = -st * (y / ay) * fabs(t) * n;
+    r.s = -st * copysign(1.0, y) * fabs(t) * n;  // y / |y| is exactly the sign of y
     r.c = n;
     return r;
 }
 
 // One two-sided Jacobi step on the pair (p, q); compile-time indices keep X, U, W in registers.
 template <int D, int p, int q>
-MPM_HD void jacobi_pair(M<D>& X, M<D>& U, M<D>& W, double& dmax, bool& rotated) {
-    const double thr = fmax(2.2250738585072014e-308, 2.0 * 2.220446049250313e-16 * dmax);
+MPM_HD void jacobi_pair(M<D>& X, M<D>& U, M<D>& W, double& dmax, bool& rotated, double tol) {
+    const double thr = fmax(2.2250738585072014e-308, tol * dmax);
     if (!(fabs(X(p, q)) > thr || fabs(X(q, p)) > thr)) return;
     rotated = true;
     // symmetrising rotation r1 acting on rows (p,q)
     const double a = X(p, p), b = X(p, q), c = X(q, p), d = X(q, q);
     const double tsum = a + d, diff = c - b;
     PlaneRot r1{1.0, 0.0};
-    if (fabs(diff) >= 2.2250738585072014e-308) { const double u = tsum / diff; const double h = sqrt(1.0 + u * u); r1.s = 1.0 / h; r1.c = u / h; }
+    if (fabs(diff) >= 2.2250738585072014e-308) { const double u = tsum / diff; const double hi = 1.0 / sqrt(1.0 + u * u); r1.s = hi; r1.c = u * hi; }
     const double s00 = r1.c * a + r1.s * c, s01 = r1.c * b + r1.s * d, s11 = -r1.s * b + r1.c * d;
     const PlaneRot jr = sym_jacobi(s00, s01, s11);
     const PlaneRot jl{r1.c * jr.c + r1.s * jr.s, -r1.c * jr.s + r1.s * jr.c};  // jl = r1 * jr^T
@@ -164,7 +164,10 @@ MPM_HD void sv_order(V<D>& s, M<D>& U, M<D>& W) {
     }
 }
 
-template <int D> MPM_HD void svd(const M<D>& A, M<D>& U, V<D>& s, M<D>& W) {
+// Eigen's JacobiSVD threshold: twice the binary64 epsilon.
+constexpr double kSvdTolEigen = 2.0 * 2.220446049250313e-16;
+
+template <int D> MPM_HD void svd(const M<D>& A, M<D>& U, V<D>& s, M<D>& W, double tol = kSvdTolEigen) {
     double scale = 0.0;
 #pragma unroll
     for (int i = 0; i < D * D; ++i) scale = fmax(scale, fabs(A.m[i]));
@@ -179,10 +182,10 @@ template <int D> MPM_HD void svd(const M<D>& A, M<D>& U, V<D>& s, M<D>& W) {
     for (int i = 0; i < D; ++i) dmax = fmax(dmax, fabs(X(i, i)));
     for (int sweep = 0; sweep < 24; ++sweep) {  // sweeps visit the pairs (1,0), (2,0), (2,1)
         bool rotated = false;
-        jacobi_pair<D, 1, 0>(X, U, W, dmax, rotated);
+        jacobi_pair<D, 1, 0>(X, U, W, dmax, rotated, tol);
         if constexpr (D == 3) {
-            jacobi_pair<D, 2, 0>(X, U, W, dmax, rotated);
-            jacobi_pair<D, 2, 1>(X, U, W, dmax, rotated);
+            jacobi_pair<D, 2, 0>(X, U, W, dmax, rotated, tol);
+            jacobi_pair<D, 2, 1>(X, U, W, dmax, rotated, tol);
         }
         if (!rotated) break;
     }

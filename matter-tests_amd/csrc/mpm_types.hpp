@@ -38,6 +38,7 @@ struct Params {
     double q_prefac, d_prefac, e_mu_prefac, f_mu_prefac, rma_prefac;
     double q_max, q_min, p_min, M, q_cohesion, perzyna_exp, perzyna_visc, beta, p0;
     double rho_s, grain_diameter, I_ref, mu_1, mu_2;
+    double svd_tol;  // Jacobi SVD: rotate while an off-diagonal entry exceeds svd_tol * max |diagonal|
     DevObject objects[kMaxObjects];
 };
 
