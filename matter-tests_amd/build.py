@@ -17,10 +17,12 @@ CSRC = os.path.join(HERE, "csrc")
 ENGINE_SO = os.path.join(HERE, "libmpm_hip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared", "-ffp-contract=off",
+HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
-ENGINE_SOURCES = ["mpm_engine.hip"]
-ENGINE_HEADERS = ["mpm_kernels.hpp", "mpm_math.hpp", "mpm_types.hpp"]
+# The C ABI and host logic, then one translation unit per (dimension, spline degree, transfer type) of the
+# step, compiled in parallel and linked into one library.
+ENGINE_SOURCES = ["mpm_engine.hip"] + [f"mpm_step_{d}d_s{s}_{t}.hip" for d in (3, 2) for s in (2, 3) for t in ("f64", "split")]
+ENGINE_HEADERS = ["mpm_context.hpp", "mpm_step.hpp", "mpm_kernels.hpp", "mpm_math.hpp", "mpm_types.hpp"]
 
 
 def run(cmd, cwd=None):
@@ -29,9 +31,18 @@ def run(cmd, cwd=None):
 
 
 def build_engine():
-    srcs = [os.path.join(CSRC, s) for s in ENGINE_SOURCES]
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = ["-I", os.path.join(REPO, "include")]
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in ENGINE_SOURCES]
+    jobs = max(1, min(len(ENGINE_SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        futs = [ex.submit(run, [HIPCC] + HIP_FLAGS + inc + ["-c", "-o", o, os.path.join(CSRC, s)]) for s, o in zip(ENGINE_SOURCES, objs)]
+        for f in futs:
+            f.result()
     tmp = ENGINE_SO + ".tmp"
-    run([HIPCC] + HIP_FLAGS + ["-I", os.path.join(REPO, "include"), "-o", tmp] + srcs)
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, ENGINE_SO)
     return ENGINE_SO
 

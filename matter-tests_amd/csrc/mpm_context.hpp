@@ -1,0 +1,100 @@
+// mpm_context.hpp — the engine context (one per mpm_create) and the host helpers shared by the
+// translation units of the engine: the typed views of its buffers and per-launch event timing.
+#pragma once
+#include "mpm_kernels.hpp"
+#include <vector>
+#include <utility>
+
+struct mpm_context {
+    mpm_config cfg;
+    mpm::Params P;
+    std::vector<mpm::DevPlate> plates;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0;
+    bool ready = false;
+    bool split = false;         // MPM_PRECISION_SPLIT: transfer quantities in float
+    // particles: two layouts (double buffer) and the counting-sort arrays
+    char* pbuf = nullptr;
+    size_t pbuf_bytes = 0;
+    uint32_t* ibuf = nullptr;   // ids (2 layouts), keys, permutation
+    size_t ibuf_n = 0;
+    mpm::PartStateT<double> ps[2]{};  // plain form
+    mpm::PartStateT<float> pf[2]{};   // split form
+    mpm::SortBufs sb{};
+    uint32_t* bbuf = nullptr;   // block histogram, offsets, cursors
+    uint16_t* cellbuf = nullptr; // per-block dual-cell offsets
+    char* stgbuf = nullptr;     // per-block node tiles (staging of the scatter kernels)
+    uint32_t cap_blocks = 0;
+    mpm::StagingT<double> stg{};
+    mpm::StagingT<float> stgf{};
+    bool flip_formed = false;   // the last enqueued step wrote grid flip
+    // grid
+    char* gbuf = nullptr;
+    double* cbuf = nullptr;
+    uint64_t cap_nodes = 0;
+    uint32_t cap_axis = 0;
+    mpm::GridPtrsT<double> gp{};
+    mpm::GridPtrsT<float> gf{};
+    // step state
+    mpm::StepState* S = nullptr;   // device
+    mpm::StepState* Sh = nullptr;  // pinned host mirror
+    uint32_t est_n[3] = {1, 1, 1};
+    // per-kernel HIP-event timing (mpm_set_profiling)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, size_t>> ev_pending;  // (kernel id, index of start event)
+    double kt_ms[16] = {0};
+    uint64_t kt_n[16] = {0};
+};
+
+namespace mpm {
+
+// The typed views of a context's buffers for transfer type TF.
+template <class TF> PartStateT<TF>* layouts(mpm_context* c);
+template <> inline PartStateT<double>* layouts<double>(mpm_context* c) { return c->ps; }
+template <> inline PartStateT<float>* layouts<float>(mpm_context* c) { return c->pf; }
+template <class TF> GridPtrsT<TF>& grid_of(mpm_context* c);
+template <> inline GridPtrsT<double>& grid_of<double>(mpm_context* c) { return c->gp; }
+template <> inline GridPtrsT<float>& grid_of<float>(mpm_context* c) { return c->gf; }
+template <class TF> StagingT<TF>& staging_of(mpm_context* c);
+template <> inline StagingT<double>& staging_of<double>(mpm_context* c) { return c->stg; }
+template <> inline StagingT<float>& staging_of<float>(mpm_context* c) { return c->stgf; }
+
+inline const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p_usl", "k_g2p_musl",
+                                    "k_musl_scatter", "k_musl_update", "k_defgrad", "k_finish", "k_bin_count", "k_bin_scan",
+                                    "k_bin_scatter"};
+enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P_USL, KS_G2P_MUSL, KS_MUSL_SCATTER, KS_MUSL_UPDATE, KS_DEFGRAD,
+                KS_FINISH, KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_COUNT };
+
+// Records an event pair around one launch when profiling; returns the launch's slot.
+struct LaunchTimer {
+    mpm_context* c; int id; size_t slot = 0; bool on;
+    LaunchTimer(mpm_context* c_, int id_) : c(c_), id(id_), on(c_->profiling) {
+        if (!on) return;
+        slot = c->ev_pending.size() * 2;
+        while (c->ev_pool.size() < slot + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { on = false; return; } c->ev_pool.push_back(e); }
+        (void)hipEventRecord(c->ev_pool[slot], c->stream);
+    }
+    ~LaunchTimer() {
+        if (!on) return;
+        (void)hipEventRecord(c->ev_pool[slot + 1], c->stream);
+        c->ev_pending.push_back({id, slot});
+    }
+};
+
+inline void collect_timings(mpm_context* c) {
+    for (auto& pr : c->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[pr.second], c->ev_pool[pr.second + 1]) == hipSuccess) {
+            c->kt_ms[pr.first] += ms; c->kt_n[pr.first] += 1;
+        }
+    }
+    c->ev_pending.clear();
+}
+
+// One step of a context for a spline degree and transfer type (mpm_step.hpp, instantiated per
+// (dimension, degree, type) in its own translation unit so that the engine compiles in parallel).
+template <int D, int SPL, class TF> void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons);
+
+}  // namespace mpm

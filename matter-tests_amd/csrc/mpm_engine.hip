@@ -4,102 +4,15 @@
 // with headroom for a batch of steps, and a device-resident StepState. mpm_step enqueues batches of
 // steps without host synchronisation; the device stops itself (StepState::stop) at errors, frame ends,
 // the final time or when the next grid would not fit, and the host re-allocates and resumes.
-#include "mpm_kernels.hpp"
+#include "mpm_context.hpp"
 #include <vector>
 #include <cstring>
 #include <cmath>
 #include <algorithm>
 #include <new>
+#include <type_traits>
 
 using namespace mpm;
-
-struct mpm_context {
-    mpm_config cfg;
-    Params P;
-    std::vector<DevPlate> plates;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t n = 0;
-    bool ready = false;
-    bool split = false;         // MPM_PRECISION_SPLIT: transfer quantities in float
-    // particles: two layouts (double buffer) and the counting-sort arrays
-    char* pbuf = nullptr;
-    size_t pbuf_bytes = 0;
-    uint32_t* ibuf = nullptr;   // ids (2 layouts), keys, permutation
-    size_t ibuf_n = 0;
-    PartStateT<double> ps[2]{};  // plain form
-    PartStateT<float> pf[2]{};   // split form
-    SortBufs sb{};
-    uint32_t* bbuf = nullptr;   // block histogram, offsets, cursors
-    uint16_t* cellbuf = nullptr; // per-block dual-cell offsets
-    char* stgbuf = nullptr;     // per-block node tiles (staging of the scatter kernels)
-    uint32_t cap_blocks = 0;
-    StagingT<double> stg{};
-    StagingT<float> stgf{};
-    bool flip_formed = false;   // the last enqueued step wrote grid flip
-    // grid
-    char* gbuf = nullptr;
-    double* cbuf = nullptr;
-    uint64_t cap_nodes = 0;
-    uint32_t cap_axis = 0;
-    GridPtrsT<double> gp{};
-    GridPtrsT<float> gf{};
-    // step state
-    StepState* S = nullptr;   // device
-    StepState* Sh = nullptr;  // pinned host mirror
-    uint32_t est_n[3] = {1, 1, 1};
-    // per-kernel HIP-event timing (mpm_set_profiling)
-    bool profiling = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, size_t>> ev_pending;  // (kernel id, index of start event)
-    double kt_ms[16] = {0};
-    uint64_t kt_n[16] = {0};
-};
-
-namespace {
-// The typed views of a context's buffers for transfer type TF.
-template <class TF> PartStateT<TF>* layouts(mpm_context* c);
-template <> PartStateT<double>* layouts<double>(mpm_context* c) { return c->ps; }
-template <> PartStateT<float>* layouts<float>(mpm_context* c) { return c->pf; }
-template <class TF> GridPtrsT<TF>& grid_of(mpm_context* c);
-template <> GridPtrsT<double>& grid_of<double>(mpm_context* c) { return c->gp; }
-template <> GridPtrsT<float>& grid_of<float>(mpm_context* c) { return c->gf; }
-template <class TF> StagingT<TF>& staging_of(mpm_context* c);
-template <> StagingT<double>& staging_of<double>(mpm_context* c) { return c->stg; }
-template <> StagingT<float>& staging_of<float>(mpm_context* c) { return c->stgf; }
-
-const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p_usl", "k_g2p_musl",
-                                    "k_musl_scatter", "k_musl_update", "k_defgrad", "k_finish", "k_bin_count", "k_bin_scan",
-                                    "k_bin_scatter"};
-enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P_USL, KS_G2P_MUSL, KS_MUSL_SCATTER, KS_MUSL_UPDATE, KS_DEFGRAD,
-                KS_FINISH, KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_COUNT };
-
-// Records an event pair around one launch when profiling; returns the launch's slot.
-struct LaunchTimer {
-    mpm_context* c; int id; size_t slot = 0; bool on;
-    LaunchTimer(mpm_context* c_, int id_) : c(c_), id(id_), on(c_->profiling) {
-        if (!on) return;
-        slot = c->ev_pending.size() * 2;
-        while (c->ev_pool.size() < slot + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { on = false; return; } c->ev_pool.push_back(e); }
-        (void)hipEventRecord(c->ev_pool[slot], c->stream);
-    }
-    ~LaunchTimer() {
-        if (!on) return;
-        (void)hipEventRecord(c->ev_pool[slot + 1], c->stream);
-        c->ev_pending.push_back({id, slot});
-    }
-};
-
-void collect_timings(mpm_context* c) {
-    for (auto& pr : c->ev_pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev_pool[pr.second], c->ev_pool[pr.second + 1]) == hipSuccess) {
-            c->kt_ms[pr.first] += ms; c->kt_n[pr.first] += 1;
-        }
-    }
-    c->ev_pending.clear();
-}
-}  // namespace
 
 namespace {
 
@@ -256,56 +169,6 @@ __global__ void k_control(StepState* S, uint32_t stop_at_frame_end, uint32_t che
     S->stop = 0; S->need_realloc = 0; S->steps_done = 0; S->err = 0; S->warn = 0; S->frame_ended = 0;
     S->stop_at_frame_end = stop_at_frame_end; S->check_conservation = check_cons;
     S->cap_axis = cap_axis; S->cap_nodes = cap_nodes; S->cap_blocks = cap_blocks;
-}
-
-// P2G and the grid update for one choice of separate force (flip needed or conservation checked) and MIBF.
-template <int D, int SPL, bool SEP, bool MIBF, class TF>
-void enqueue_p2g(mpm_context* c, const PartStateT<TF>& src, const PartStateT<TF>& dst, uint32_t gblocks) {
-    constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB;
-    hipStream_t s = c->stream;
-    const GridPtrsT<TF>& g = grid_of<TF>(c);
-    const StagingT<TF>& stg = staging_of<TF>(c);
-    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL, SEP, MIBF, TF><<<c->cap_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg); }
-    { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF, TF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, g, stg); }
-}
-
-template <int D, int SPL, class TF>
-void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
-    hipStream_t s = c->stream;
-    const Params& P = c->P;
-    const PartStateT<TF>& src = layouts<TF>(c)[xsel & 1];
-    const PartStateT<TF>& dst = layouts<TF>(c)[(xsel + 1) & 1];
-    const GridPtrsT<TF>& g = grid_of<TF>(c);
-    const StagingT<TF>& stg = staging_of<TF>(c);
-    const uint32_t n = c->n;
-    const uint32_t pblocks = (n + kBlock - 1) / kBlock;
-    const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 8192);
-    const uint32_t zblocks = (c->cap_blocks + kBlock - 1) / kBlock;
-    { LaunchTimer t(c, KS_SETUP); k_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g); }
-    { LaunchTimer t(c, KS_ZERO); k_zero_grid<<<zblocks, kBlock, 0, s>>>(c->S, c->sb); }
-    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, g, n); }
-    { LaunchTimer t(c, KS_BIN_SCAN); k_bin_scan<<<1, 1024, 0, s>>>(c->S, c->sb); }
-    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<<<pblocks, kBlock, 0, s>>>(c->S, c->sb, n); }
-    const bool sep = P.flip_ratio >= -1 || check_cons;
-    c->flip_formed = sep;
-    if (sep && P.use_mibf) enqueue_p2g<D, SPL, true, true, TF>(c, src, dst, gblocks);
-    else if (sep) enqueue_p2g<D, SPL, true, false, TF>(c, src, dst, gblocks);
-    else if (P.use_mibf) enqueue_p2g<D, SPL, false, true, TF>(c, src, dst, gblocks);
-    else enqueue_p2g<D, SPL, false, false, TF>(c, src, dst, gblocks);
-    if (P.use_musl) {
-        constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
-        {
-            LaunchTimer t(c, KS_G2P_MUSL);
-            k_g2p_musl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
-        }
-        { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->cap_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
-        { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL, TF><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, g, stg); }
-        { LaunchTimer t(c, KS_DEFGRAD); k_defgrad<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g, n); }
-    } else {
-        LaunchTimer t(c, KS_G2P_USL);
-        k_g2p_usl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
-    }
-    { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
 }
 
 template <class TF>

@@ -270,10 +270,19 @@ template <int LC> struct LdsCoords {
     }
 };
 
-// Evaluates the 4-node window of the reference (weights are those of wip/grad_wip, tools.hpp:222-252) and
-// keeps its non-zero part: for the quadratic spline exactly one end node has zero weight (both at
-// f = 0.5), and skipping an exactly-zero term leaves every sum of the reference unchanged. The twelve node
-// coordinates are fetched unconditionally (from a clamped index) so that their loads issue together.
+// Node K of a window: weight and derivative (BSpline::node) at the coordinate xi.
+template <int SPL, int K, class TF>
+__device__ __forceinline__ void window_node(double x, double xi, double h1, TF& w, TF& dw) {
+    double wd, dwd;
+    BSpline<SPL>::template node<K>((x - xi) * h1, wd, dwd);
+    w = (TF)wd; dw = (TF)dwd;
+}
+
+// Evaluates the non-zero part of the reference's 4-node window (weights are those of wip/grad_wip,
+// tools.hpp:222-252): for the quadratic spline exactly one end node has zero weight, the base node when
+// |u| >= 1.5 there, else the last one, and skipping an exactly-zero term leaves every sum of the reference
+// unchanged. The four coordinates per axis are fetched unconditionally (from a clamped index) so that
+// their loads issue together.
 template <int D, int SPL, class CS>
 __device__ __forceinline__ bool make_stencil(Stencil<D, SPL>& st, const double* xp, const double* org, const uint32_t* n,
                                              const CS& cs, double h1, bool clamp) {
@@ -288,31 +297,20 @@ __device__ __forceinline__ bool make_stencil(Stencil<D, SPL>& st, const double* 
     }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        double w4[4], dw4[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const double t = (xp[d] - c4[d][a]) * h1;
-            w4[a] = BSpline<SPL>::N(t);
-            dw4[a] = BSpline<SPL>::dN(t);
-        }
-        const bool shift = SPL == 2 && w4[0] == 0.0;
+        const bool shift = SPL == 2 && !(fabs((xp[d] - c4[d][0]) * h1) < 1.5);
         st.base[d] = b[d] + (shift ? 1 : 0);
 #pragma unroll
-        for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) {
-            if (SPL == 2) {
-                st.w[d][a] = MPM_SEL(shift) ? w4[a + 1] : w4[a];
-                st.dw[d][a] = MPM_SEL(shift) ? dw4[a + 1] : dw4[a];
-                st.xi[d][a] = MPM_SEL(shift) ? c4[d][a + 1] : c4[d][a];
-            } else {
-                st.w[d][a] = w4[a]; st.dw[d][a] = dw4[a]; st.xi[d][a] = c4[d][a];
-            }
-        }
+        for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) st.xi[d][a] = (SPL == 2 && MPM_SEL(shift)) ? c4[d][a + 1] : c4[d][a];
+        window_node<SPL, 0>(xp[d], st.xi[d][0], h1, st.w[d][0], st.dw[d][0]);
+        window_node<SPL, 1>(xp[d], st.xi[d][1], h1, st.w[d][1], st.dw[d][1]);
+        window_node<SPL, 2>(xp[d], st.xi[d][2], h1, st.w[d][2], st.dw[d][2]);
+        if (SPL == 3) window_node<SPL, SPL == 3 ? 3 : 2>(xp[d], st.xi[d][SPL == 3 ? 3 : 2], h1, st.w[d][SPL == 3 ? 3 : 2], st.dw[d][SPL == 3 ? 3 : 2]);
     }
     return ok;
 }
 
-// First node of the non-zero window per axis, as make_stencil chooses it (for the quadratic spline the
-// base node is skipped when its weight is exactly zero); false if the 4-node window leaves the grid.
+// First node of the non-zero window per axis, as make_stencil chooses it; false if the 4-node window
+// leaves the grid.
 template <int D, int SPL, class CS>
 __device__ __forceinline__ bool window_base(int* wb, const double* xp, const double* org, const uint32_t* n, const CS& cs,
                                             double h1, bool clamp) {
@@ -320,10 +318,7 @@ __device__ __forceinline__ bool window_base(int* wb, const double* xp, const dou
     const bool ok = stencil_base<D>(b, xp, org, n, h1, clamp);
     for (int d = 0; d < D; ++d) {
         int s = 0;
-        if (SPL == 2) {
-            const double t = (xp[d] - cs(d, cs.clamp4(d, b[d]))) * h1;
-            s = BSpline<2>::N(t) == 0.0 ? 1 : 0;
-        }
+        if (SPL == 2) s = fabs((xp[d] - cs(d, cs.clamp4(d, b[d]))) * h1) < 1.5 ? 0 : 1;
         wb[d] = b[d] + s;
     }
     return ok;
@@ -474,6 +469,7 @@ __global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrsT<
 // ----------------------------------------------------------------------------------------------
 // k_zero_grid: clears the block histogram (the grid arrays are written whole by k_grid_update).
 // ----------------------------------------------------------------------------------------------
+template <int = 0>  // a template, so that every translation unit may instantiate it
 __global__ __launch_bounds__(kBlock) void k_zero_grid(const StepState* S, SortBufs sb) {
     if (S->stop) return;
     const uint32_t nb = S->nblocks;
@@ -521,6 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState*
 
 // k_bin_scan: exclusive prefix sum of the block histogram; one workgroup of 1024 threads, each owning
 // a contiguous run of blocks.
+template <int = 0>  // a template, so that every translation unit may instantiate it
 __global__ __launch_bounds__(1024) void k_bin_scan(const StepState* S, SortBufs sb) {
     if (S->stop) return;
     __shared__ uint32_t part[1024];
@@ -542,6 +539,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const StepState* S, SortBufs 
 }
 
 // k_bin_scatter: counting-sort permutation (the order inside a block is fixed later by k_p2g).
+template <int = 0>  // a template, so that every translation unit may instantiate it
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, SortBufs sb, uint32_t n) {
     if (S->stop) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -616,7 +614,7 @@ template <int D, int SPL, int NV, class TF> struct ScatterLDS {
     uint32_t cstart[G::NC + 2];      // bucket c = cell c occupies p[cstart[c] .. cstart[c+1]); bucket NC: outside
     union {
         struct { uint32_t tmp[kMaxBP]; uint32_t ccount[G::NC + 1]; uint32_t ccur[G::NC + 1]; uint8_t cell[kMaxBP]; } sort;
-        TF tile[G::NT * NV];         // the block's node sums, [value][node]
+        double tile[G::NT * NV];     // the block's node sums, [value][node] (binary64 in both forms, see add_partials)
     } u;
 };
 
@@ -718,14 +716,14 @@ __device__ __forceinline__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, c
 
 template <int D, int SPL, int NV, class TF>
 __device__ __forceinline__ void zero_tile(ScatterLDS<D, SPL, NV, TF>& L) {
-    for (int q = threadIdx.x; q < BG<D, SPL>::NT * NV; q += blockDim.x) L.u.tile[q] = TF(0);
+    for (int q = threadIdx.x; q < BG<D, SPL>::NT * NV; q += blockDim.x) L.u.tile[q] = 0.0;
 }
 
 template <int D, int SPL, int NV, class TF>
 __device__ __forceinline__ void write_tile(const ScatterLDS<D, SPL, NV, TF>& L, const StagingT<TF>& stg, uint32_t b) {
     using G = BG<D, SPL>;
     for (int q = threadIdx.x; q < G::NT; q += blockDim.x)
-        for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = L.u.tile[v * G::NT + q];
+        for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = (TF)L.u.tile[v * G::NT + q];
 }
 
 // Adds one wavefront's partial sums acc (slices a0 .. a0+SPP-1, second-axis half `half`) of dual cell
@@ -733,7 +731,9 @@ __device__ __forceinline__ void write_tile(const ScatterLDS<D, SPL, NV, TF>& L, 
 // may add to a node that another lane updated earlier, so the adds are LDS atomics: a plain load/add/store
 // would let the compiler move later loads above earlier stores (legal for one lane, a lost update across
 // lanes). A wavefront's LDS instructions execute in program order, so each node still receives its terms
-// in a fixed order.
+// in a fixed order. The tile is binary64 in the split form too: LDS binary32 atomic adds issue about ten
+// times slower than binary64 ones on gfx950 (measured: ~80 ns against ~7 ns per wavefront instruction
+// per CU for this access pattern).
 template <int D, int SPL, int NV, class TF>
 __device__ __forceinline__ void add_partials(ScatterLDS<D, SPL, NV, TF>& L, int cell, int a0, int half,
                                              const TF (&acc)[ScatterShape<D, SPL, NV, TF>::SPP][ScatterShape<D, SPL, NV, TF>::NSH][NV]) {
@@ -754,12 +754,12 @@ __device__ __forceinline__ void add_partials(ScatterLDS<D, SPL, NV, TF>& L, int 
                 for (int oc = 0; oc < G::W; ++oc) {
                     const int q = tile_index<D, SPL>(c[0] + a, c[1] + ob, c[2] + oc);
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], acc[s][obl * G::W + oc][v]);
+                    for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], (double)acc[s][obl * G::W + oc][v]);
                 }
             } else {
                 const int q = tile_index<D, SPL>(c[0] + a, c[1] + ob, 0);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], acc[s][obl][v]);
+                for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], (double)acc[s][obl][v]);
             }
         }
     }
@@ -772,7 +772,7 @@ template <int D, int SPL, int NV, class TF, class Contrib>
 __device__ __forceinline__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, uint32_t b,
                               uint32_t start) {
     using SS = ScatterShape<D, SPL, NV, TF>;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int grp = wave % SS::NG, half = wave / SS::NG, a0 = grp * SS::SPP;
     TF acc[SS::SPP][SS::NSH][NV];
 #pragma unroll
@@ -850,12 +850,12 @@ __device__ __forceinline__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV, 
                             for (int oc = 0; oc < G::W; ++oc) {
                                 const int qn = tile_index<D, SPL>(l[0] + a, l[1] + ob, l[2] + oc);
 #pragma unroll
-                                for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + qn], acc[s][obl * G::W + oc][v]);
+                                for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + qn], (double)acc[s][obl * G::W + oc][v]);
                             }
                         } else {
                             const int qn = tile_index<D, SPL>(l[0] + a, l[1] + ob, 0);
 #pragma unroll
-                            for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + qn], acc[s][obl][v]);
+                            for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + qn], (double)acc[s][obl][v]);
                         }
                     }
                 }
@@ -867,16 +867,6 @@ __device__ __forceinline__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV, 
     return err;
 }
 
-// Weight, derivative and coordinate of window node `off` on axis d, evaluated as make_stencil does
-// (the same expression on the same operands gives the same bits) but for a run-time offset.
-template <int SPL, class CS>
-__device__ __forceinline__ void axis_node(const CS& cs, int d, int base, int off, double x, double h1, double& w, double& dw, double& xi) {
-    xi = cs(d, base + off);
-    const double t = (x - xi) * h1;
-    w = BSpline<SPL>::N(t);
-    dw = BSpline<SPL>::dN(t);
-}
-
 // Multiply-add used by the scatter arithmetic (a fused multiply-add in both forms).
 template <class TF> __device__ __forceinline__ TF madd(TF a, TF b, TF c) { return fma(a, b, c); }
 
@@ -884,6 +874,8 @@ template <class TF> __device__ __forceinline__ TF madd(TF a, TF b, TF c) { retur
 // (make_stencil's base, with the same shift test), and weight, derivative and node-minus-particle offset of
 // the first-axis nodes a0 .. a0+SPP-1, the second-axis nodes of `half` and all third-axis nodes. Every
 // value is the expression make_stencil evaluates on the same operands; they are rounded to TF.
+// a0 and half are uniform across the wavefront, so the piece of a first- or second-axis node is picked
+// by a scalar branch.
 template <int D, int SPL, int SPP, int OBN, class TF>
 struct SliceStencil {
     static constexpr int W = BG<D, SPL>::W;
@@ -894,11 +886,12 @@ struct SliceStencil {
 };
 
 template <int SPL, class CS, class TF>
-__device__ __forceinline__ void node_vals(const CS& cs, int d, int idx, double x, double h1, TF& w, TF& dw, TF& pd) {
-    const double xi = cs(d, idx);
-    const double t = (x - xi) * h1;
-    w = (TF)BSpline<SPL>::N(t);
-    dw = (TF)BSpline<SPL>::dN(t);
+__device__ __forceinline__ void node_vals(const CS& cs, int d, int i0, int k, double x, double h1, TF& w, TF& dw, TF& pd) {
+    const double xi = cs(d, i0 + k);
+    if (k == 0) window_node<SPL, 0>(x, xi, h1, w, dw);
+    else if (k == 1) window_node<SPL, 1>(x, xi, h1, w, dw);
+    else if (SPL == 3 && k == 3) window_node<SPL, SPL == 3 ? 3 : 2>(x, xi, h1, w, dw);
+    else window_node<SPL, 2>(x, xi, h1, w, dw);
     pd = (TF)(xi - x);
 }
 
@@ -923,13 +916,13 @@ __device__ __forceinline__ bool slice_stencil(SliceStencil<D, SPL, SPP, OBN, TF>
 #pragma unroll
     for (int s = 0; s < SPP; ++s) {
         const int a = min(a0 + s, W - 1);
-        node_vals<SPL>(cs, 0, i0[0] + a, xp[0], h1, ss.w0[s], ss.dw0[s], ss.pd0[s]);
+        node_vals<SPL>(cs, 0, i0[0], a, xp[0], h1, ss.w0[s], ss.dw0[s], ss.pd0[s]);
     }
 #pragma unroll
-    for (int o = 0; o < OBN; ++o) node_vals<SPL>(cs, 1, i0[1] + half * OBN + o, xp[1], h1, ss.w1[o], ss.dw1[o], ss.pd1[o]);
+    for (int o = 0; o < OBN; ++o) node_vals<SPL>(cs, 1, i0[1], half * OBN + o, xp[1], h1, ss.w1[o], ss.dw1[o], ss.pd1[o]);
     if (D == 3) {
 #pragma unroll
-        for (int o = 0; o < W; ++o) node_vals<SPL>(cs, 2, i0[2] + o, xp[2], h1, ss.w2[o], ss.dw2[o], ss.pd2[o]);
+        for (int o = 0; o < W; ++o) node_vals<SPL>(cs, 2, i0[2], o, xp[2], h1, ss.w2[o], ss.dw2[o], ss.pd2[o]);
     }
     return ok;
 }
@@ -1265,9 +1258,13 @@ __device__ __forceinline__ double blend_velocity(double fr, double v, double fli
     return fr * (v + flip) + (1 - fr) * pic;
 }
 
+// Material of a deformation kernel, fixed at compile time: a plastic model (mpm_plastic_model) on Hencky
+// elasticity, or kMatNeoHookean (no return map).
+constexpr int kMatNeoHookean = 16;
+
 // F <- F + (dt*grad v) * F, return map, Kirchhoff stress for the next step (deformation_update.cpp:13-46,
 // plasticity.cpp:11-665). Reads particle p of the source layout, writes slot i of the destination layout.
-template <int D, class TF>
+template <int D, class TF, int MAT>
 __device__ __forceinline__ void deform_and_project(const Params& P, double dt, const PartStateT<TF>& src, uint32_t p,
                                                    const PartStateT<TF>& dst, uint32_t i, const double (&gv)[3][3],
                                                    uint32_t& err, uint32_t& pcount) {
@@ -1280,15 +1277,15 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
     ps.eps_dev = src.eps_dev[p]; ps.eps_vol = src.eps_vol[p]; ps.eps_prad = src.eps_prad[p];
     ps.dgamma = 0.0; ps.visc = src.visc[p]; ps.muI = src.muI[p];
     double tau[6];
-    if (P.elastic_model == MPM_ELASTIC_NEOHOOKEAN) {
+    if (MAT == kMatNeoHookean) {
         neohookean_tau<D>(F, P.mu, P.lambda, tau);
     } else {
         M<D> U, W; V<D> s;
         svd<D>(F, U, s, W, P.svd_tol);
         V<D> ls;
-        if (P.plastic_model != MPM_PLASTIC_NONE) {
-            V<D> h; for (int k = 0; k < D; ++k) h[k] = log(fmax(fabs(s[k]), 1e-4));
-            RMResult r = return_map<D>(P, dt, h, ps);
+        if (MAT != MPM_PLASTIC_NONE) {
+            V<D> h; for (int k = 0; k < D; ++k) h[k] = log_d(fmax(fabs(s[k]), 1e-4));
+            RMResult r = return_map<D, MAT>(P, dt, h, ps);
             err |= r.err;
             pcount += r.plastic_incr;
             if (r.recompose) {  // the new F = U exp(h) W^T has singular values exp(h): its Hencky strain is h
@@ -1296,10 +1293,10 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
                 F = udvt<D>(U, e, W);
                 for (int k = 0; k < D; ++k) ls[k] = h[k];
             } else {
-                for (int k = 0; k < D; ++k) ls[k] = log(fabs(s[k]));
+                for (int k = 0; k < D; ++k) ls[k] = log_d(fabs(s[k]));
             }
         } else {
-            for (int k = 0; k < D; ++k) ls[k] = log(fabs(s[k]));
+            for (int k = 0; k < D; ++k) ls[k] = log_d(fabs(s[k]));
         }
         hencky_tau<D>(U, ls, P.mu, P.lambda, tau);
     }
@@ -1429,7 +1426,7 @@ __device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& 
 // ----------------------------------------------------------------------------------------------
 // k_g2p_usl (USL): G2P (g2p.cpp:17-77), deformationUpdate + plasticity, positionUpdate.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL, class TF>
+template <int D, int SPL, class TF, int MAT>
 __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
     using G = BG<D, SPL>;
     __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
@@ -1462,7 +1459,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, Part
             vnew[d] = (double)(TF)blend_velocity(P.flip_ratio, vo, (double)fl[d], (double)pic[d]);
         }
         for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { dst.B[r + c * D][i] = apic ? Bp[r][c] : TF(0); gv[r][c] = (double)gvt[r][c]; }
-        deform_and_project<D, TF>(P, dt, src, p, dst, i, gv, err, pcount);
+        deform_and_project<D, TF, MAT>(P, dt, src, p, dst, i, gv, err, pcount);
         for (int d = 0; d < D; ++d) xnew[d] = xp[d] + dt * (double)pic[d];
         if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
         for (int d = 0; d < D; ++d) { dst.x[d][i] = xnew[d]; dst.v[d][i] = (TF)vnew[d]; }
@@ -1529,7 +1526,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
 // One thread per sorted particle slot (the return map is long and register-heavy, so this kernel is
 // laid out for occupancy rather than LDS reuse); the velocity gradient gathers from the grid arrays.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL, class TF>
+template <int D, int SPL, class TF, int MAT>
 __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
     if (S->stop) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1549,7 +1546,7 @@ __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, Part
             gather<D, SPL, false, true, TF>(st, xp, h1, GridFetch<D, TF>{g, nn, st.base[0], st.base[1], st.base[2]}, false, false, pic, fl, Bp, gvt);
             double gv[3][3];
             for (int r = 0; r < 3; ++r) for (int k = 0; k < 3; ++k) gv[r][k] = (double)gvt[r][k];
-            deform_and_project<D, TF>(P, dt, src, p, dst, i, gv, err, pcount);
+            deform_and_project<D, TF, MAT>(P, dt, src, p, dst, i, gv, err, pcount);
         }
     }
     pcount = wave_sum_u32(pcount);

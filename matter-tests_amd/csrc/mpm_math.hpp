@@ -33,8 +33,49 @@ template <int SPL> struct BSpline;
 #else
 #define MPM_SEL(c) (c)
 #endif
+
+// Natural logarithm for positive normal x (other inputs go to the library log): x = m 2^e with m in
+// [sqrt(1/2), sqrt(2)), f = (m-1)/(m+1), ln m = 2 atanh(f) = 2 sum_k f^(2k+1)/(2k+1) (|f| <= 0.1716, ten
+// terms leave a remainder below 1e-18 relative), ln x = e ln 2 + ln m with ln 2 split in two parts so that
+// e ln 2 carries no rounding error of its own. Accurate to about 1 ulp, in a third of the library's
+// instructions.
+MPM_HD double log_d(double x) {
+    if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308)) return log(x);
+    int e;
+    double m = frexp(x, &e);                       // m in [0.5, 1)
+    const bool lo = m < 0.70710678118654752440;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const double f = (m - 1.0) / (m + 1.0);
+    const double s = f * f;
+    double q = 1.0 / 21.0;
+    q = fma(q, s, 1.0 / 19.0); q = fma(q, s, 1.0 / 17.0); q = fma(q, s, 1.0 / 15.0); q = fma(q, s, 1.0 / 13.0);
+    q = fma(q, s, 1.0 / 11.0); q = fma(q, s, 1.0 / 9.0); q = fma(q, s, 1.0 / 7.0); q = fma(q, s, 1.0 / 5.0);
+    q = fma(q, s, 1.0 / 3.0);
+    const double f2 = f + f;
+    const double lm = fma(f2 * s, q, f2);
+    const double de = (double)e;
+    return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lm));
+}
+
+// 1/sqrt(x): the device library's reciprocal square root (a hardware estimate refined to about 1 ulp).
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline double rsqrt_d(double x) { return rsqrt(x); }
+#else
+inline double rsqrt_d(double x) { return 1.0 / sqrt(x); }
+#endif
+// BSpline<SPL>::node<K>(u, w, dw): weight and derivative of node K of a particle's non-zero window
+// (K = 0 .. SPL), u = (x_p - x_i)/dx. Within the window the node fixes the polynomial piece: u lies in
+// [0.5,1.5], [-0.5,0.5], [-1.5,-0.5] for K = 0,1,2 of the quadratic spline and in [1,2], [0,1], [-1,0],
+// [-2,-1] for K = 0..3 of the cubic one, so the piece is chosen at compile time. The pieces meet with equal
+// value and slope, so a u that rounds across a boundary changes w and dw by a few ulp at most.
 template <> struct BSpline<2> {
     static constexpr double apic_factor = 4.0;  // apicDinverse = 4/dx^2 (simulation.cpp:98)
+    template <int K> MPM_HD static void node(double u, double& w, double& dw) {
+        if (K == 1) { w = 0.75 - u * u; dw = -2 * u; }
+        else if (K == 0) { const double r = 1.5 - u; w = 0.5 * r * r; dw = u - 1.5; }
+        else { const double r = 1.5 + u; w = 0.5 * r * r; dw = u + 1.5; }
+    }
     MPM_HD static double N(double u) {
         const double a = fabs(u);
         const double inner = 0.75 - a * a;
@@ -53,6 +94,14 @@ template <> struct BSpline<2> {
 };
 template <> struct BSpline<3> {
     static constexpr double apic_factor = 3.0;  // apicDinverse = 3/dx^2 (simulation.cpp:95)
+    template <int K> MPM_HD static void node(double u, double& w, double& dw) {
+        constexpr double c23 = 0.6666666666666666666666666666666666666666666666666;
+        constexpr double c16 = 0.1666666666666666666666666666666666666666666;
+        if (K == 0) { const double r = 2.0 - u; w = c16 * r * r * r; dw = -0.5 * r * r; }
+        else if (K == 1) { w = 0.5 * u * u * u - u * u + c23; dw = u * (1.5 * u - 2.0); }
+        else if (K == 2) { const double a = -u; w = 0.5 * a * a * a - a * a + c23; dw = u * (1.5 * a - 2.0); }
+        else { const double r = 2.0 + u; w = c16 * r * r * r; dw = 0.5 * r * r; }
+    }
     MPM_HD static double N(double u) {
         const double a = fabs(u);
         const double inner = 0.5 * a * a * a - a * a + 0.6666666666666666666666666666666666666666666666666;
@@ -128,10 +177,36 @@ MPM_HD void jacobi_pair(M<D>& X, M<D>& U, M<D>& W, double& dmax, bool& rotated, 
     // symmetrising rotation r1 acting on rows (p,q)
     const double a = X(p, p), b = X(p, q), c = X(q, p), d = X(q, q);
     const double tsum = a + d, diff = c - b;
-    PlaneRot r1{1.0, 0.0};
-    if (fabs(diff) >= 2.2250738585072014e-308) { const double u = tsum / diff; const double hi = 1.0 / sqrt(1.0 + u * u); r1.s = hi; r1.c = u * hi; }
-    const double s00 = r1.c * a + r1.s * c, s01 = r1.c * b + r1.s * d, s11 = -r1.s * b + r1.c * d;
-    const PlaneRot jr = sym_jacobi(s00, s01, s11);
+    PlaneRot r1{1.0, 0.0}, jr;
+    if (fmax(fabs(tsum), fabs(diff)) > 1e-140) {
+        // The same rotations in closed form, without divisions: with u = tsum/diff, 1/sqrt(1+u^2) =
+        // |diff|/hypot(tsum,diff) and u/sqrt(1+u^2) = sgn(diff) tsum/hypot; for the symmetric step with
+        // alpha = x - z, beta = 2|y|, r = hypot(alpha, beta): c = (|alpha|+r)/hypot(|alpha|+r, beta) and
+        // |s| = beta/hypot(|alpha|+r, beta). The squares cannot underflow to zero while the larger operand
+        // exceeds 1e-140.
+        if (fabs(diff) >= 2.2250738585072014e-308) {
+            const double ih = rsqrt_d(fma(tsum, tsum, diff * diff));
+            r1.s = fabs(diff) * ih;
+            r1.c = (diff < 0 ? -tsum : tsum) * ih;
+        }
+        const double s00 = r1.c * a + r1.s * c, s01 = r1.c * b + r1.s * d, s11 = -r1.s * b + r1.c * d;
+        const double al = s00 - s11, be = 2.0 * fabs(s01);
+        jr = PlaneRot{1.0, 0.0};
+        if (be >= 2.2250738585072014e-308 && fmax(fabs(al), be) > 1e-140) {
+            const double r2 = fma(al, al, be * be);
+            const double g = fabs(al) + r2 * rsqrt_d(r2);
+            const double ig = rsqrt_d(fma(g, g, be * be));
+            jr.c = g * ig;
+            const double sy = copysign(be * ig, s01);  // sgn(y) |t| n
+            jr.s = al > 0 ? -sy : sy;                  // times -sgn(t), where t > 0 exactly when alpha > 0
+        } else {
+            jr = sym_jacobi(s00, s01, s11);
+        }
+    } else {
+        if (fabs(diff) >= 2.2250738585072014e-308) { const double u = tsum / diff; const double hi = 1.0 / sqrt(1.0 + u * u); r1.s = hi; r1.c = u * hi; }
+        const double s00 = r1.c * a + r1.s * c, s01 = r1.c * b + r1.s * d, s11 = -r1.s * b + r1.c * d;
+        jr = sym_jacobi(s00, s01, s11);
+    }
     const PlaneRot jl{r1.c * jr.c + r1.s * jr.s, -r1.c * jr.s + r1.s * jr.c};  // jl = r1 * jr^T
 #pragma unroll
     for (int k = 0; k < D; ++k) {  // rows p,q of X <- jl applied on the left; U <- U * jl^T
@@ -378,8 +453,9 @@ MPM_HD double perzyna_newton(const Params& P, double dt, double q_trial, double 
 
 // plasticity(), plasticity.cpp:17-621. `h` holds the clamped principal Hencky strains
 // ln(max(|s|,1e-4)) of F_trial on entry and the projected strains on exit when recompose is set.
-// For VMVisc's capped branch the result is F = U W^T, i.e. h = 0.
-template <int D>
+// For VMVisc's capped branch the result is F = U W^T, i.e. h = 0. PM >= 0 fixes the plastic model at
+// compile time (the kernel then carries only that model's code and registers); PM < 0 reads it from P.
+template <int D, int PM = -1>
 MPM_HD RMResult return_map(const Params& P, double dt, V<D>& h, PlasticState& st) {
     RMResult r{false, false, 0, 0u};
     const double dim = D;
@@ -392,7 +468,7 @@ MPM_HD RMResult return_map(const Params& P, double dt, V<D>& h, PlasticState& st
     auto set_iso = [&](double pp) { double e = -pp / (K * dim); for (int i = 0; i < D; ++i) h[i] = e * 1.0; r.recompose = true; };
     auto shift_dev = [&](double amount) { for (int i = 0; i < D; ++i) h[i] -= amount * dev[i]; r.recompose = true; };
 
-    switch (P.plastic_model) {
+    switch (PM >= 0 ? PM : P.plastic_model) {
     case MPM_PLASTIC_VM: {  // :30-49
         double dg = dn - P.q_max / P.e_mu_prefac;
         if (dg > 0) {

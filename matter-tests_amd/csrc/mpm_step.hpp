@@ -1,0 +1,80 @@
+// mpm_step.hpp — enqueues one MPM step (simulation.cpp:219-273) of a context on its stream. Included by
+// the per-(dimension, spline degree, transfer type) translation units mpm_step_*.hip.
+#pragma once
+#include "mpm_context.hpp"
+#include <type_traits>
+
+namespace mpm {
+
+// P2G and the grid update for one choice of separate force (flip needed or conservation checked) and MIBF.
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+void enqueue_p2g(mpm_context* c, const PartStateT<TF>& src, const PartStateT<TF>& dst, uint32_t gblocks) {
+    constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB;
+    hipStream_t s = c->stream;
+    const GridPtrsT<TF>& g = grid_of<TF>(c);
+    const StagingT<TF>& stg = staging_of<TF>(c);
+    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL, SEP, MIBF, TF><<<c->cap_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg); }
+    { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF, TF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, g, stg); }
+}
+
+// Calls f(std::integral_constant<int, MAT>) for the material of the run (see kMatNeoHookean).
+template <class Fn>
+void with_material(const Params& P, Fn&& f) {
+    if (P.elastic_model == MPM_ELASTIC_NEOHOOKEAN) { f(std::integral_constant<int, kMatNeoHookean>()); return; }
+    switch (P.plastic_model) {
+    case MPM_PLASTIC_VM: f(std::integral_constant<int, MPM_PLASTIC_VM>()); break;
+    case MPM_PLASTIC_DP: f(std::integral_constant<int, MPM_PLASTIC_DP>()); break;
+    case MPM_PLASTIC_DPSOFT: f(std::integral_constant<int, MPM_PLASTIC_DPSOFT>()); break;
+    case MPM_PLASTIC_MCC: f(std::integral_constant<int, MPM_PLASTIC_MCC>()); break;
+    case MPM_PLASTIC_VMVISC: f(std::integral_constant<int, MPM_PLASTIC_VMVISC>()); break;
+    case MPM_PLASTIC_DPVISC: f(std::integral_constant<int, MPM_PLASTIC_DPVISC>()); break;
+    case MPM_PLASTIC_MCCVISC: f(std::integral_constant<int, MPM_PLASTIC_MCCVISC>()); break;
+    case MPM_PLASTIC_DPMUI: f(std::integral_constant<int, MPM_PLASTIC_DPMUI>()); break;
+    case MPM_PLASTIC_MCCMUI: f(std::integral_constant<int, MPM_PLASTIC_MCCMUI>()); break;
+    default: f(std::integral_constant<int, MPM_PLASTIC_NONE>()); break;
+    }
+}
+
+template <int D, int SPL, class TF>
+void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
+    hipStream_t s = c->stream;
+    const Params& P = c->P;
+    const PartStateT<TF>& src = layouts<TF>(c)[xsel & 1];
+    const PartStateT<TF>& dst = layouts<TF>(c)[(xsel + 1) & 1];
+    const GridPtrsT<TF>& g = grid_of<TF>(c);
+    const StagingT<TF>& stg = staging_of<TF>(c);
+    const uint32_t n = c->n;
+    const uint32_t pblocks = (n + kBlock - 1) / kBlock;
+    const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 8192);
+    const uint32_t zblocks = (c->cap_blocks + kBlock - 1) / kBlock;
+    { LaunchTimer t(c, KS_SETUP); k_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g); }
+    { LaunchTimer t(c, KS_ZERO); k_zero_grid<><<<zblocks, kBlock, 0, s>>>(c->S, c->sb); }
+    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, g, n); }
+    { LaunchTimer t(c, KS_BIN_SCAN); k_bin_scan<><<<1, 1024, 0, s>>>(c->S, c->sb); }
+    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<><<<pblocks, kBlock, 0, s>>>(c->S, c->sb, n); }
+    const bool sep = P.flip_ratio >= -1 || check_cons;
+    c->flip_formed = sep;
+    if (sep && P.use_mibf) enqueue_p2g<D, SPL, true, true, TF>(c, src, dst, gblocks);
+    else if (sep) enqueue_p2g<D, SPL, true, false, TF>(c, src, dst, gblocks);
+    else if (P.use_mibf) enqueue_p2g<D, SPL, false, true, TF>(c, src, dst, gblocks);
+    else enqueue_p2g<D, SPL, false, false, TF>(c, src, dst, gblocks);
+    if (P.use_musl) {
+        constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
+        {
+            LaunchTimer t(c, KS_G2P_MUSL);
+            k_g2p_musl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
+        }
+        { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->cap_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
+        { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL, TF><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, g, stg); }
+        LaunchTimer t(c, KS_DEFGRAD);
+        with_material(P, [&](auto mat) { k_defgrad<D, SPL, TF, decltype(mat)::value><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g, n); });
+    } else {
+        LaunchTimer t(c, KS_G2P_USL);
+        with_material(P, [&](auto mat) { k_g2p_usl<D, SPL, TF, decltype(mat)::value><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g); });
+    }
+    { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
+}
+
+}  // namespace mpm
+
+#define MPM_INSTANTIATE_STEP(D, SPL, TF) template void mpm::enqueue_step<D, SPL, TF>(mpm_context*, uint32_t, bool);
