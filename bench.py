@@ -205,6 +205,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 256 MiB device copy first (counter runs)")
     ap.add_argument("--precision", choices=["both", "split", "fp64"], default="both",
                     help="forms to run; the first one run (split when both) gives the headline value")
+    ap.add_argument("--extra", default="c3,batched,served",
+                    help="comma-separated extra workloads reported beside the headline: c3 (inclined flow), batched "
+                         "(128 scenes), served (Poisson request stream); empty for none")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-kernel HBM traffic summary of two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
@@ -246,6 +249,13 @@ def main():
             "position_max_abs_m": float(np.max(np.abs(xs - xd))),
             "after_steps": args.warmup + args.steps, "particles": n,
             "note": "same seeded input, both forms on the GPU; the plain form is held to the CPU oracle at 1e-10"}
+    extra = [w for w in args.extra.split(",") if w]
+    if "c3" in extra:
+        result["c3"] = incline_run(args)
+    if "batched" in extra:
+        result["batched"] = batched_run(args)
+    if "served" in extra:
+        result["served"] = served_run(args)
     if not args.no_parity:
         result["error_vs_fp64_oracle"] = {prec: parity_check(prec) for prec in forms}
     if not args.no_cpu_baseline:
@@ -254,6 +264,97 @@ def main():
         cfg64.precision = A.PRECISION["fp64"]
         result["cpu_baseline"] = cpu_baseline(cfg64, plates, objs, x)
     print(json.dumps(result), flush=True)
+
+
+def incline_run(args):
+    """Flow down a 35 deg incline with a bump, 7,954,688 particles (644 x 64 x 193 lattice at 1.5536 mm, about 17 per
+    cell at dx = 0.004 m), cubic B-spline, AFLIP, mu(I)-MCC, MIBF, SlipStick floor and bump, split form
+    (float transfers, binary64 return map): W warm-up steps, then K timed steps."""
+    cfg, plates, objs, x = S.incline_flow(lattice=[644, 64, 193], ppc_axis=0.004 / 0.0015536, precision="split")
+    n = x.shape[0]
+    eng = A.Engine(A.product_library(), cfg, plates, objs)
+    eng.set_particles(x)
+    del x
+    if args.warmup:
+        rc, _ = eng.step(args.warmup)
+        if rc:
+            raise A.MPMError(rc, "c3 warm-up")
+    eng.set_profiling(True)
+    t0 = time.perf_counter()
+    rc, info = eng.step(args.steps)
+    t1 = time.perf_counter()
+    if rc or info.steps_done != args.steps:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], "c3 timed steps")
+    kt = {k: {"avg_ms": ms / max(nl, 1), "launches": nl} for k, (ms, nl) in eng.kernel_times().items()}
+    wall = t1 - t0
+    return {"workload": "3D flow down a 35 deg incline with a rectangular bump, mu(I)-MCC, cubic, AFLIP, MIBF",
+            "particles": n, "precision": "split", "value": n * args.steps / wall, "unit": "particle-updates/s",
+            "ms_per_step": 1e3 * wall / args.steps, "steps": args.steps, "plastic_count_last_step": info.plastic_count,
+            "kernels": kt}
+
+
+def batched_run(args, count=128):
+    """128 independent 2D/3D collapses (scenes.batch_collapses: 5k-500k particles log-uniform, aspect 0.5-2,
+    friction angle 20-40 deg, Drucker-Prager or Perzyna-von Mises, quadratic APIC USL, split form) in one batch:
+    W warm-up steps, then K steps of every scene timed around one synchronised mpm_batch_step; the same scenes
+    advanced one after another with mpm_step over the same K steps give the unbatched rate."""
+    L = A.product_library()
+    scs = S.batch_collapses(count=count)
+    engs = []
+    for name, cfg, plates, objs, x in scs:
+        e = A.Engine(L, cfg, plates, objs)
+        e.set_particles(x)
+        engs.append(e)
+    del scs
+    total = sum(e.n for e in engs)
+    b = A.Batch(L, chunk_steps=16, use_graph=True)
+    for e in engs:
+        b.add(e, args.warmup + args.steps)
+    if args.warmup:
+        rc, _ = b.step(args.warmup)
+        if rc:
+            raise A.MPMError(rc, "batched warm-up")
+    t0 = time.perf_counter()
+    rc, info = b.step(args.steps)
+    t1 = time.perf_counter()
+    if rc or info.particle_steps != total * args.steps:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], "batched timed steps")
+    wall = t1 - t0
+    b.close()
+    t2 = time.perf_counter()
+    for e in engs:
+        rc, _ = e.step(args.steps)
+        if rc:
+            raise A.MPMError(rc, "unbatched steps")
+    t3 = time.perf_counter()
+    dims = [e.dim for e in engs]
+    return {"workload": f"{count} independent 2D/3D collapse scenes, DP / Perzyna-VM, quadratic APIC USL",
+            "scenes": count, "scenes_3d": int(sum(d == 3 for d in dims)), "particles": total, "precision": "split",
+            "value": total * args.steps / wall, "unit": "particle-updates/s", "ms_per_step": 1e3 * wall / args.steps,
+            "steps": args.steps, "launch_classes": info.n_classes, "chunk_steps": 16, "hip_graph": True,
+            "unbatched_value": total * args.steps / (t3 - t2), "batching_speedup": (t3 - t2) / wall}
+
+
+def served_run(args, count=48):
+    """Open-loop Poisson stream (50 requests/s) of 48 requests, each a 3D collapse or an inclined flow with
+    10k-2M particles (log-uniform) asking for 50-500 steps, split form, served by mpm_server (footprint
+    admission, continuous batching in chunks of 8 steps, HIP graphs)."""
+    reqs = S.served_requests(count=count)
+    srv = A.Server(A.product_library(), chunk_steps=8, use_graph=True)
+    for name, cfg, plates, objs, x, steps, t in reqs:
+        srv.submit(cfg, plates, objs, x, steps, t)
+    offered = sum(q[4].shape[0] * q[5] for q in reqs)
+    last_arrival = reqs[-1][6]
+    del reqs
+    rc, st = srv.run(600.0)
+    srv.close()
+    if rc or st.completed != count:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], f"served run ({st.completed}/{count} completed)")
+    d = st.as_dict()
+    d.update(workload=f"Poisson stream of {count} collapse / inclined-flow requests at 50 req/s, 10k-2M particles, "
+                      "50-500 steps each", precision="split", unit="particle-updates/s", value=st.throughput,
+             offered_particle_updates=offered, arrival_window_s=last_arrival)
+    return d
 
 
 def timed_run(prec, args):

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 1
+#define MPM_ABI_VERSION 2
 
 /* Enumerations. Integer values follow the declaration order of the reference enums
  * (src/tools.hpp:53-57), so `static_cast<int32_t>(PlasticModel::DP)` is a valid value. */
@@ -253,6 +253,114 @@ int mpm_set_profiling(mpm_context* ctx, int enable);
 
 /* Copies the accumulated timings (one entry per kernel that ran) into out[0..capacity). */
 int mpm_get_kernel_times(mpm_context* ctx, mpm_kernel_timing* out, uint32_t capacity, uint32_t* count);
+
+/* ---- Batched form: many independent scenes advanced together ------------------------------------------
+ * The reference advances one Simulation at a time; independent instances share nothing
+ * (tests/tests.cpp:313-465 runs three of them one after another). A batch advances many contexts
+ * together: each kernel of the step is launched once for all member scenes that share dimension, spline
+ * degree, precision form, transfer variant and material, and a chunk of steps is replayed as one HIP
+ * graph. Every member keeps its own dt, grid, time, frame and stop conditions, so its results are those
+ * of mpm_step on the same context (bit for bit: no step sum depends on another scene). None of these
+ * calls has a counterpart in the reference. */
+typedef struct mpm_batch mpm_batch;
+
+typedef struct mpm_batch_options {
+    uint32_t chunk_steps;   /* steps enqueued per host synchronisation (default 16)              */
+    int32_t use_graph;      /* capture a chunk as a HIP graph and replay it (default 1)          */
+} mpm_batch_options;
+
+typedef struct mpm_batch_info {
+    uint32_t n_scenes;       /* members of the batch                                          */
+    uint32_t n_active;       /* members with steps left in their budget after the call        */
+    uint32_t n_classes;      /* launch classes (distinct compile-time variants) of the members */
+    uint32_t chunks;         /* chunks of steps launched by the call                          */
+    uint32_t status;         /* batch-level MPM_ERR_* bits (HIP errors, out of memory)        */
+    uint32_t reserved;
+    uint64_t particle_steps; /* sum over members of particles x steps taken by the call       */
+} mpm_batch_info;
+
+/* Creates an empty batch on the current HIP device, with its own stream. */
+int mpm_batch_create(const mpm_batch_options* opt, mpm_batch** out);
+
+/* Destroys the batch; its members stay valid contexts (no longer members). */
+void mpm_batch_destroy(mpm_batch* b);
+
+/* Adds a context whose particles are set (mpm_set_particles) with a budget of nsteps steps. While
+ * it is a member, mpm_step on it returns MPM_ERR_INVALID_ARGUMENT; mpm_get_particles/grid and
+ * mpm_scene_info read its state after any mpm_batch_step. The context must outlive its membership. */
+int mpm_batch_add(mpm_batch* b, mpm_context* scene, uint32_t nsteps);
+
+/* Removes a member (e.g. a finished request); its state is kept. */
+int mpm_batch_remove(mpm_batch* b, mpm_context* scene);
+
+/* Sets the remaining step budget of a member. */
+int mpm_batch_set_budget(mpm_batch* b, mpm_context* scene, uint32_t nsteps);
+
+/* Advances every member by up to min(max_steps, its budget) steps; a member stops early on the
+ * conditions of mpm_step (final time, an exit-class error, with MPM_STEP_STOP_AT_FRAME_END a frame
+ * end), and a member that finished or failed gets budget 0. Returns the batch-level error bits;
+ * per-member results come from mpm_scene_info. */
+int mpm_batch_step(mpm_batch* b, uint32_t max_steps, uint32_t flags, mpm_batch_info* info);
+
+/* Result of the member's part of the last mpm_batch_step (as mpm_step's info) and its remaining budget. */
+int mpm_scene_info(const mpm_context* scene, mpm_step_info* info, uint32_t* budget_left);
+
+/* Device bytes held by a context (particles, sort arrays, grid, staging): the footprint that an
+ * admission policy charges for it. */
+uint64_t mpm_device_bytes(const mpm_context* ctx);
+
+/* ---- Served form: a request loop with continuous batching ---------------------------------------------
+ * Requests (a scene and a number of steps) arrive over time. The server admits a request when its
+ * device footprint fits the HBM budget beside the resident scenes, prepares it (context, upload of the
+ * particles) on a host thread while the resident scenes keep stepping on the device, adds it to the
+ * running batch at the next chunk boundary, and retires it when its steps are done. No counterpart in
+ * the reference (one Simulation per process run). */
+typedef struct mpm_server mpm_server;
+
+typedef struct mpm_server_options {
+    uint64_t hbm_budget_bytes; /* admission limit on the summed footprint of resident scenes (0: 80% of free memory) */
+    uint32_t max_scenes;       /* resident scenes at most (0: 256)                                           */
+    uint32_t chunk_steps;      /* steps between admission points (0: 8)                                       */
+    int32_t use_graph;         /* replay chunks as HIP graphs                                                 */
+    int32_t keep_results;      /* keep each request's final particle state for mpm_server_result              */
+} mpm_server_options;
+
+typedef struct mpm_request_desc {
+    const mpm_config* cfg;
+    const mpm_plate* plates;
+    uint32_t n_plates;
+    uint32_t n_objects;
+    const mpm_object* objects;
+    const mpm_particle_arrays* particles; /* copied by mpm_server_submit */
+    uint32_t n;                           /* particles */
+    uint32_t nsteps;                      /* steps the request asks for */
+    double arrival_s;                     /* arrival, seconds after the start of mpm_server_run */
+} mpm_request_desc;
+
+typedef struct mpm_server_stats {
+    uint32_t submitted, completed, failed, peak_resident;
+    uint64_t peak_resident_bytes;
+    uint64_t particle_steps;  /* particle-updates done by the run                                */
+    double wall_s;            /* duration of the run                                             */
+    double throughput;        /* particle_steps / wall_s                                         */
+    double step_ms_p50, step_ms_p99;        /* per step of a resident request: chunk time / chunk steps */
+    double request_ms_p50, request_ms_p99;  /* completion - arrival                              */
+    double queue_ms_p50, queue_ms_p99;      /* admission (resident on the device) - arrival      */
+} mpm_server_stats;
+
+int mpm_server_create(const mpm_server_options* opt, mpm_server** out);
+void mpm_server_destroy(mpm_server* s);
+
+/* Queues a request (its arrays are copied); *id numbers requests from 0 in submission order. */
+int mpm_server_submit(mpm_server* s, const mpm_request_desc* req, uint64_t* id);
+
+/* Runs the request loop until every submitted request has completed or failed, or time_limit_s has
+ * passed; requests arrive at their arrival_s after the start of the call. */
+int mpm_server_run(mpm_server* s, double time_limit_s, mpm_server_stats* stats);
+
+/* Final state of a completed request (keep_results), its last step info and its latency
+ * (completion - arrival, ms). Any output may be NULL. */
+int mpm_server_result(mpm_server* s, uint64_t id, mpm_particle_arrays* out, mpm_step_info* info, double* latency_ms);
 
 /* Human-readable name of the lowest set status bit. */
 const char* mpm_status_string(uint32_t status);

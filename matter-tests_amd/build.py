@@ -21,8 +21,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-ffp-contra
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 # The C ABI and host logic, then one translation unit per (dimension, spline degree, transfer type) of the
 # step, compiled in parallel and linked into one library.
-ENGINE_SOURCES = ["mpm_engine.hip"] + [f"mpm_step_{d}d_s{s}_{t}.hip" for d in (3, 2) for s in (2, 3) for t in ("f64", "split")]
-ENGINE_HEADERS = ["mpm_context.hpp", "mpm_step.hpp", "mpm_kernels.hpp", "mpm_math.hpp", "mpm_types.hpp"]
+ENGINE_SOURCES = ["mpm_engine.hip", "mpm_server.hip"] + [f"mpm_step_{d}d_s{s}_{t}.hip" for d in (3, 2) for s in (2, 3) for t in ("f64", "split")]
+ENGINE_HEADERS = ["mpm_context.hpp", "mpm_step.hpp", "mpm_batch_kernels.hpp", "mpm_kernels.hpp", "mpm_math.hpp", "mpm_types.hpp"]
 
 
 def run(cmd, cwd=None):

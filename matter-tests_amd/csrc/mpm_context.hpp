@@ -1,9 +1,10 @@
 // mpm_context.hpp — the engine context (one per mpm_create) and the host helpers shared by the
 // translation units of the engine: the typed views of its buffers and per-launch event timing.
 #pragma once
-#include "mpm_kernels.hpp"
+#include "mpm_batch_kernels.hpp"
 #include <vector>
 #include <utility>
+#include <mutex>
 
 struct mpm_context {
     mpm_config cfg;
@@ -46,6 +47,11 @@ struct mpm_context {
     std::vector<std::pair<int, size_t>> ev_pending;  // (kernel id, index of start event)
     double kt_ms[16] = {0};
     uint64_t kt_n[16] = {0};
+    // membership of a batch (mpm_batch_add): device copy of P, remaining step budget, result of the last call
+    mpm_batch* batch = nullptr;
+    mpm::Params* dP = nullptr;
+    uint32_t budget = 0;
+    mpm_step_info last{};
 };
 
 namespace mpm {
@@ -96,5 +102,32 @@ inline void collect_timings(mpm_context* c) {
 // One step of a context for a spline degree and transfer type (mpm_step.hpp, instantiated per
 // (dimension, degree, type) in its own translation unit so that the engine compiles in parallel).
 template <int D, int SPL, class TF> void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons);
+
+// Material index of a run: the plastic model on Hencky elasticity, or kMatNeoHookean.
+inline int material_of(const Params& P) { return P.elastic_model == MPM_ELASTIC_NEOHOOKEAN ? kMatNeoHookean : P.plastic_model; }
+
+// Compile-time switches shared by the scenes of one batched launch.
+struct BatchClassKey {
+    int D, SPL, split, sep, mibf, musl, mat;
+    bool operator==(const BatchClassKey& o) const {
+        return D == o.D && SPL == o.SPL && split == o.split && sep == o.sep && mibf == o.mibf && musl == o.musl && mat == o.mat;
+    }
+};
+
+// One class of a batch: its scenes, the device table of the launches and their workgroup counts.
+struct BatchClass {
+    BatchClassKey key{};
+    std::vector<mpm_context*> members;
+    BatchTable T{};
+    uint32_t wg[BR_KINDS] = {0, 0, 0, 0};
+};
+
+// One step of every scene of a class (one launch per kernel), instantiated like enqueue_step.
+template <int D, int SPL, class TF> void enqueue_batch_step(const BatchClass& bc, hipStream_t s);
+
+// A mutex that another host thread holds while it makes HIP calls (allocations, synchronous copies) that
+// must not run during a stream capture of this batch. A chunk that needs a new graph while the mutex is
+// taken is launched without a graph instead.
+void batch_set_capture_lock(mpm_batch* b, std::mutex* m);
 
 }  // namespace mpm

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <new>
 #include <type_traits>
+#include <cstddef>
 
 using namespace mpm;
 
@@ -167,7 +168,7 @@ __global__ void k_control(StepState* S, uint32_t stop_at_frame_end, uint32_t che
                           uint32_t cap_blocks) {
     if (threadIdx.x != 0) return;
     S->stop = 0; S->need_realloc = 0; S->steps_done = 0; S->err = 0; S->warn = 0; S->frame_ended = 0;
-    S->stop_at_frame_end = stop_at_frame_end; S->check_conservation = check_cons;
+    S->stop_at_frame_end = stop_at_frame_end; S->check_conservation = check_cons; S->step_limit = 0xFFFFFFFFu;
     S->cap_axis = cap_axis; S->cap_nodes = cap_nodes; S->cap_blocks = cap_blocks;
 }
 
@@ -294,7 +295,193 @@ void launch_binning(mpm_context* c, uint32_t n, int32_t* dbase, uint32_t* dkey) 
     else k_binning<2, 3, TF><<<pblocks, kBlock, 0, c->stream>>>(c->P, c->S, L, g, n, dbase, dkey);
 }
 
+
+// ---------------------------------------------------------------------------------------------------------
+// Batches (mpm_batch_*): member contexts grouped into launch classes, one device table per class.
+// ---------------------------------------------------------------------------------------------------------
+
+constexpr uint32_t kHeadBytes = (uint32_t)offsetof(StepState, red);  // loop state read back per chunk
+static_assert(kHeadBytes % 4 == 0, "StepState head must be whole words");
+
+BatchClassKey class_key(const mpm_context* c, bool check_cons) {
+    BatchClassKey k{};
+    k.D = c->cfg.dim; k.SPL = c->cfg.spline_degree; k.split = c->split ? 1 : 0;
+    k.sep = (c->P.flip_ratio >= -1 || check_cons) ? 1 : 0; k.mibf = c->P.use_mibf ? 1 : 0; k.musl = c->P.use_musl ? 1 : 0;
+    k.mat = material_of(c->P);
+    return k;
+}
+
+uint32_t grid_workgroups(const mpm_context* c) {
+    return (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 8192);
+}
+
+template <class TF>
+SceneRef<TF> scene_ref(mpm_context* c) {
+    SceneRef<TF> r{};
+    r.P = c->dP; r.S = c->S;
+    r.ps[0] = layouts<TF>(c)[0]; r.ps[1] = layouts<TF>(c)[1];
+    r.sb = c->sb; r.g = grid_of<TF>(c); r.stg = staging_of<TF>(c);
+    r.n = c->n; r.cap_axis = c->cap_axis; r.cap_nodes = (uint32_t)c->cap_nodes; r.cap_blocks = c->cap_blocks;
+    return r;
+}
+
+template <class TF>
+void enqueue_batch_class(const BatchClass& bc, hipStream_t s) {
+    const BatchClassKey& k = bc.key;
+    if (k.D == 3 && k.SPL == 2) enqueue_batch_step<3, 2, TF>(bc, s);
+    else if (k.D == 3) enqueue_batch_step<3, 3, TF>(bc, s);
+    else if (k.SPL == 2) enqueue_batch_step<2, 2, TF>(bc, s);
+    else enqueue_batch_step<2, 3, TF>(bc, s);
+}
+
 }  // namespace
+
+struct mpm_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    mpm_batch_options opt{};
+    std::vector<mpm_context*> scenes;     // members, in insertion order
+    std::vector<BatchClass> classes;
+    std::vector<mpm_context*> order;      // members in class order (rows of the limit and state arrays)
+    bool dirty = true;                    // classes or device tables must be rebuilt
+    bool check_cons = false;              // variant the classes were built for
+    char* dtab = nullptr; size_t dtab_bytes = 0;     // refs, range starts, limits, StepState pointers, heads
+    uint32_t* hlimit = nullptr; uint32_t* hhead = nullptr; size_t hcap = 0;  // pinned
+    uint32_t* dlimit = nullptr; uint32_t* dhead = nullptr; StepState** dSp = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint32_t g_chunk = 0, g_flags = 0;
+    std::mutex* capture_lock = nullptr;   // see batch_set_capture_lock
+};
+
+namespace {
+
+void drop_graph(mpm_batch* b) {
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+}
+
+// Groups the members into classes and uploads every class's scene table and range starts.
+int rebuild_tables(mpm_batch* b, bool check_cons) {
+    drop_graph(b);
+    b->classes.clear();
+    for (mpm_context* c : b->scenes) {
+        const BatchClassKey k = class_key(c, check_cons);
+        auto it = std::find_if(b->classes.begin(), b->classes.end(), [&](const BatchClass& x) { return x.key == k; });
+        if (it == b->classes.end()) { b->classes.push_back(BatchClass{}); b->classes.back().key = k; it = b->classes.end() - 1; }
+        it->members.push_back(c);
+    }
+    b->order.clear();
+    for (auto& bc : b->classes) for (mpm_context* c : bc.members) b->order.push_back(c);
+    const size_t ns = b->order.size();
+    // layout of the device table: per class [refs][starts]; then limits[ns], S pointers[ns], heads[ns]
+    std::vector<size_t> ref_off, start_off;
+    size_t off = 0;
+    auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    for (auto& bc : b->classes) {
+        const size_t rs = bc.key.split ? sizeof(SceneRef<float>) : sizeof(SceneRef<double>);
+        ref_off.push_back(off); off = align(off + rs * bc.members.size());
+        start_off.push_back(off); off = align(off + sizeof(uint32_t) * BR_KINDS * (bc.members.size() + 1));
+    }
+    const size_t lim_off = off; off = align(off + sizeof(uint32_t) * std::max<size_t>(ns, 1));
+    const size_t sp_off = off; off = align(off + sizeof(StepState*) * std::max<size_t>(ns, 1));
+    const size_t head_off = off; off = align(off + (size_t)kHeadBytes * std::max<size_t>(ns, 1));
+    if (off > b->dtab_bytes) {
+        if (b->dtab) { (void)hipFree(b->dtab); b->dtab = nullptr; b->dtab_bytes = 0; }
+        if (hipMalloc(&b->dtab, off) != hipSuccess) { b->dtab = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        b->dtab_bytes = off;
+    }
+    if (ns > b->hcap) {
+        if (b->hlimit) { (void)hipHostFree(b->hlimit); b->hlimit = nullptr; }
+        if (b->hhead) { (void)hipHostFree(b->hhead); b->hhead = nullptr; }
+        b->hcap = 0;
+        if (hipHostMalloc(&b->hlimit, sizeof(uint32_t) * ns, hipHostMallocDefault) != hipSuccess) { b->hlimit = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        if (hipHostMalloc(&b->hhead, (size_t)kHeadBytes * ns, hipHostMallocDefault) != hipSuccess) { b->hhead = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        b->hcap = ns;
+    }
+    std::vector<char> host(off, 0);
+    b->dlimit = (uint32_t*)(b->dtab + lim_off);
+    b->dSp = (StepState**)(b->dtab + sp_off);
+    b->dhead = (uint32_t*)(b->dtab + head_off);
+    for (size_t k = 0; k < b->classes.size(); ++k) {
+        BatchClass& bc = b->classes[k];
+        const uint32_t m = (uint32_t)bc.members.size();
+        uint32_t* st = (uint32_t*)(host.data() + start_off[k]);
+        for (int r = 0; r < BR_KINDS; ++r) st[r * (m + 1)] = 0;
+        for (uint32_t i = 0; i < m; ++i) {
+            mpm_context* c = bc.members[i];
+            if (bc.key.split) { SceneRef<float> r = scene_ref<float>(c); std::memcpy(host.data() + ref_off[k] + i * sizeof(r), &r, sizeof(r)); }
+            else { SceneRef<double> r = scene_ref<double>(c); std::memcpy(host.data() + ref_off[k] + i * sizeof(r), &r, sizeof(r)); }
+            const uint32_t wg[BR_KINDS] = {(c->n + kBlock - 1) / kBlock, c->cap_blocks, grid_workgroups(c), (c->cap_blocks + kBlock - 1) / kBlock};
+            for (int r = 0; r < BR_KINDS; ++r) st[r * (m + 1) + i + 1] = st[r * (m + 1) + i] + wg[r];
+        }
+        for (int r = 0; r < BR_KINDS; ++r) bc.wg[r] = st[r * (m + 1) + m];
+        bc.T.refs = b->dtab + ref_off[k];
+        bc.T.start = (const uint32_t*)(b->dtab + start_off[k]);
+        bc.T.ns = m;
+        size_t first = 0;
+        for (size_t j = 0; j < k; ++j) first += b->classes[j].members.size();
+        bc.T.limit = b->dlimit + first;
+    }
+    StepState** sp = (StepState**)(host.data() + sp_off);
+    for (size_t i = 0; i < ns; ++i) sp[i] = b->order[i]->S;
+    for (mpm_context* c : b->scenes) {  // the scene's constants, as they are now
+        HIP_TRY(hipMemcpy(c->dP, &c->P, sizeof(Params), hipMemcpyHostToDevice));
+        c->flip_formed = class_key(c, check_cons).sep != 0;
+    }
+    HIP_TRY(hipMemcpy(b->dtab, host.data(), off, hipMemcpyHostToDevice));
+    b->check_cons = check_cons;
+    b->dirty = false;
+    return 0;
+}
+
+// One chunk: control, `chunk` steps of every class, then the loop-state heads into one buffer.
+void enqueue_chunk(mpm_batch* b, uint32_t chunk, uint32_t flags) {
+    const uint32_t sfe = (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u, cons = (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u;
+    for (const BatchClass& bc : b->classes) {
+        if (bc.key.split) kb_control<float><<<bc.T.ns, 64, 0, b->stream>>>(bc.T, sfe, cons);
+        else kb_control<double><<<bc.T.ns, 64, 0, b->stream>>>(bc.T, sfe, cons);
+    }
+    for (uint32_t j = 0; j < chunk; ++j)
+        for (const BatchClass& bc : b->classes) {
+            if (bc.key.split) enqueue_batch_class<float>(bc, b->stream);
+            else enqueue_batch_class<double>(bc, b->stream);
+        }
+    kb_gather_state<><<<(uint32_t)b->order.size(), kBlock, 0, b->stream>>>(b->dSp, kHeadBytes / 4, b->dhead);
+}
+
+int launch_chunk(mpm_batch* b, uint32_t chunk, uint32_t flags) {
+    if (!b->opt.use_graph) {
+        enqueue_chunk(b, chunk, flags);
+        return hipGetLastError() == hipSuccess ? 0 : MPM_ERR_HIP;
+    }
+    if (!b->gexec || b->g_chunk != chunk || b->g_flags != flags) {
+        drop_graph(b);
+        std::unique_lock<std::mutex> lk;
+        if (b->capture_lock) {
+            lk = std::unique_lock<std::mutex>(*b->capture_lock, std::try_to_lock);
+            if (!lk.owns_lock()) {  // another thread is making HIP calls: launch this chunk directly
+                enqueue_chunk(b, chunk, flags);
+                return hipGetLastError() == hipSuccess ? 0 : MPM_ERR_HIP;
+            }
+        }
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_chunk(b, chunk, flags);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        if (e != hipSuccess || !graph) { if (graph) (void)hipGraphDestroy(graph); return MPM_ERR_HIP; }
+        const hipError_t ei = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) { b->gexec = nullptr; return MPM_ERR_HIP; }
+        b->g_chunk = chunk; b->g_flags = flags;
+    }
+    HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+    return 0;
+}
+
+}  // namespace
+
+namespace mpm {
+void batch_set_capture_lock(mpm_batch* b, std::mutex* m) { if (b) b->capture_lock = m; }
+}  // namespace mpm
 
 extern "C" {
 
@@ -353,15 +540,17 @@ void mpm_destroy(mpm_context* c) {
     if (c->stgbuf) (void)hipFree(c->stgbuf);
     if (c->gbuf) (void)hipFree(c->gbuf);
     if (c->cbuf) (void)hipFree(c->cbuf);
+    if (c->batch) (void)mpm_batch_remove(c->batch, c);
     if (c->S) (void)hipFree(c->S);
     if (c->Sh) (void)hipHostFree(c->Sh);
+    if (c->dP) (void)hipFree(c->dP);
     for (hipEvent_t ev : c->ev_pool) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
 int mpm_add_plate(mpm_context* c, const mpm_plate* p) {
-    if (!c || !p) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c || !p || c->batch) return MPM_ERR_INVALID_ARGUMENT;
     if ((int)c->plates.size() >= kMaxPlates) return MPM_ERR_UNSUPPORTED;
     if (!valid_enum(p->plate_type, 0, 5) || !valid_enum(p->bc, 0, 2)) return MPM_ERR_INVALID_ARGUMENT;
     DevPlate d{};
@@ -381,7 +570,7 @@ int mpm_add_plate(mpm_context* c, const mpm_plate* p) {
 }
 
 int mpm_add_object(mpm_context* c, const mpm_object* o) {
-    if (!c || !o) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c || !o || c->batch) return MPM_ERR_INVALID_ARGUMENT;
     if (c->P.n_objects >= kMaxObjects) return MPM_ERR_UNSUPPORTED;
     if (!valid_enum(o->kind, MPM_OBJECT_BOX_ROTATED, MPM_OBJECT_CURVE) || !valid_enum(o->bc, 0, 2)) return MPM_ERR_INVALID_ARGUMENT;
     if (o->kind == MPM_OBJECT_SILO && c->cfg.dim != 3) return MPM_ERR_UNSUPPORTED;
@@ -398,6 +587,7 @@ int mpm_add_object(mpm_context* c, const mpm_object* o) {
 
 int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in) {
     if (!c || !in || !in->x || n == 0) return MPM_ERR_INVALID_ARGUMENT;
+    if (c->batch) return MPM_ERR_INVALID_ARGUMENT;
     const int D = c->cfg.dim;
     if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -444,6 +634,7 @@ int mpm_step(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* inf
     if (!c) return MPM_ERR_INVALID_ARGUMENT;
     if (info) std::memset(info, 0, sizeof(*info));
     if (!c->ready) return MPM_ERR_NOT_READY;
+    if (c->batch) return MPM_ERR_INVALID_ARGUMENT;  // a batch member advances with its batch
     if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
     uint32_t remaining = nsteps, done = 0, status = 0, reallocs = 0;
     const int D = c->cfg.dim;
@@ -588,5 +779,167 @@ const char* mpm_status_string(uint32_t s) {
 }
 
 int mpm_abi_version(void) { return MPM_ABI_VERSION; }
+
+int mpm_batch_create(const mpm_batch_options* opt, mpm_batch** out) {
+    if (!out) return MPM_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPM_ERR_HIP;
+    mpm_batch* b = new (std::nothrow) mpm_batch();
+    if (!b) return MPM_ERR_OUT_OF_MEMORY;
+    b->opt.chunk_steps = 16; b->opt.use_graph = 1;
+    if (opt) {
+        if (opt->chunk_steps) b->opt.chunk_steps = std::min<uint32_t>(opt->chunk_steps, 256);
+        b->opt.use_graph = opt->use_graph;
+    }
+    (void)hipGetDevice(&b->device);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return MPM_ERR_HIP; }
+    *out = b;
+    return 0;
+}
+
+void mpm_batch_destroy(mpm_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (mpm_context* c : b->scenes) c->batch = nullptr;
+    drop_graph(b);
+    if (b->dtab) (void)hipFree(b->dtab);
+    if (b->hlimit) (void)hipHostFree(b->hlimit);
+    if (b->hhead) (void)hipHostFree(b->hhead);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int mpm_batch_add(mpm_batch* b, mpm_context* c, uint32_t nsteps) {
+    if (!b || !c || c->batch) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c->ready) return MPM_ERR_NOT_READY;
+    if (c->device != b->device) return MPM_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(b->device) != hipSuccess) return MPM_ERR_HIP;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (!c->dP && hipMalloc(&c->dP, sizeof(Params)) != hipSuccess) { c->dP = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+    c->batch = b; c->budget = nsteps;
+    std::memset(&c->last, 0, sizeof(c->last));
+    b->scenes.push_back(c);
+    b->dirty = true;
+    return 0;
+}
+
+int mpm_batch_remove(mpm_batch* b, mpm_context* c) {
+    if (!b || !c || c->batch != b) return MPM_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(b->device);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->scenes.erase(std::remove(b->scenes.begin(), b->scenes.end(), c), b->scenes.end());
+    c->batch = nullptr;
+    b->dirty = true;
+    return 0;
+}
+
+int mpm_batch_set_budget(mpm_batch* b, mpm_context* c, uint32_t nsteps) {
+    if (!b || !c || c->batch != b) return MPM_ERR_INVALID_ARGUMENT;
+    c->budget = nsteps;
+    return 0;
+}
+
+int mpm_batch_step(mpm_batch* b, uint32_t max_steps, uint32_t flags, mpm_batch_info* info) {
+    if (!b) return MPM_ERR_INVALID_ARGUMENT;
+    if (info) std::memset(info, 0, sizeof(*info));
+    if (hipSetDevice(b->device) != hipSuccess) return MPM_ERR_HIP;
+    const bool cons = (flags & MPM_STEP_CHECK_CONSERVATION) != 0;
+    uint32_t status = 0, chunks = 0, reallocs = 0;
+    uint64_t particle_steps = 0;
+    // steps each member is to take in this call
+    std::vector<uint32_t> want(b->scenes.size()), done(b->scenes.size(), 0);
+    for (size_t i = 0; i < b->scenes.size(); ++i) {
+        mpm_context* c = b->scenes[i];
+        want[i] = std::min(max_steps, c->budget);
+        std::memset(&c->last, 0, sizeof(c->last));
+    }
+    auto index_of = [&](const mpm_context* c) { return (size_t)(std::find(b->scenes.begin(), b->scenes.end(), c) - b->scenes.begin()); };
+    while (!b->scenes.empty()) {
+        uint32_t chunk = 0;
+        for (size_t i = 0; i < b->scenes.size(); ++i) chunk = std::max(chunk, want[i] - done[i]);
+        chunk = std::min(chunk, b->opt.chunk_steps);
+        if (chunk == 0) break;
+        // grid capacity for the chunk; a reallocation changes the member's buffers and so the tables
+        for (size_t i = 0; i < b->scenes.size(); ++i) {
+            mpm_context* c = b->scenes[i];
+            const uint32_t lim = std::min(chunk, want[i] - done[i]);
+            if (!lim && c->cap_blocks && c->cap_nodes) continue;  // idle member whose buffers exist
+            const void* before[3] = {c->gbuf, c->stgbuf, c->bbuf};
+            const int rc = ensure_grid(c, std::max(lim, 1u));
+            if (rc) { status |= rc; break; }
+            if (before[0] != c->gbuf || before[1] != c->stgbuf || before[2] != c->bbuf) b->dirty = true;
+        }
+        if (status) break;
+        if (b->dirty || b->check_cons != cons) {
+            const int rc = rebuild_tables(b, cons);
+            if (rc) { status |= rc; break; }
+        }
+        for (size_t r = 0; r < b->order.size(); ++r) {
+            const size_t i = index_of(b->order[r]);
+            b->hlimit[r] = std::min(chunk, want[i] - done[i]);
+        }
+        const size_t ns = b->order.size();
+        if (hipMemcpyAsync(b->dlimit, b->hlimit, sizeof(uint32_t) * ns, hipMemcpyHostToDevice, b->stream) != hipSuccess) { status |= MPM_ERR_HIP; break; }
+        int rc = launch_chunk(b, chunk, flags);
+        if (rc) { status |= rc; break; }
+        if (hipMemcpyAsync(b->hhead, b->dhead, (size_t)kHeadBytes * ns, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+            hipStreamSynchronize(b->stream) != hipSuccess) { status |= MPM_ERR_HIP; break; }
+        ++chunks;
+        bool realloc = false;
+        for (size_t r = 0; r < ns; ++r) {
+            mpm_context* c = b->order[r];
+            const size_t i = index_of(c);
+            std::memcpy((void*)c->Sh, (const char*)b->hhead + (size_t)kHeadBytes * r, kHeadBytes);
+            if (b->hlimit[r] == 0) continue;  // idle in this chunk
+            const StepState& S = *c->Sh;
+            done[i] += S.steps_done;
+            c->budget -= std::min(c->budget, S.steps_done);
+            particle_steps += (uint64_t)c->n * S.steps_done;
+            mpm_step_info& L = c->last;
+            L.steps_done += S.steps_done; L.status |= S.err | S.warn;
+            L.dt = S.dt; L.time = S.time; L.step = S.step; L.frame = S.frame; L.plastic_count = S.plastic_count;
+            L.frame_ended = (int32_t)S.frame_ended; L.finished = (int32_t)S.finished;
+            for (int d = 0; d < c->cfg.dim; ++d) c->est_n[d] = S.n[d];
+            if (S.need_realloc) { realloc = true; continue; }
+            const bool ended = S.finished || (S.err & MPM_ERR_EXIT_MASK);
+            if (ended) c->budget = 0;
+            if (ended || (S.stop && S.frame_ended && S.stop_at_frame_end) || (S.steps_done < S.step_limit && S.stop)) want[i] = done[i];
+        }
+        if (realloc) {
+            if (++reallocs > 8) { status |= MPM_ERR_OUT_OF_MEMORY; break; }
+        } else {
+            reallocs = 0;
+        }
+    }
+    if (info) {
+        info->n_scenes = (uint32_t)b->scenes.size();
+        for (mpm_context* c : b->scenes) info->n_active += c->budget > 0 ? 1u : 0u;
+        info->n_classes = (uint32_t)b->classes.size();
+        info->chunks = chunks; info->status = status; info->particle_steps = particle_steps;
+    }
+    return (int)(status & MPM_ERR_EXIT_MASK);
+}
+
+int mpm_scene_info(const mpm_context* c, mpm_step_info* info, uint32_t* budget_left) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    if (info) *info = c->last;
+    if (budget_left) *budget_left = c->budget;
+    return 0;
+}
+
+uint64_t mpm_device_bytes(const mpm_context* c) {
+    if (!c) return 0;
+    const int D = c->cfg.dim;
+    const size_t tf = c->split ? sizeof(float) : sizeof(double);
+    uint64_t bytes = c->pbuf_bytes + c->ibuf_n * sizeof(uint32_t);
+    bytes += (uint64_t)3 * c->cap_blocks * sizeof(uint32_t) + (uint64_t)65 * c->cap_blocks * sizeof(uint16_t);
+    bytes += (uint64_t)kMaxNV * tile_nodes(D, c->cfg.spline_degree) * c->cap_blocks * tf;
+    bytes += (uint64_t)(1 + 2 * D + 1) * c->cap_nodes * tf + (uint64_t)3 * c->cap_axis * sizeof(double);
+    bytes += sizeof(StepState) + (c->dP ? sizeof(Params) : 0);
+    return bytes;
+}
 
 }  // extern "C"

@@ -364,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, Part
 // One workgroup of 64 lanes; lane d builds the node coordinates of axis d (arange, tools.hpp:291-296).
 // ----------------------------------------------------------------------------------------------
 template <int D, class TF>
-__global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrsT<TF> g) {
+__device__ __forceinline__ void setup_body(const Params& P, StepState* S, const GridPtrsT<TF>& g) {
     __shared__ double lo_s[3], stop_s[3];
     __shared__ int go;
     fold_bound_slots<D>(S);
@@ -464,17 +464,20 @@ __global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrsT<
         for (double val = lo_s[d]; val < stop; val += dx) g.coord[d][i++] = val;
     }
 }
+template <int D, class TF>
+__global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrsT<TF> g) { setup_body<D, TF>(P, S, g); }
 
 
 // ----------------------------------------------------------------------------------------------
 // k_zero_grid: clears the block histogram (the grid arrays are written whole by k_grid_update).
 // ----------------------------------------------------------------------------------------------
-template <int = 0>  // a template, so that every translation unit may instantiate it
-__global__ __launch_bounds__(kBlock) void k_zero_grid(const StepState* S, SortBufs sb) {
+__device__ __forceinline__ void zero_grid_body(const StepState* S, const SortBufs& sb, uint32_t bid, uint32_t nbid) {
     if (S->stop) return;
     const uint32_t nb = S->nblocks;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) sb.count[i] = 0;
+    for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nb; i += nbid * blockDim.x) sb.count[i] = 0;
 }
+template <int = 0>  // a template, so that every translation unit may instantiate it
+__global__ __launch_bounds__(kBlock) void k_zero_grid(const StepState* S, SortBufs sb) { zero_grid_body(S, sb, blockIdx.x, gridDim.x); }
 
 // Block coordinates of linear block b.
 template <int D>
@@ -501,9 +504,10 @@ __device__ __forceinline__ uint32_t block_key(const double* xp, const StepState*
 
 // k_bin_count: block of every particle and the histogram (wavefront-aggregated atomics).
 template <int D, int SPL, class TF>
-__global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState* S, PartStateT<TF> src, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
+__device__ __forceinline__ void bin_count_body(const Params& P, const StepState* S, const PartStateT<TF>& src, const SortBufs& sb,
+                                               const GridPtrsT<TF>& g, uint32_t n, uint32_t bid) {
     if (S->stop) return;
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t p = bid * blockDim.x + threadIdx.x;
     const bool active = p < n;
     uint32_t key = 0;
     if (active) {
@@ -514,11 +518,14 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState*
     }
     (void)wave_claim(sb.count, key, active);
 }
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState* S, PartStateT<TF> src, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
+    bin_count_body<D, SPL, TF>(P, S, src, sb, g, n, blockIdx.x);
+}
 
 // k_bin_scan: exclusive prefix sum of the block histogram; one workgroup of 1024 threads, each owning
 // a contiguous run of blocks.
-template <int = 0>  // a template, so that every translation unit may instantiate it
-__global__ __launch_bounds__(1024) void k_bin_scan(const StepState* S, SortBufs sb) {
+__device__ __forceinline__ void bin_scan_body(const StepState* S, const SortBufs& sb) {
     if (S->stop) return;
     __shared__ uint32_t part[1024];
     const uint32_t nb = S->nblocks;
@@ -537,17 +544,20 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const StepState* S, SortBufs 
     uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
     for (uint32_t b = b0; b < b1; ++b) { sb.offset[b] = run; sb.cursor[b] = run; run += sb.count[b]; }
 }
+template <int = 0>  // a template, so that every translation unit may instantiate it
+__global__ __launch_bounds__(1024) void k_bin_scan(const StepState* S, SortBufs sb) { bin_scan_body(S, sb); }
 
 // k_bin_scatter: counting-sort permutation (the order inside a block is fixed later by k_p2g).
-template <int = 0>  // a template, so that every translation unit may instantiate it
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, SortBufs sb, uint32_t n) {
+__device__ __forceinline__ void bin_scatter_body(const StepState* S, const SortBufs& sb, uint32_t n, uint32_t bid) {
     if (S->stop) return;
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t p = bid * blockDim.x + threadIdx.x;
     const bool active = p < n;
     const uint32_t key = active ? sb.key[p] : 0u;
     const uint32_t pos = wave_claim(sb.cursor, key, active);
     if (active) sb.perm[pos] = p;
 }
+template <int = 0>  // a template, so that every translation unit may instantiate it
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, SortBufs sb, uint32_t n) { bin_scatter_body(S, sb, n, blockIdx.x); }
 
 // k_binning: cell base index and block of every particle of a layout (diagnostic download).
 template <int D, int SPL, class TF>
@@ -1075,12 +1085,11 @@ struct MuslContrib {
 // m_i = 0 are never updated or read, so the force is scattered unconditionally.
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, bool SEP, bool MIBF, class TF>
-__global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB))
-void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+__device__ __forceinline__ void p2g_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                         const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
     constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
     __shared__ ScatterLDS<D, SPL, NV, TF> L;
     if (S->stop) return;
-    const uint32_t b = blockIdx.x;
     if (b >= S->nblocks) return;
     const uint32_t cnt = sb.count[b];
     if (cnt == 0) return;
@@ -1115,15 +1124,19 @@ void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortB
     err = wave_or_u32(err);
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+__global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB))
+void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+    p2g_body<D, SPL, SEP, MIBF, TF>(P, S, src, dst, sb, g, stg, blockIdx.x);
+}
 
 // k_musl_scatter: the MUSL re-scatter, in the particle order that k_p2g fixed for the step.
 template <int D, int SPL, class TF>
-__global__ __launch_bounds__((ScatterShape<D, SPL, D, TF>::TB))
-void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+__device__ __forceinline__ void musl_scatter_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                                  const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
     using G = BG<D, SPL>;
     __shared__ ScatterLDS<D, SPL, D, TF> L;
     if (S->stop) return;
-    const uint32_t b = blockIdx.x;
     if (b >= S->nblocks) return;
     const uint32_t cnt = sb.count[b];
     if (cnt == 0) return;
@@ -1148,6 +1161,11 @@ void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> d
     }
     err = wave_or_u32(err);
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
+}
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__((ScatterShape<D, SPL, D, TF>::TB))
+void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+    musl_scatter_body<D, SPL, TF>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }
 
 // Sum over the blocks that cover node gx (at most two per axis), in increasing block order, of the
@@ -1180,7 +1198,8 @@ __device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& 
 // (explicit_euler_update.cpp:183-232, boundary_collision.cpp:5-394), flip = v_new - v_old.
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, bool SEP, bool MIBF, class TF>
-__global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+__device__ __forceinline__ void grid_update_body(const Params& P, StepState* S, const SortBufs& sb, const GridPtrsT<TF>& g,
+                                                 const StagingT<TF>& stg, uint32_t bid, uint32_t nbid) {
     constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
     if (S->stop) return;
     const uint32_t nn = S->grid_nodes;
@@ -1190,7 +1209,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, 
     double dtg[3]; for (int d = 0; d < 3; ++d) dtg[d] = dt * S->gravity[d];
     const bool cons = S->check_conservation;
     double cm = 0.0, cmv[3] = {0, 0, 0};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nn; i += nbid * blockDim.x) {
         int gx[3] = {0, 0, 0};
         if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
         else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
@@ -1228,14 +1247,19 @@ __global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, 
         }
     }
 }
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+__global__ __launch_bounds__(kBlock) void k_grid_update(Params P, StepState* S, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+    grid_update_body<D, SPL, SEP, MIBF, TF>(P, S, sb, g, stg, blockIdx.x, gridDim.x);
+}
 
 // k_musl_update: MUSL normalisation, musl.cpp:182-188: v_i = (sum) / (m_i / m_p), zero where m_i == 0.
 template <int D, int SPL, class TF>
-__global__ __launch_bounds__(kBlock) void k_musl_update(Params P, const StepState* S, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+__device__ __forceinline__ void musl_update_body(const Params& P, const StepState* S, const SortBufs& sb, const GridPtrsT<TF>& g,
+                                                 const StagingT<TF>& stg, uint32_t bid, uint32_t nbid) {
     if (S->stop) return;
     const uint32_t nn = S->grid_nodes;
     const uint32_t n[3] = {S->n[0], S->n[1], S->n[2]};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nn; i += nbid * blockDim.x) {
         int gx[3] = {0, 0, 0};
         if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
         else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
@@ -1249,6 +1273,10 @@ __global__ __launch_bounds__(kBlock) void k_musl_update(Params P, const StepStat
             for (int d = 0; d < D; ++d) g.v[d][i] = TF(0);
         }
     }
+}
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_musl_update(Params P, const StepState* S, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
+    musl_update_body<D, SPL, TF>(P, S, sb, g, stg, blockIdx.x, gridDim.x);
 }
 
 // Velocity update of positionUpdate / MUSL (position_update.cpp:14-23, musl.cpp:13-25).
@@ -1412,9 +1440,9 @@ __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* 
 // Block prologue shared by the gather kernels: returns false when the block has nothing to do.
 struct BlockRange { uint32_t b, start, cnt; int bc[3]; };
 template <int D>
-__device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& sb, BlockRange& r) {
+__device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& sb, uint32_t b, BlockRange& r) {
     if (S->stop) return false;
-    r.b = blockIdx.x;
+    r.b = b;
     if (r.b >= S->nblocks) return false;
     r.cnt = sb.count[r.b];
     if (r.cnt == 0) return false;
@@ -1427,12 +1455,13 @@ __device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& 
 // k_g2p_usl (USL): G2P (g2p.cpp:17-77), deformationUpdate + plasticity, positionUpdate.
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, class TF, int MAT>
-__global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
+__device__ __forceinline__ void g2p_usl_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                             const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b) {
     using G = BG<D, SPL>;
     __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
     __shared__ BlockCoordLDS<D, SPL> s_c;
     BlockRange br;
-    if (!block_range<D>(S, sb, br)) return;
+    if (!block_range<D>(S, sb, b, br)) return;
     const double h1 = P.one_over_dx, dt = S->dt;
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
@@ -1473,18 +1502,23 @@ __global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, Part
         if (err) atomicOr(&S->err, err);
     }
 }
+template <int D, int SPL, class TF, int MAT>
+__global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
+    g2p_usl_body<D, SPL, TF, MAT>(P, S, src, dst, sb, g, blockIdx.x);
+}
 
 // ----------------------------------------------------------------------------------------------
 // k_g2p_musl (MUSL): G2P (g2p.cpp:17-77), the MUSL velocity update (musl.cpp:13-25) and the position
 // update of positionUpdate (position_update.cpp:12).
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, class TF>
-__global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
+__device__ __forceinline__ void g2p_musl_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                              const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b) {
     using G = BG<D, SPL>;
     __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
     __shared__ BlockCoordLDS<D, SPL> s_c;
     BlockRange br;
-    if (!block_range<D>(S, sb, br)) return;
+    if (!block_range<D>(S, sb, b, br)) return;
     const double h1 = P.one_over_dx, dt = S->dt;
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
@@ -1520,6 +1554,10 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
     err = wave_or_u32(err);
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
+template <int D, int SPL, class TF>
+__global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
+    g2p_musl_body<D, SPL, TF>(P, S, src, dst, sb, g, blockIdx.x);
+}
 
 // ----------------------------------------------------------------------------------------------
 // k_defgrad (MUSL): deformationUpdate (deformation_update.cpp:16-46) on the re-scattered velocity.
@@ -1527,9 +1565,10 @@ __global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, Par
 // laid out for occupancy rather than LDS reuse); the velocity gradient gathers from the grid arrays.
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, class TF, int MAT>
-__global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
+__device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                             const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t n, uint32_t bid) {
     if (S->stop) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     uint32_t err = 0, pcount = 0;
     if (i < n) {
         const uint32_t p = sb.perm[i];
@@ -1556,13 +1595,17 @@ __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, Part
         if (err) atomicOr(&S->err, err);
     }
 }
+template <int D, int SPL, class TF, int MAT>
+__global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
+    defgrad_body<D, SPL, TF, MAT>(P, S, src, dst, sb, g, n, blockIdx.x);
+}
 
 // ----------------------------------------------------------------------------------------------
 // k_finish: moveObjects (simulation.cpp:277-281) and the loop bookkeeping of simulate()
 // (simulation.cpp:176-200), conservation checks (simulation.cpp:292-333) when requested.
 // ----------------------------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(64) void k_finish(Params P, StepState* S, uint32_t n_particles) {
+__device__ __forceinline__ void finish_body(const Params& P, StepState* S, uint32_t n_particles) {
     if (S->stop) return;
     uint32_t pc = 0;
     for (int s = threadIdx.x; s < kRedSlots; s += 64) { pc += S->red[s].plastic; S->red[s].plastic = 0; }
@@ -1590,6 +1633,9 @@ __global__ __launch_bounds__(64) void k_finish(Params P, StepState* S, uint32_t 
         if (S->stop_at_frame_end) S->stop = 1;
     }
     if (fabs(P.final_time - S->time) < 1e-15 || P.final_time < S->time) { S->finished = 1; S->stop = 1; }
+    if (S->steps_done >= S->step_limit) S->stop = 1;
 }
+template <int D>
+__global__ __launch_bounds__(64) void k_finish(Params P, StepState* S, uint32_t n_particles) { finish_body<D>(P, S, n_particles); }
 
 }  // namespace mpm

@@ -17,11 +17,11 @@ void enqueue_p2g(mpm_context* c, const PartStateT<TF>& src, const PartStateT<TF>
     { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF, TF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, g, stg); }
 }
 
-// Calls f(std::integral_constant<int, MAT>) for the material of the run (see kMatNeoHookean).
+// Calls f(std::integral_constant<int, MAT>) for material index mat (material_of).
 template <class Fn>
-void with_material(const Params& P, Fn&& f) {
-    if (P.elastic_model == MPM_ELASTIC_NEOHOOKEAN) { f(std::integral_constant<int, kMatNeoHookean>()); return; }
-    switch (P.plastic_model) {
+void with_material_id(int mat, Fn&& f) {
+    if (mat == kMatNeoHookean) { f(std::integral_constant<int, kMatNeoHookean>()); return; }
+    switch (mat) {
     case MPM_PLASTIC_VM: f(std::integral_constant<int, MPM_PLASTIC_VM>()); break;
     case MPM_PLASTIC_DP: f(std::integral_constant<int, MPM_PLASTIC_DP>()); break;
     case MPM_PLASTIC_DPSOFT: f(std::integral_constant<int, MPM_PLASTIC_DPSOFT>()); break;
@@ -34,6 +34,9 @@ void with_material(const Params& P, Fn&& f) {
     default: f(std::integral_constant<int, MPM_PLASTIC_NONE>()); break;
     }
 }
+
+template <class Fn>
+void with_material(const Params& P, Fn&& f) { with_material_id(material_of(P), static_cast<Fn&&>(f)); }
 
 template <int D, int SPL, class TF>
 void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
@@ -75,6 +78,39 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
 }
 
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+void enqueue_batch_p2g(const BatchClass& bc, hipStream_t s) {
+    constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB;
+    kb_p2g<D, SPL, SEP, MIBF, TF><<<bc.wg[BR_BLOCK], TB, 0, s>>>(bc.T);
+    kb_grid_update<D, SPL, SEP, MIBF, TF><<<bc.wg[BR_GRID], kBlock, 0, s>>>(bc.T);
+}
+
+template <int D, int SPL, class TF>
+void enqueue_batch_step(const BatchClass& bc, hipStream_t s) {
+    const BatchTable& T = bc.T;
+    const BatchClassKey& k = bc.key;
+    kb_setup<D, TF><<<T.ns, 64, 0, s>>>(T);
+    kb_zero_grid<TF><<<bc.wg[BR_ZERO], kBlock, 0, s>>>(T);
+    kb_bin_count<D, SPL, TF><<<bc.wg[BR_PART], kBlock, 0, s>>>(T);
+    kb_bin_scan<TF><<<T.ns, 1024, 0, s>>>(T);
+    kb_bin_scatter<TF><<<bc.wg[BR_PART], kBlock, 0, s>>>(T);
+    if (k.sep && k.mibf) enqueue_batch_p2g<D, SPL, true, true, TF>(bc, s);
+    else if (k.sep) enqueue_batch_p2g<D, SPL, true, false, TF>(bc, s);
+    else if (k.mibf) enqueue_batch_p2g<D, SPL, false, true, TF>(bc, s);
+    else enqueue_batch_p2g<D, SPL, false, false, TF>(bc, s);
+    if (k.musl) {
+        constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
+        kb_g2p_musl<D, SPL, TF><<<bc.wg[BR_BLOCK], kBlock, 0, s>>>(T);
+        kb_musl_scatter<D, SPL, TF><<<bc.wg[BR_BLOCK], TBM, 0, s>>>(T);
+        kb_musl_update<D, SPL, TF><<<bc.wg[BR_GRID], kBlock, 0, s>>>(T);
+        with_material_id(k.mat, [&](auto mat) { kb_defgrad<D, SPL, TF, decltype(mat)::value><<<bc.wg[BR_PART], kBlock, 0, s>>>(T); });
+    } else {
+        with_material_id(k.mat, [&](auto mat) { kb_g2p_usl<D, SPL, TF, decltype(mat)::value><<<bc.wg[BR_BLOCK], kBlock, 0, s>>>(T); });
+    }
+    kb_finish<D, TF><<<T.ns, 64, 0, s>>>(T);
+}
+
 }  // namespace mpm
 
-#define MPM_INSTANTIATE_STEP(D, SPL, TF) template void mpm::enqueue_step<D, SPL, TF>(mpm_context*, uint32_t, bool);
+#define MPM_INSTANTIATE_STEP(D, SPL, TF) template void mpm::enqueue_step<D, SPL, TF>(mpm_context*, uint32_t, bool); \
+    template void mpm::enqueue_batch_step<D, SPL, TF>(const mpm::BatchClass&, hipStream_t);

@@ -72,6 +72,8 @@ struct StepState {
     uint32_t stop;          // nonzero: remaining kernels of this batch are no-ops
     uint32_t need_realloc;  // the grid of the next step exceeds capacity
     uint32_t steps_done, frame_ended, finished;
+    uint32_t step_limit;    // stop once steps_done reaches it (a scene's step budget in a batch)
+    uint32_t pad1;
     uint32_t stop_at_frame_end, check_conservation;
     uint32_t cap_axis, cap_nodes;
     uint32_t xsel;          // which particle-state buffer is current (toggles per executed step)

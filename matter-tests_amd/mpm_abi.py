@@ -77,6 +77,37 @@ class KernelTiming(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("total_ms", C.c_double), ("launches", C.c_uint64)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("chunk_steps", C.c_uint32), ("use_graph", C.c_int32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("n_scenes", C.c_uint32), ("n_active", C.c_uint32), ("n_classes", C.c_uint32), ("chunks", C.c_uint32),
+                ("status", C.c_uint32), ("reserved", C.c_uint32), ("particle_steps", C.c_uint64)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_scenes", C.c_uint32), ("chunk_steps", C.c_uint32),
+                ("use_graph", C.c_int32), ("keep_results", C.c_int32)]
+
+
+class RequestDesc(C.Structure):
+    _fields_ = [("cfg", C.POINTER(Config)), ("plates", C.POINTER(Plate)), ("n_plates", C.c_uint32),
+                ("n_objects", C.c_uint32), ("objects", C.POINTER(Object)), ("particles", C.POINTER(ParticleArrays)),
+                ("n", C.c_uint32), ("nsteps", C.c_uint32), ("arrival_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint32), ("completed", C.c_uint32), ("failed", C.c_uint32),
+                ("peak_resident", C.c_uint32), ("peak_resident_bytes", C.c_uint64), ("particle_steps", C.c_uint64),
+                ("wall_s", C.c_double), ("throughput", C.c_double), ("step_ms_p50", C.c_double),
+                ("step_ms_p99", C.c_double), ("request_ms_p50", C.c_double), ("request_ms_p99", C.c_double),
+                ("queue_ms_p50", C.c_double), ("queue_ms_p99", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class GridInfo(C.Structure):
     _fields_ = [("Nx", C.c_uint32), ("Ny", C.c_uint32), ("Nz", C.c_uint32), ("grid_nodes", C.c_uint32),
                 ("xc", C.c_double), ("yc", C.c_double), ("zc", C.c_double)]
@@ -181,6 +212,31 @@ class Library:
             L.mpm_set_profiling.argtypes = [vp, C.c_int]
             L.mpm_get_binning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]
             L.mpm_get_kernel_times.argtypes = [vp, C.POINTER(KernelTiming), C.c_uint32, C.POINTER(C.c_uint32)]
+            L.mpm_abi_version.restype = C.c_int
+            L.mpm_batch_create.argtypes = [C.POINTER(BatchOptions), C.POINTER(vp)]
+            L.mpm_batch_create.restype = C.c_int
+            L.mpm_batch_destroy.argtypes = [vp]
+            L.mpm_batch_destroy.restype = None
+            for name, args in (("mpm_batch_add", [vp, vp, C.c_uint32]), ("mpm_batch_remove", [vp, vp]),
+                               ("mpm_batch_set_budget", [vp, vp, C.c_uint32]),
+                               ("mpm_batch_step", [vp, C.c_uint32, C.c_uint32, C.POINTER(BatchInfo)]),
+                               ("mpm_scene_info", [vp, C.POINTER(StepInfo), C.POINTER(C.c_uint32)])):
+                f = getattr(L, name)
+                f.argtypes = args
+                f.restype = C.c_int
+            L.mpm_device_bytes.argtypes = [vp]
+            L.mpm_device_bytes.restype = C.c_uint64
+            L.mpm_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(vp)]
+            L.mpm_server_create.restype = C.c_int
+            L.mpm_server_destroy.argtypes = [vp]
+            L.mpm_server_destroy.restype = None
+            L.mpm_server_submit.argtypes = [vp, C.POINTER(RequestDesc), C.POINTER(C.c_uint64)]
+            L.mpm_server_submit.restype = C.c_int
+            L.mpm_server_run.argtypes = [vp, C.c_double, C.POINTER(ServerStats)]
+            L.mpm_server_run.restype = C.c_int
+            L.mpm_server_result.argtypes = [vp, C.c_uint64, C.POINTER(ParticleArrays), C.POINTER(StepInfo),
+                                            C.POINTER(C.c_double)]
+            L.mpm_server_result.restype = C.c_int
         else:
             L.oracle_create.argtypes = [C.POINTER(Config)]
             L.oracle_create.restype = vp
@@ -353,3 +409,131 @@ class Engine:
         cnt = C.c_uint32()
         self._check(self.L.fn("get_plates")(self.h, arr, cap, C.byref(cnt)), "get_plates")
         return [arr[i] for i in range(cnt.value)]
+
+
+class Batch:
+    """A batch of product-engine contexts advanced together (mpm_batch_* of include/mpm.h)."""
+
+    def __init__(self, lib=None, chunk_steps=16, use_graph=True):
+        self.L = lib or product_library()
+        opt = BatchOptions(chunk_steps, 1 if use_graph else 0)
+        h = C.c_void_p()
+        rc = self.L.lib.mpm_batch_create(C.byref(opt), C.byref(h))
+        if rc:
+            raise MPMError(rc, "mpm_batch_create")
+        self.h = h
+        self.members = []
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.L.lib.mpm_batch_destroy(self.h)
+        self.h = None
+        self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, eng, nsteps):
+        rc = self.L.lib.mpm_batch_add(self.h, eng.h, nsteps)
+        if rc:
+            raise MPMError(rc, "mpm_batch_add")
+        self.members.append(eng)
+
+    def remove(self, eng):
+        rc = self.L.lib.mpm_batch_remove(self.h, eng.h)
+        if rc:
+            raise MPMError(rc, "mpm_batch_remove")
+        self.members.remove(eng)
+
+    def set_budget(self, eng, nsteps):
+        rc = self.L.lib.mpm_batch_set_budget(self.h, eng.h, nsteps)
+        if rc:
+            raise MPMError(rc, "mpm_batch_set_budget")
+
+    def step(self, max_steps, flags=0):
+        info = BatchInfo()
+        rc = self.L.lib.mpm_batch_step(self.h, max_steps, flags, C.byref(info))
+        return rc, info
+
+    @staticmethod
+    def scene_info(eng):
+        info = StepInfo()
+        left = C.c_uint32()
+        rc = eng.L.lib.mpm_scene_info(eng.h, C.byref(info), C.byref(left))
+        if rc:
+            raise MPMError(rc, "mpm_scene_info")
+        return info, left.value
+
+
+class Server:
+    """The served form (mpm_server_* of include/mpm.h): requests with arrival times, admitted by device
+    footprint into a continuously batched run."""
+
+    def __init__(self, lib=None, hbm_budget_bytes=0, max_scenes=0, chunk_steps=8, use_graph=True, keep_results=False):
+        self.L = lib or product_library()
+        opt = ServerOptions(hbm_budget_bytes, max_scenes, chunk_steps, 1 if use_graph else 0, 1 if keep_results else 0)
+        h = C.c_void_p()
+        rc = self.L.lib.mpm_server_create(C.byref(opt), C.byref(h))
+        if rc:
+            raise MPMError(rc, "mpm_server_create")
+        self.h = h
+        self.dims = []
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.L.lib.mpm_server_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, cfg, plates, objects, x, nsteps, arrival_s, **fields):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        keep = [x]
+        pa = ParticleArrays()
+        pa.x = _dptr(x)
+        for k, a in fields.items():
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if k in ("F", "Bmat") and a.ndim == 3:
+                a = np.ascontiguousarray(a.transpose(0, 2, 1))
+            keep.append(a)
+            setattr(pa, k, _dptr(a))
+        parr = (Plate * max(1, len(plates)))(*plates)
+        oarr = (Object * max(1, len(objects)))(*objects)
+        d = RequestDesc(C.pointer(cfg), parr, len(plates), len(objects), oarr, C.pointer(pa), x.shape[0], nsteps,
+                        float(arrival_s))
+        rid = C.c_uint64()
+        rc = self.L.lib.mpm_server_submit(self.h, C.byref(d), C.byref(rid))
+        if rc:
+            raise MPMError(rc, "mpm_server_submit")
+        self.dims.append((cfg.dim, x.shape[0]))
+        return rid.value
+
+    def run(self, time_limit_s=0.0):
+        st = ServerStats()
+        rc = self.L.lib.mpm_server_run(self.h, time_limit_s, C.byref(st))
+        return rc, st
+
+    def result(self, rid):
+        dim, n = self.dims[rid]
+        out = {"x": np.zeros((n, dim)), "v": np.zeros((n, dim)), "F": np.zeros((n, dim * dim)),
+               "Bmat": np.zeros((n, dim * dim))}
+        for k in SCALARS:
+            out[k] = np.zeros(n)
+        pa = ParticleArrays()
+        for k, a in out.items():
+            setattr(pa, k, _dptr(a))
+        info = StepInfo()
+        lat = C.c_double()
+        rc = self.L.lib.mpm_server_result(self.h, rid, C.byref(pa), C.byref(info), C.byref(lat))
+        if rc:
+            raise MPMError(rc, "mpm_server_result")
+        out["F"] = out["F"].reshape(n, dim, dim).transpose(0, 2, 1).copy()
+        out["Bmat"] = out["Bmat"].reshape(n, dim, dim).transpose(0, 2, 1).copy()
+        return out, info, lat.value

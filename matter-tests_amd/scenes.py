@@ -84,3 +84,104 @@ def column_2d(n_side=100, dx=0.01, ppc_axis=2, seed=42, end_frame=1, fps=1.0):
                            end_frame=end_frame, fps=fps)
     plates = [A.make_plate(0.0, "bottom", "NoSlip", 0.0)]
     return cfg, plates, [], x
+
+
+def batch_collapses(count=128, seed=42, n_min=5_000, n_max=500_000, precision="split", dx=0.005, musl=False):
+    """Independent granular column collapses of the batched workload: per scene, dimension 2 or 3 (even
+    odds), particle count log-uniform in [n_min, n_max], aspect ratio height/width uniform in [0.5, 2],
+    friction angle uniform in [20, 40] deg, and material Drucker-Prager or Perzyna-von Mises (even odds;
+    yield stress q_y uniform in [1e3, 1e4] Pa, viscosity eta uniform in [1, 100] Pa s, given to the reference's
+    Perzyna law as perzyna_visc = eta / q_y in seconds, with q_max = q_min = q_y and exponent 1). Quadratic
+    B-spline, APIC, USL by default, 4 (2D) or 8 (3D) particles per cell, NoSlip floor in 2D and SlipFree
+    floor with mu = tan(friction angle) in 3D, E = 1e6 Pa, nu = 0.3, rho = 1550. Draws come from stream
+    1000 + k of the seed. Returns a list of (name, cfg, plates, objects, x)."""
+    out = []
+    h = dx / 2
+    for k in range(count):
+        u = uniform(seed, 6, stream=1000 + k)
+        dim = 2 if u[0] < 0.5 else 3
+        n_target = math.exp(math.log(n_min) + u[1] * (math.log(n_max) - math.log(n_min)))
+        aspect = 0.5 + 1.5 * u[2]
+        phi = 20.0 + 20.0 * u[3]
+        model = "DP" if u[4] < 0.5 else "VMVisc"
+        if dim == 3:
+            nx = max(4, int(round((n_target / aspect) ** (1.0 / 3.0))))
+            counts = [nx, max(4, int(round(aspect * nx))), nx]
+        else:
+            nx = max(4, int(round(math.sqrt(n_target / aspect))))
+            counts = [nx, max(4, int(round(aspect * nx)))]
+        x = jittered_lattice(counts, h, [0.0] * dim, seed=seed + 1 + k)
+        vol = h ** dim
+        params = dict(dim=dim, spline_degree=2, precision=precision, E=1e6, nu=0.3, rho=1550.0,
+                      M=math.tan(math.radians(phi)), q_cohesion=0.0, use_musl=1 if musl else 0, flip_ratio=-2.0,
+                      cfl=0.5, gravity=[0.0, -9.81, 0.0][:dim], dx=dx, particle_volume=vol,
+                      particle_mass=1550.0 * vol, end_frame=1000, fps=1.0)
+        if model == "VMVisc":
+            qy = 1e3 + 9e3 * u[5]
+            eta = 1.0 + 99.0 * uniform(seed, 7, stream=1000 + k)[6]
+            params.update(plastic_model="VMVisc", q_max=qy, q_min=qy, perzyna_exp=1.0, perzyna_visc=eta / qy, xi=0.0)
+        else:
+            params.update(plastic_model="DP")
+        cfg = A.default_config(**params)
+        if dim == 3:
+            plates = [A.make_plate(0.0, "bottom", "SlipFree", math.tan(math.radians(phi)))]
+        else:
+            plates = [A.make_plate(0.0, "bottom", "NoSlip", 0.0)]
+        out.append((f"scene{k:03d}_{dim}d_{model}", cfg, plates, [], x))
+    return out
+
+
+def incline_flow(n_target=None, dx=0.004, ppc_axis=2, seed=42, precision="split", slope_deg=35.0, lattice=None):
+    """Flow down an inclined plane with a rectangular bump (the analytic-plane configuration): cubic
+    B-spline, AFLIP (flip_ratio = -1), mu(I)-Modified Cam-Clay (mu_s = 0.38, mu_2 = 0.64, I_0 = 0.279, grain
+    diameter 1 mm, rho_s = 2500 kg/m^3, solid fraction 0.6 so rho = 1500 kg/m^3, MCC defaults p0 = 1000 Pa,
+    beta = 0, xi = 0, ExpoImpl hardening), Hencky K = 1e7 Pa with nu = 0.3 (E = 3K(1 - 2 nu) = 1.2e7 Pa), MIBF
+    friction, SlipStick floor and a SlipStick box bump, gravity tilted by the slope angle. The slab has the
+    proportions 1.0 x 0.1 x 0.3 m (along the slope x, height y, width z); `lattice` gives its lattice counts
+    directly, otherwise they are chosen so that the slab holds about n_target particles at ppc_axis^3
+    particles per cell. Returns (cfg, plates, objects, x)."""
+    h = dx / ppc_axis
+    if lattice is None:
+        nx = max(8, int(round((n_target / 0.03) ** (1.0 / 3.0))))
+        lattice = [nx, max(4, int(round(0.1 * nx))), max(4, int(round(0.3 * nx)))]
+    L = [c * h for c in lattice]
+    x = jittered_lattice(lattice, h, [0.0, 0.0, 0.0], seed=seed)
+    th = math.radians(slope_deg)
+    g = [9.81 * math.sin(th), -9.81 * math.cos(th), 0.0]
+    K, nu = 1e7, 0.3
+    E = 3.0 * K * (1.0 - 2.0 * nu)
+    vol = h ** 3
+    rho = 0.6 * 2500.0
+    cfg = A.default_config(dim=3, spline_degree=3, precision=precision, plastic_model="MCCMui", hardening_law="ExpoImpl",
+                           mu_1=0.38, mu_2=0.64, I_ref=0.279, grain_diameter=1e-3, rho_s=2500.0, E=E, nu=nu, rho=rho,
+                           use_mibf=1, flip_ratio=-1.0, gravity=g, dx=dx, particle_volume=vol, particle_mass=rho * vol,
+                           p0=1000.0, beta=0.0, xi=0.0, cfl=0.5, end_frame=1000, fps=1.0)
+    plates = [A.make_plate(0.0, "bottom", "SlipStick", 0.38)]
+    bump = A.make_object("BoxRotated", "SlipStick", 0.38, center=(L[0] + 0.1 * L[0], 0.0, 0.5 * L[2]),
+                         half_extents=(0.02 * L[0] + 2 * dx, 0.3 * L[1] + dx, L[2]), rotation=np.eye(3))
+    return cfg, plates, [bump], x
+
+
+def served_requests(count=48, rate=50.0, seed=42, n_min=10_000, n_max=2_000_000, steps_min=50, steps_max=500,
+                    precision="split"):
+    """Open-loop Poisson stream of scene requests: inter-arrival times exponential with mean 1/rate, each
+    request a 3D collapse (Drucker-Prager, quadratic, APIC, MUSL) or an inclined flow (incline_flow) with even
+    odds, particle count log-uniform in [n_min, n_max], steps uniform in [steps_min, steps_max]. Draws come
+    from stream 5000 + k of the seed. Returns a list of (name, cfg, plates, objects, x, steps, arrival_s)."""
+    out = []
+    t = 0.0
+    for k in range(count):
+        u = uniform(seed, 4, stream=5000 + k)
+        t += -math.log(1.0 - u[0]) / rate
+        n_target = math.exp(math.log(n_min) + u[1] * (math.log(n_max) - math.log(n_min)))
+        steps = steps_min + int(u[2] * (steps_max - steps_min + 1))
+        if u[3] < 0.5:
+            side = max(8, int(round(n_target ** (1.0 / 3.0))))
+            cfg, plates, objs, x = dp_collapse(n_per_axis=side, musl=True, precision=precision, seed=seed + 100 + k,
+                                               end_frame=1000)
+            name = f"req{k:03d}_collapse"
+        else:
+            cfg, plates, objs, x = incline_flow(n_target, seed=seed + 100 + k, precision=precision)
+            name = f"req{k:03d}_incline"
+        out.append((name, cfg, plates, objs, x, steps, t))
+    return out

@@ -82,7 +82,7 @@ def test_status_strings_and_version():
     L = A.product_library().lib
     assert L.mpm_status_string(0) == b"ok"
     assert b"Newton" in L.mpm_status_string(A.ERR["NEWTON_MAXITER"])
-    assert L.mpm_abi_version() == 1
+    assert L.mpm_abi_version() == 2
 
 
 def test_ctypes_struct_sizes_match_c_layout():
