@@ -1,17 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the MPM timestep engine on one MI355X.
 
-Workload (default): 3D granular column collapse, 0.4 m cube, dx = 0.005 m, 8 particles per cell on a
-seeded jittered lattice (4,096,000 particles), quadratic B-spline, MUSL, APIC, Hencky + Drucker-Prager
-(30 deg, E = 1e6 Pa, nu = 0.3, rho = 1550 kg/m^3), SlipFree floor with Coulomb mu = 0.4, gravity 9.81,
-plain FP64 form. A step is one advanceStep() of all particles (+ the time bookkeeping of simulate()).
+Headline workload: 3D granular column collapse, 0.4 m cube, dx = 0.005 m, 8 particles per cell on a seeded
+jittered lattice (4,096,000 particles), quadratic B-spline, MUSL, APIC, Hencky + Drucker-Prager (30 deg,
+E = 1e6 Pa, nu = 0.3, rho = 1550 kg/m^3), SlipFree floor with Coulomb mu = 0.4, gravity 9.81, in the
+FP32/FP64 split form (`value`), with the plain FP64 form beside it (`fp64`). A step is one advanceStep() of
+all particles (+ the time bookkeeping of simulate()).
 
 Prints one JSON line: throughput in particle-updates/s (particles x steps / device-synchronised wall time of
 the timed steps), the roofline of the dominant kernel from per-kernel HIP-event durations over the timed
-region, the parity of the same configuration against the CPU oracle at a small size, and the oracle timed
-on the host cores on a bounded sample of the same workload (a reported baseline, not a target).
+region, the split form's position error against the plain form, the parity of the same configuration
+against the CPU oracle at a small size, the oracle timed on the host cores on a bounded sample of the same
+workload (a reported baseline, not a target), and the extra workloads: the inclined flow (`c3`), 128 batched
+scenes (`batched`) and a served Poisson request stream (`served`, with p50/p99 step and request latency).
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--no-cpu-baseline] [--extra c3,batched,served]
 """
 import argparse
 import json
@@ -53,7 +56,7 @@ def algorithmic_bytes(kernel, cfg, n_particles, active_nodes, grid_nodes, blocks
     if kernel == "k_grid_update":
         out = 1 + D + (D if flip else 0) + (1 if mibf else 0)
         return ft * (nv * staged + out * grid_nodes)
-    if kernel == "k_g2p_musl":
+    if kernel == "k_g2p":
         part_r = f8 * D + (ft * D if flip else 0)
         part_w = f8 * D + ft * (B + D)
         return (part_r + part_w) * n_particles + ft * (D + (D if flip else 0)) * active_nodes
@@ -61,12 +64,10 @@ def algorithmic_bytes(kernel, cfg, n_particles, active_nodes, grid_nodes, blocks
         return (f8 * D + ft * (D + B)) * n_particles + ft * D * staged
     if kernel == "k_musl_update":
         return ft * (D * staged + (1 + D) * grid_nodes)
-    if kernel in ("k_defgrad", "k_g2p_usl"):
-        usl = kernel == "k_g2p_usl"
-        part_r = f8 * (D + D * D + 2) + (ft * D if usl and flip else 0)
-        part_w = f8 * (D * D + 3) + ft * tau + ((f8 * D + ft * (D + B)) if usl else 0)
-        node = ft * (D + (D if (usl and flip) else 0))
-        return (part_r + part_w) * n_particles + node * active_nodes
+    if kernel == "k_defgrad":
+        part_r = f8 * (D + D * D + 2)
+        part_w = f8 * (D * D + 3) + ft * tau
+        return (part_r + part_w) * n_particles + ft * D * active_nodes
     if kernel == "k_bin_count":
         return f8 * D * n_particles + 4 * n_particles
     if kernel == "k_bin_scatter":
@@ -308,8 +309,13 @@ def batched_run(args, count=128):
     del scs
     total = sum(e.n for e in engs)
     b = A.Batch(L, chunk_steps=16, use_graph=True)
+    tune_steps = 180
     for e in engs:
-        b.add(e, args.warmup + args.steps)
+        b.add(e, tune_steps + args.warmup + args.steps)
+    chunk, tuned_rate = b.autotune(tune_steps)  # these steps are ordinary steps of the scenes
+    rc, tinfo = b.step(tune_steps)  # scenes that took fewer tuning steps catch up, so all start the timed region level
+    if rc:
+        raise A.MPMError(rc, "batched tuning steps")
     if args.warmup:
         rc, _ = b.step(args.warmup)
         if rc:
@@ -331,7 +337,8 @@ def batched_run(args, count=128):
     return {"workload": f"{count} independent 2D/3D collapse scenes, DP / Perzyna-VM, quadratic APIC USL",
             "scenes": count, "scenes_3d": int(sum(d == 3 for d in dims)), "particles": total, "precision": "split",
             "value": total * args.steps / wall, "unit": "particle-updates/s", "ms_per_step": 1e3 * wall / args.steps,
-            "steps": args.steps, "launch_classes": info.n_classes, "chunk_steps": 16, "hip_graph": True,
+            "steps": args.steps, "launch_classes": info.n_classes, "chunk_steps": chunk, "hip_graph": True,
+            "autotune": {"candidates": [4, 8, 16, 32], "chosen": chunk, "rate_at_chosen": tuned_rate},
             "unbatched_value": total * args.steps / (t3 - t2), "batching_speedup": (t3 - t2) / wall}
 
 

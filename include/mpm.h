@@ -239,6 +239,23 @@ int mpm_get_plates(mpm_context* ctx, mpm_plate* plates, uint32_t capacity, uint3
  * i_base/j_base/k_base expressions of the lines cited above. */
 int mpm_get_binning(mpm_context* ctx, int32_t* cell_base, uint32_t* block_key);
 
+/* Per-particle output quantities of Simulation::saveParticleData (save_data.cpp:9-38), computed on the
+ * device in the order of mpm_set_particles: p = -tr(tau)/dim, q = sqrt(3/2 tau_dev:tau_dev) if use_mises_q
+ * else sqrt(1/2 tau_dev:tau_dev), Je = det(F), with tau = HenckyPiola(F) F^T (NeoHookeanPiola for the
+ * Neo-Hookean model). Any pointer may be NULL. */
+int mpm_get_stress_measures(mpm_context* ctx, double* p, double* q, double* Je);
+
+/* Writes the particle file of Simulation::saveParticleData (save_data.cpp:7-171): binary little-endian
+ * PLY, element "vertex" with float64 properties x y [z] vx vy [vz] p q Je, then, when a plastic model is
+ * set, eps_pl_vol eps_pl_dev delta_gamma, then muI (DPVisc or MCCVisc with use_mibf, DPMui, MCCMui) and
+ * viscosity (DPMui, MCCMui), one row per particle in the order of mpm_set_particles. */
+int mpm_write_particles_ply(mpm_context* ctx, const char* path);
+
+/* Writes the grid file of Simulation::saveGridData (save_data.cpp:173-259): binary little-endian PLY,
+ * float64 properties x y [z] vx vy [vz] mass [friction (use_mibf)], one row per node of the last step's
+ * grid in the order ind(i,j,k) (node coordinates from grid.x/y/z). */
+int mpm_write_grid_ply(mpm_context* ctx, const char* path);
+
 /* Accumulated device time of one kernel of the step pipeline. */
 typedef struct mpm_kernel_timing {
     char name[32];       /* kernel name, e.g. "k_p2g" */
@@ -301,6 +318,13 @@ int mpm_batch_set_budget(mpm_batch* b, mpm_context* scene, uint32_t nsteps);
  * end), and a member that finished or failed gets budget 0. Returns the batch-level error bits;
  * per-member results come from mpm_scene_info. */
 int mpm_batch_step(mpm_batch* b, uint32_t max_steps, uint32_t flags, mpm_batch_info* info);
+
+/* Picks the chunk length (steps per host synchronisation and per graph) of the batch: for each candidate
+ * in {4, 8, 16, 32} it advances the members by one warm-up chunk (graph capture) and two timed chunks,
+ * within their budgets and at most max_steps steps in all, and keeps the length with the highest
+ * particle-updates per second. The steps taken are ordinary steps of the members (results do not depend on
+ * the chunk length). *chosen receives the length; *rate (optional) the measured particle-updates/s. */
+int mpm_batch_autotune(mpm_batch* b, uint32_t max_steps, uint32_t* chosen, double* rate);
 
 /* Result of the member's part of the last mpm_batch_step (as mpm_step's info) and its remaining budget. */
 int mpm_scene_info(const mpm_context* scene, mpm_step_info* info, uint32_t* budget_left);

@@ -125,9 +125,9 @@ __global__ __launch_bounds__(kBlock) void kb_grid_update(BatchTable T) {
 }
 
 template <int D, int SPL, class TF>
-__global__ __launch_bounds__(kBlock) void kb_g2p_musl(BatchTable T) {
+__global__ __launch_bounds__(kBlock) void kb_g2p(BatchTable T) {
     BatchSlot<TF> q(T, BR_BLOCK);
-    g2p_musl_body<D, SPL, TF>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.local);
+    g2p_body<D, SPL, TF>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.local);
 }
 
 template <int D, int SPL, class TF>
@@ -146,12 +146,6 @@ template <int D, int SPL, class TF, int MAT>
 __global__ __launch_bounds__(kBlock) void kb_defgrad(BatchTable T) {
     BatchSlot<TF> q(T, BR_PART);
     defgrad_body<D, SPL, TF, MAT>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.r->n, q.local);
-}
-
-template <int D, int SPL, class TF, int MAT>
-__global__ __launch_bounds__(kBlock) void kb_g2p_usl(BatchTable T) {
-    BatchSlot<TF> q(T, BR_BLOCK);
-    g2p_usl_body<D, SPL, TF, MAT>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.local);
 }
 
 template <int D, class TF>

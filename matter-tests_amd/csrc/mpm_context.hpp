@@ -67,10 +67,10 @@ template <class TF> StagingT<TF>& staging_of(mpm_context* c);
 template <> inline StagingT<double>& staging_of<double>(mpm_context* c) { return c->stg; }
 template <> inline StagingT<float>& staging_of<float>(mpm_context* c) { return c->stgf; }
 
-inline const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p_usl", "k_g2p_musl",
+inline const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p",
                                     "k_musl_scatter", "k_musl_update", "k_defgrad", "k_finish", "k_bin_count", "k_bin_scan",
                                     "k_bin_scatter"};
-enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P_USL, KS_G2P_MUSL, KS_MUSL_SCATTER, KS_MUSL_UPDATE, KS_DEFGRAD,
+enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P, KS_MUSL_SCATTER, KS_MUSL_UPDATE, KS_DEFGRAD,
                 KS_FINISH, KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_COUNT };
 
 // Records an event pair around one launch when profiling; returns the launch's slot.
@@ -103,8 +103,14 @@ inline void collect_timings(mpm_context* c) {
 // (dimension, degree, type) in its own translation unit so that the engine compiles in parallel).
 template <int D, int SPL, class TF> void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons);
 
-// Material index of a run: the plastic model on Hencky elasticity, or kMatNeoHookean.
-inline int material_of(const Params& P) { return P.elastic_model == MPM_ELASTIC_NEOHOOKEAN ? kMatNeoHookean : P.plastic_model; }
+// Material index of a run: the plastic model on Hencky elasticity, kMatNeoHookean, or a closed-form
+// Perzyna variant (kMatVMViscLinear, kMatDPViscLinear) when the parameters select that branch.
+inline int material_of(const Params& P) {
+    if (P.elastic_model == MPM_ELASTIC_NEOHOOKEAN) return kMatNeoHookean;
+    if (P.plastic_model == MPM_PLASTIC_VMVISC && P.perzyna_exp == 1 && P.xi == 0) return kMatVMViscLinear;
+    if (P.plastic_model == MPM_PLASTIC_DPVISC && P.perzyna_exp == 1) return kMatDPViscLinear;
+    return P.plastic_model;
+}
 
 // Compile-time switches shared by the scenes of one batched launch.
 struct BatchClassKey {

@@ -12,6 +12,9 @@
 #include <new>
 #include <type_traits>
 #include <cstddef>
+#include <cstdio>
+#include <chrono>
+#include <string>
 
 using namespace mpm;
 
@@ -348,15 +351,18 @@ struct mpm_batch {
     char* dtab = nullptr; size_t dtab_bytes = 0;     // refs, range starts, limits, StepState pointers, heads
     uint32_t* hlimit = nullptr; uint32_t* hhead = nullptr; size_t hcap = 0;  // pinned
     uint32_t* dlimit = nullptr; uint32_t* dhead = nullptr; StepState** dSp = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    uint32_t g_chunk = 0, g_flags = 0;
+    struct Graph { uint32_t chunk, flags; hipGraphExec_t exec; };
+    std::vector<Graph> graphs;            // instantiated chunks of the current tables, by (steps, flags)
     std::mutex* capture_lock = nullptr;   // see batch_set_capture_lock
 };
 
 namespace {
 
+constexpr size_t kMaxGraphs = 4;
+
 void drop_graph(mpm_batch* b) {
-    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+    b->graphs.clear();
 }
 
 // Groups the members into classes and uploads every class's scene table and range starts.
@@ -453,8 +459,8 @@ int launch_chunk(mpm_batch* b, uint32_t chunk, uint32_t flags) {
         enqueue_chunk(b, chunk, flags);
         return hipGetLastError() == hipSuccess ? 0 : MPM_ERR_HIP;
     }
-    if (!b->gexec || b->g_chunk != chunk || b->g_flags != flags) {
-        drop_graph(b);
+    auto it = std::find_if(b->graphs.begin(), b->graphs.end(), [&](const mpm_batch::Graph& g) { return g.chunk == chunk && g.flags == flags; });
+    if (it == b->graphs.end()) {
         std::unique_lock<std::mutex> lk;
         if (b->capture_lock) {
             lk = std::unique_lock<std::mutex>(*b->capture_lock, std::try_to_lock);
@@ -468,15 +474,43 @@ int launch_chunk(mpm_batch* b, uint32_t chunk, uint32_t flags) {
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(b->stream, &graph);
         if (e != hipSuccess || !graph) { if (graph) (void)hipGraphDestroy(graph); return MPM_ERR_HIP; }
-        const hipError_t ei = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
-        if (ei != hipSuccess) { b->gexec = nullptr; return MPM_ERR_HIP; }
-        b->g_chunk = chunk; b->g_flags = flags;
+        if (ei != hipSuccess) return MPM_ERR_HIP;
+        if (b->graphs.size() >= kMaxGraphs) { (void)hipGraphExecDestroy(b->graphs.front().exec); b->graphs.erase(b->graphs.begin()); }
+        b->graphs.push_back({chunk, flags, exec});
+        it = b->graphs.end() - 1;
     }
-    HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+    HIP_TRY(hipGraphLaunch(it->exec, b->stream));
     return 0;
 }
 
+}  // namespace
+
+namespace {
+// Binary little-endian PLY with one element "vertex" of float64 properties, rows interleaved.
+int write_ply(const char* path, size_t rows, const std::vector<std::string>& names, const std::vector<const double*>& cols,
+              const std::vector<int>& stride) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return MPM_ERR_INVALID_ARGUMENT;
+    std::string h = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(rows) + "\n";
+    for (const std::string& nm : names) h += "property double " + nm + "\n";
+    h += "end_header\n";
+    bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+    std::vector<double> row(names.size());
+    std::vector<double> buf;
+    buf.reserve(names.size() * 4096);
+    for (size_t r = 0; r < rows && ok; ++r) {
+        for (size_t k = 0; k < names.size(); ++k) buf.push_back(cols[k][r * (size_t)stride[k]]);
+        if (buf.size() >= names.size() * 4096 || r + 1 == rows) {
+            ok = std::fwrite(buf.data(), sizeof(double), buf.size(), f) == buf.size();
+            buf.clear();
+        }
+    }
+    if (std::fclose(f) != 0) ok = false;
+    return ok ? 0 : MPM_ERR_INVALID_ARGUMENT;
+}
 }  // namespace
 
 namespace mpm {
@@ -742,6 +776,102 @@ int mpm_get_binning(mpm_context* c, int32_t* cell_base, uint32_t* block_key) {
     return 0;
 }
 
+int mpm_get_stress_measures(mpm_context* c, double* p, double* q, double* Je) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c->ready) return MPM_ERR_NOT_READY;
+    if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint32_t n = c->n;
+    double* d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(double) * 3 * (size_t)n));
+    const uint32_t blocks = (n + kBlock - 1) / kBlock;
+    const uint32_t xs = c->Sh->xsel & 1;
+    if (c->cfg.dim == 3) {
+        if (c->split) k_stress_measures<3, float><<<blocks, kBlock, 0, c->stream>>>(c->P, c->pf[xs], n, d, d + n, d + 2 * (size_t)n);
+        else k_stress_measures<3, double><<<blocks, kBlock, 0, c->stream>>>(c->P, c->ps[xs], n, d, d + n, d + 2 * (size_t)n);
+    } else {
+        if (c->split) k_stress_measures<2, float><<<blocks, kBlock, 0, c->stream>>>(c->P, c->pf[xs], n, d, d + n, d + 2 * (size_t)n);
+        else k_stress_measures<2, double><<<blocks, kBlock, 0, c->stream>>>(c->P, c->ps[xs], n, d, d + n, d + 2 * (size_t)n);
+    }
+    int rc = 0;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) rc = MPM_ERR_HIP;
+    double* outs[3] = {p, q, Je};
+    for (int k = 0; k < 3 && !rc; ++k)
+        if (outs[k] && hipMemcpy(outs[k], d + (size_t)k * n, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) rc = MPM_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
+}
+
+
+int mpm_write_particles_ply(mpm_context* c, const char* path) {
+    if (!c || !path) return MPM_ERR_INVALID_ARGUMENT;
+    if (!c->ready) return MPM_ERR_NOT_READY;
+    const int D = c->cfg.dim;
+    const size_t n = c->n;
+    std::vector<double> x(n * D), v(n * D), sc[6], pq[3];
+    for (auto& a : sc) a.resize(n);
+    for (auto& a : pq) a.resize(n);
+    mpm_particle_arrays a{};
+    a.x = x.data(); a.v = v.data(); a.eps_pl_dev = sc[0].data(); a.eps_pl_vol = sc[1].data(); a.delta_gamma = sc[3].data();
+    a.viscosity = sc[4].data(); a.muI = sc[5].data();
+    int rc = mpm_get_particles(c, &a);
+    if (!rc) rc = mpm_get_stress_measures(c, pq[0].data(), pq[1].data(), pq[2].data());
+    if (rc) return rc;
+    std::vector<std::string> names;
+    std::vector<const double*> cols;
+    std::vector<int> stride;
+    const char* ax[3] = {"x", "y", "z"};
+    const char* av[3] = {"vx", "vy", "vz"};
+    for (int d = 0; d < D; ++d) { names.push_back(ax[d]); cols.push_back(x.data() + d); stride.push_back(D); }
+    for (int d = 0; d < D; ++d) { names.push_back(av[d]); cols.push_back(v.data() + d); stride.push_back(D); }
+    const char* pn[3] = {"p", "q", "Je"};
+    for (int k = 0; k < 3; ++k) { names.push_back(pn[k]); cols.push_back(pq[k].data()); stride.push_back(1); }
+    const int pm = c->cfg.plastic_model;
+    if (pm != MPM_PLASTIC_NONE) {
+        names.push_back("eps_pl_vol"); cols.push_back(sc[1].data()); stride.push_back(1);
+        names.push_back("eps_pl_dev"); cols.push_back(sc[0].data()); stride.push_back(1);
+        names.push_back("delta_gamma"); cols.push_back(sc[3].data()); stride.push_back(1);
+        const bool mui = pm == MPM_PLASTIC_DPMUI || pm == MPM_PLASTIC_MCCMUI;
+        if (((pm == MPM_PLASTIC_DPVISC || pm == MPM_PLASTIC_MCCVISC) && c->cfg.use_mibf) || mui) {
+            names.push_back("muI"); cols.push_back(sc[5].data()); stride.push_back(1);
+        }
+        if (mui) { names.push_back("viscosity"); cols.push_back(sc[4].data()); stride.push_back(1); }
+    }
+    return write_ply(path, n, names, cols, stride);
+}
+
+int mpm_write_grid_ply(mpm_context* c, const char* path) {
+    if (!c || !path) return MPM_ERR_INVALID_ARGUMENT;
+    mpm_grid_info gi{};
+    int rc = mpm_get_grid(c, &gi, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    const int D = c->cfg.dim;
+    const size_t nn = gi.grid_nodes;
+    std::vector<double> ax[3] = {std::vector<double>(gi.Nx), std::vector<double>(gi.Ny), std::vector<double>(gi.Nz)};
+    std::vector<double> mass(nn), v(nn * D), fric(nn);
+    rc = mpm_get_grid(c, &gi, ax[0].data(), ax[1].data(), D == 3 ? ax[2].data() : nullptr, mass.data(), v.data(), nullptr,
+                      c->cfg.use_mibf ? fric.data() : nullptr);
+    if (rc) return rc;
+    std::vector<double> X(nn * D);  // node coordinates in ind(i,j,k) order, as saveGridData builds them
+    size_t q = 0;
+    for (uint32_t i = 0; i < gi.Nx; ++i)
+        for (uint32_t j = 0; j < gi.Ny; ++j)
+            for (uint32_t k = 0; k < (D == 3 ? gi.Nz : 1u); ++k, ++q) {
+                X[q * D] = ax[0][i]; X[q * D + 1] = ax[1][j];
+                if (D == 3) X[q * D + 2] = ax[2][k];
+            }
+    std::vector<std::string> names;
+    std::vector<const double*> cols;
+    std::vector<int> stride;
+    const char* an[3] = {"x", "y", "z"};
+    const char* avn[3] = {"vx", "vy", "vz"};
+    for (int d = 0; d < D; ++d) { names.push_back(an[d]); cols.push_back(X.data() + d); stride.push_back(D); }
+    for (int d = 0; d < D; ++d) { names.push_back(avn[d]); cols.push_back(v.data() + d); stride.push_back(D); }
+    names.push_back("mass"); cols.push_back(mass.data()); stride.push_back(1);
+    if (c->cfg.use_mibf) { names.push_back("friction"); cols.push_back(fric.data()); stride.push_back(1); }
+    return write_ply(path, nn, names, cols, stride);
+}
+
 int mpm_set_profiling(mpm_context* c, int enable) {
     if (!c) return MPM_ERR_INVALID_ARGUMENT;
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -921,6 +1051,34 @@ int mpm_batch_step(mpm_batch* b, uint32_t max_steps, uint32_t flags, mpm_batch_i
         info->chunks = chunks; info->status = status; info->particle_steps = particle_steps;
     }
     return (int)(status & MPM_ERR_EXIT_MASK);
+}
+
+int mpm_batch_autotune(mpm_batch* b, uint32_t max_steps, uint32_t* chosen, double* rate) {
+    if (!b) return MPM_ERR_INVALID_ARGUMENT;
+    const uint32_t cand[4] = {4, 8, 16, 32};
+    const uint32_t keep = b->opt.chunk_steps;
+    uint32_t best = keep, used = 0;
+    double best_rate = 0.0;
+    for (uint32_t k : cand) {
+        if (used + 3 * k > max_steps) break;
+        b->opt.chunk_steps = k;
+        mpm_batch_info bi{};
+        int rc = mpm_batch_step(b, k, 0, &bi);  // warm-up chunk: captures this length's graph
+        used += k;
+        if (rc) { b->opt.chunk_steps = keep; return rc; }
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = mpm_batch_step(b, 2 * k, 0, &bi);
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        used += 2 * k;
+        if (rc) { b->opt.chunk_steps = keep; return rc; }
+        if (bi.particle_steps == 0) break;  // budgets spent
+        const double r = (double)bi.particle_steps / sec;
+        if (r > best_rate) { best_rate = r; best = k; }
+    }
+    b->opt.chunk_steps = best;
+    if (chosen) *chosen = best;
+    if (rate) *rate = best_rate;
+    return 0;
 }
 
 int mpm_scene_info(const mpm_context* c, mpm_step_info* info, uint32_t* budget_left) {

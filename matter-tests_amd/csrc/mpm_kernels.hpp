@@ -17,12 +17,12 @@
 //                  written to a staging area (P2G + the stress force of explicitEulerUpdate)
 //   k_grid_update  per node: sum the tiles of the (at most 2^dim) blocks that cover it in block order,
 //                  normalise, explicit update, boundary conditions, flip
-//   USL:  k_g2p_usl       gather from an LDS copy of the block's tile; F update, return map, position and
-//                         velocity update (G2P + deformationUpdate + plasticity + positionUpdate)
-//   MUSL: k_g2p_musl      gather, velocity update, new position (G2P + positionUpdate)
-//         k_musl_scatter  re-scatter of the new momentum, like k_p2g (MUSL)
-//         k_musl_update   per node: sum the tiles, divide by m_i/m_p (MUSL tail)
-//         k_defgrad       gather grad v, F update, return map (deformationUpdate + plasticity)
+//   k_g2p          gather from an LDS copy of the block's tile: pic, flip, B, velocity and position
+//                  update (G2P + the velocity blend + positionUpdate)
+//   MUSL only: k_musl_scatter  re-scatter of the new momentum, like k_p2g (MUSL)
+//              k_musl_update   per node: sum the tiles, divide by m_i/m_p (MUSL tail)
+//   k_defgrad      gather grad v (USL: the updated grid velocity; MUSL: the re-scattered one), F update,
+//                  return map, Kirchhoff stress for the next step (deformationUpdate + plasticity)
 //   k_finish       1 wavefront: moveObjects, time/frame bookkeeping of simulate()
 //
 // No floating-point atomics are used: every sum has a fixed order given the particle state, so a step is
@@ -1452,67 +1452,12 @@ __device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& 
 }
 
 // ----------------------------------------------------------------------------------------------
-// k_g2p_usl (USL): G2P (g2p.cpp:17-77), deformationUpdate + plasticity, positionUpdate.
-// ----------------------------------------------------------------------------------------------
-template <int D, int SPL, class TF, int MAT>
-__device__ __forceinline__ void g2p_usl_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
-                                             const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b) {
-    using G = BG<D, SPL>;
-    __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
-    __shared__ BlockCoordLDS<D, SPL> s_c;
-    BlockRange br;
-    if (!block_range<D>(S, sb, b, br)) return;
-    const double h1 = P.one_over_dx, dt = S->dt;
-    const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
-    const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
-    const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
-    load_tile<D, SPL, TF>(s_v, g.v, D, br.bc, nn);
-    if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, D, br.bc, nn);
-    s_c.load(g.coord, br.bc, nn);
-    __syncthreads();
-    uint32_t err = 0, pcount = 0;
-    BoundsAcc ba;
-#pragma unroll 1
-    for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) {
-        const uint32_t i = br.start + k, p = sb.perm[i];
-        double xp[3] = {0, 0, 0};
-        for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
-        Stencil<D, SPL> st;
-        int tl[3];
-        if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, s_c.view(br.bc), h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
-        TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gvt[3][3] = {{0}};
-        gather<D, SPL, true, true, TF>(st, xp, h1, TileFetch<D, SPL, TF>{s_v, s_f, tl[0], tl[1], tl[2]}, apic, useflip, pic, fl, Bp, gvt);
-        double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0}, gv[3][3];
-        for (int d = 0; d < D; ++d) {
-            const double vo = useflip ? (double)src.v[d][p] : 0.0;
-            vnew[d] = (double)(TF)blend_velocity(P.flip_ratio, vo, (double)fl[d], (double)pic[d]);
-        }
-        for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { dst.B[r + c * D][i] = apic ? Bp[r][c] : TF(0); gv[r][c] = (double)gvt[r][c]; }
-        deform_and_project<D, TF, MAT>(P, dt, src, p, dst, i, gv, err, pcount);
-        for (int d = 0; d < D; ++d) xnew[d] = xp[d] + dt * (double)pic[d];
-        if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
-        for (int d = 0; d < D; ++d) { dst.x[d][i] = xnew[d]; dst.v[d][i] = (TF)vnew[d]; }
-        ba.add<D>(xnew, vnew);
-    }
-    reduce_bounds<D>(S, ba);
-    pcount = wave_sum_u32(pcount);
-    err = wave_or_u32(err);
-    if ((threadIdx.x & 63) == 0) {
-        add_plastic_count(S, pcount);
-        if (err) atomicOr(&S->err, err);
-    }
-}
-template <int D, int SPL, class TF, int MAT>
-__global__ __launch_bounds__(kBlock) void k_g2p_usl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
-    g2p_usl_body<D, SPL, TF, MAT>(P, S, src, dst, sb, g, blockIdx.x);
-}
-
-// ----------------------------------------------------------------------------------------------
-// k_g2p_musl (MUSL): G2P (g2p.cpp:17-77), the MUSL velocity update (musl.cpp:13-25) and the position
-// update of positionUpdate (position_update.cpp:12).
+// k_g2p: G2P (g2p.cpp:17-77), the particle velocity update (USL: position_update.cpp:14-23; MUSL: the
+// same formula in musl.cpp:13-25) and the position update (position_update.cpp:12). The deformation
+// update, which reads the particles' positions of the step's start, follows in k_defgrad.
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, class TF>
-__device__ __forceinline__ void g2p_musl_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+__device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                               const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b) {
     using G = BG<D, SPL>;
     __shared__ TF s_v[3 * G::NT], s_f[3 * G::NT];
@@ -1555,8 +1500,8 @@ __device__ __forceinline__ void g2p_musl_body(const Params& P, StepState* S, con
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
 template <int D, int SPL, class TF>
-__global__ __launch_bounds__(kBlock) void k_g2p_musl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
-    g2p_musl_body<D, SPL, TF>(P, S, src, dst, sb, g, blockIdx.x);
+__global__ __launch_bounds__(kBlock) void k_g2p(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
+    g2p_body<D, SPL, TF>(P, S, src, dst, sb, g, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1598,6 +1543,37 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
 template <int D, int SPL, class TF, int MAT>
 __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
     defgrad_body<D, SPL, TF, MAT>(P, S, src, dst, sb, g, n, blockIdx.x);
+}
+
+// ----------------------------------------------------------------------------------------------
+// k_stress_measures: the per-particle output quantities of saveParticleData (save_data.cpp:9-38):
+// tau = P(F) F^T (HenckyPiola without the strain clamp of the return map, or NeoHookeanPiola),
+// pressure p = -tr(tau)/dim, deviatoric measure q = sqrt(c tau_dev : tau_dev) with c = 3/2 (use_mises_q)
+// or 1/2, Je = det F. Written at the caller's particle index.
+// ----------------------------------------------------------------------------------------------
+template <int D, class TF>
+__global__ __launch_bounds__(kBlock) void k_stress_measures(Params P, PartStateT<TF> ps, uint32_t n, double* pr, double* qd, double* je) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    M<D> F;
+    for (int k = 0; k < D * D; ++k) F.m[k] = ps.F[k][i];
+    double tau[6];
+    if (P.elastic_model == MPM_ELASTIC_NEOHOOKEAN) {
+        neohookean_tau<D>(F, P.mu, P.lambda, tau);
+    } else {
+        M<D> U, W; V<D> s;
+        svd<D>(F, U, s, W, P.svd_tol);
+        V<D> ls; for (int k = 0; k < D; ++k) ls[k] = log_d(fabs(s[k]));
+        hencky_tau<D>(U, ls, P.mu, P.lambda, tau);
+    }
+    double tr = tau[0] + tau[1] + (D == 3 ? tau[2] : 0.0);
+    const double p = -tr / D;
+    double dd = 0.0;
+    for (int k = 0; k < D; ++k) { const double t = tau[k] + p; dd += t * t; }
+    for (int k = D; k < SymPack<D>::n; ++k) dd += 2.0 * tau[k] * tau[k];  // off-diagonal entries appear twice
+    const double q = P.use_mises_q ? sqrt(1.5 * dd) : sqrt(0.5 * dd);
+    const uint32_t id = ps.id[i];
+    pr[id] = p; qd[id] = q; je[id] = det(F);
 }
 
 // ----------------------------------------------------------------------------------------------

@@ -451,6 +451,13 @@ MPM_HD double perzyna_newton(const Params& P, double dt, double q_trial, double 
     return dg;
 }
 
+// Compile-time material variants beyond the plastic-model values: the Perzyna laws whose parameters select
+// the closed-form rate update (plasticity.cpp:188-194 for VMVisc: exponent 1 and no softening;
+// :284-290 for DPVisc: exponent 1) carry only that branch, not the Newton iteration and its registers.
+// The arithmetic is the same as the generic variant's closed form.
+constexpr int kMatVMViscLinear = 17;
+constexpr int kMatDPViscLinear = 18;
+
 // plasticity(), plasticity.cpp:17-621. `h` holds the clamped principal Hencky strains
 // ln(max(|s|,1e-4)) of F_trial on entry and the projected strains on exit when recompose is set.
 // For VMVisc's capped branch the result is F = U W^T, i.e. h = 0. PM >= 0 fixes the plastic model at
@@ -468,7 +475,9 @@ MPM_HD RMResult return_map(const Params& P, double dt, V<D>& h, PlasticState& st
     auto set_iso = [&](double pp) { double e = -pp / (K * dim); for (int i = 0; i < D; ++i) h[i] = e * 1.0; r.recompose = true; };
     auto shift_dev = [&](double amount) { for (int i = 0; i < D; ++i) h[i] -= amount * dev[i]; r.recompose = true; };
 
-    switch (PM >= 0 ? PM : P.plastic_model) {
+    constexpr bool kLin = PM == kMatVMViscLinear || PM == kMatDPViscLinear;
+    constexpr int PMB = PM == kMatVMViscLinear ? MPM_PLASTIC_VMVISC : (PM == kMatDPViscLinear ? MPM_PLASTIC_DPVISC : PM);
+    switch (PMB >= 0 ? PMB : P.plastic_model) {
     case MPM_PLASTIC_VM: {  // :30-49
         double dg = dn - P.q_max / P.e_mu_prefac;
         if (dg > 0) {
@@ -558,11 +567,11 @@ MPM_HD RMResult return_map(const Params& P, double dt, V<D>& h, PlasticState& st
         } else if (q_trial > q_yield) {
             r.plastic_incr++;
             double dg;
-            if (P.perzyna_exp == 1 && P.xi == 0) {
+            if (kLin || (P.perzyna_exp == 1 && P.xi == 0)) {
                 dg = (q_trial - q_yield) / (P.f_mu_prefac + q_yield * P.perzyna_visc / dt);
                 if (dg < 0) r.err |= MPM_ERR_PERZYNA_NEGATIVE;
             } else {
-                dg = perzyna_newton(P, dt, q_trial, q_yield, st.eps_dev, true, r.err);
+                if constexpr (!kLin) dg = perzyna_newton(P, dt, q_trial, q_yield, st.eps_dev, true, r.err);
             }
             shift_dev(inv_d * dg);
             st.eps_dev += inv_d * dg;
@@ -593,11 +602,11 @@ MPM_HD RMResult return_map(const Params& P, double dt, V<D>& h, PlasticState& st
         if (ps > p_tip && q_trial > q_yield) {
             r.plastic_incr++;
             double dg;
-            if (P.perzyna_exp == 1) {
+            if (kLin || P.perzyna_exp == 1) {
                 dg = (q_trial - q_yield) / (P.f_mu_prefac + q_yield * P.perzyna_visc / dt);
                 if (dg < 0) r.err |= MPM_ERR_PERZYNA_NEGATIVE;
             } else {
-                dg = perzyna_newton(P, dt, q_trial, q_yield, st.eps_dev, false, r.err);
+                if constexpr (!kLin) dg = perzyna_newton(P, dt, q_trial, q_yield, st.eps_dev, false, r.err);
             }
             if (P.use_mibf) {
                 if (fabs(ps) < 1e-10) st.muI = 1e10;

@@ -31,6 +31,8 @@ void with_material_id(int mat, Fn&& f) {
     case MPM_PLASTIC_MCCVISC: f(std::integral_constant<int, MPM_PLASTIC_MCCVISC>()); break;
     case MPM_PLASTIC_DPMUI: f(std::integral_constant<int, MPM_PLASTIC_DPMUI>()); break;
     case MPM_PLASTIC_MCCMUI: f(std::integral_constant<int, MPM_PLASTIC_MCCMUI>()); break;
+    case kMatVMViscLinear: f(std::integral_constant<int, kMatVMViscLinear>()); break;
+    case kMatDPViscLinear: f(std::integral_constant<int, kMatDPViscLinear>()); break;
     default: f(std::integral_constant<int, MPM_PLASTIC_NONE>()); break;
     }
 }
@@ -61,19 +63,15 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     else if (sep) enqueue_p2g<D, SPL, true, false, TF>(c, src, dst, gblocks);
     else if (P.use_mibf) enqueue_p2g<D, SPL, false, true, TF>(c, src, dst, gblocks);
     else enqueue_p2g<D, SPL, false, false, TF>(c, src, dst, gblocks);
+    { LaunchTimer t(c, KS_G2P); k_g2p<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g); }
     if (P.use_musl) {
         constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
-        {
-            LaunchTimer t(c, KS_G2P_MUSL);
-            k_g2p_musl<D, SPL, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
-        }
         { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->cap_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
         { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL, TF><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, g, stg); }
+    }
+    {
         LaunchTimer t(c, KS_DEFGRAD);
         with_material(P, [&](auto mat) { k_defgrad<D, SPL, TF, decltype(mat)::value><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g, n); });
-    } else {
-        LaunchTimer t(c, KS_G2P_USL);
-        with_material(P, [&](auto mat) { k_g2p_usl<D, SPL, TF, decltype(mat)::value><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g); });
     }
     { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
 }
@@ -98,15 +96,13 @@ void enqueue_batch_step(const BatchClass& bc, hipStream_t s) {
     else if (k.sep) enqueue_batch_p2g<D, SPL, true, false, TF>(bc, s);
     else if (k.mibf) enqueue_batch_p2g<D, SPL, false, true, TF>(bc, s);
     else enqueue_batch_p2g<D, SPL, false, false, TF>(bc, s);
+    kb_g2p<D, SPL, TF><<<bc.wg[BR_BLOCK], kBlock, 0, s>>>(T);
     if (k.musl) {
         constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
-        kb_g2p_musl<D, SPL, TF><<<bc.wg[BR_BLOCK], kBlock, 0, s>>>(T);
         kb_musl_scatter<D, SPL, TF><<<bc.wg[BR_BLOCK], TBM, 0, s>>>(T);
         kb_musl_update<D, SPL, TF><<<bc.wg[BR_GRID], kBlock, 0, s>>>(T);
-        with_material_id(k.mat, [&](auto mat) { kb_defgrad<D, SPL, TF, decltype(mat)::value><<<bc.wg[BR_PART], kBlock, 0, s>>>(T); });
-    } else {
-        with_material_id(k.mat, [&](auto mat) { kb_g2p_usl<D, SPL, TF, decltype(mat)::value><<<bc.wg[BR_BLOCK], kBlock, 0, s>>>(T); });
     }
+    with_material_id(k.mat, [&](auto mat) { kb_defgrad<D, SPL, TF, decltype(mat)::value><<<bc.wg[BR_PART], kBlock, 0, s>>>(T); });
     kb_finish<D, TF><<<T.ns, 64, 0, s>>>(T);
 }
 

@@ -11,7 +11,10 @@
 #pragma once
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
+#include <fstream>
+#include <sys/stat.h>
 #include <cstdint>
 #include <cstring>
 #include <iostream>
@@ -203,9 +206,14 @@ public:
     std::vector<std::unique_ptr<ObjectPlate>> plates;
     std::vector<std::unique_ptr<ObjectGeneral>> objects;
 
-    // simulation.cpp:24-46 (no output directory is created: particle/grid files are not written)
+    // simulation.cpp:24-46 and createDirectory (simulation.cpp:49-71): the output folders are created when
+    // saving; the copy of the set-up source that the reference keeps as provenance is not made.
     void initialize(bool save = true, std::string dir = "output/", std::string name = "dummy") {
         save_sim = save; directory = dir; sim_name = name; is_initialized = true;
+        if (save_sim) {
+            if (mkdir(directory.c_str(), 0777) == -1) std::cerr << "Directory " << directory << " has already been created" << std::endl;
+            if (mkdir((directory + sim_name).c_str(), 0777) == -1) std::cerr << "Simulation " << sim_name << " has already been created" << std::endl;
+        }
     }
 
     // simulation.cpp:73-213: the frame loop, one device batch per frame; particles are copied back to
@@ -216,15 +224,70 @@ public:
             std::cout << "This plastic model is only compatible with Hencky's elasticity model" << std::endl; return;
         }
         if (!start()) return;
+        frame = 0;
+        if (save_sim) {  // simulation.cpp:160-163
+            saveInfo();
+            saveParticleData();
+            mpm_set_profiling(ctx, 1);
+        }
+        const auto begin = std::chrono::steady_clock::now();
         while (true) {
             mpm_step_info info;
             const int rc = mpm_step(ctx, 1u << 30, MPM_STEP_STOP_AT_FRAME_END, &info);
             absorb(info);
             if (rc) { exit = 1; report(rc); break; }
-            if (info.frame_ended && !reduce_verbose) std::cout << "End of frame " << info.frame << std::endl;
-            if (info.finished || info.steps_done == 0) break;
+            if (info.frame_ended) {  // simulation.cpp:180-190
+                if (!reduce_verbose) std::cout << "End of frame " << info.frame << std::endl;
+                if (save_sim) { saveParticleData(); if (save_grid) saveGridData(); }
+            }
+            if (info.finished) {  // simulation.cpp:191-199: the last frame is written again
+                if (save_sim) { saveParticleData(); if (save_grid) saveGridData(); }
+                break;
+            }
+            if (info.steps_done == 0) break;
         }
+        runtime_total = (T)std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - begin).count();
         download();
+        if (save_sim) saveTiming();
+    }
+
+    // Output files of save_data.cpp, written by the engine (binary little-endian PLY, float64 properties).
+    // saveParticleData, save_data.cpp:7-171: particles_f<extra><frame>.ply and last_written.txt.
+    void saveParticleData(std::string extra = "") {
+        const std::string f = directory + sim_name + "/particles_f" + extra + std::to_string(frame) + ".ply";
+        if (mpm_write_particles_ply(ctx, f.c_str())) { std::cerr << "Could not write " << f << std::endl; exit = 1; return; }
+        std::ofstream out(directory + sim_name + "/last_written.txt");
+        out << std::to_string(frame) << "\n";
+    }
+    // saveGridData, save_data.cpp:173-259: grid_f<extra><frame>.ply.
+    void saveGridData(std::string extra = "") {
+        const std::string f = directory + sim_name + "/grid_f" + extra + std::to_string(frame) + ".ply";
+        if (mpm_write_grid_ply(ctx, f.c_str())) { std::cerr << "Could not write " << f << std::endl; exit = 1; }
+    }
+    // saveInfo, save_data.cpp:352-361.
+    void saveInfo() const {
+        std::ofstream info(directory + sim_name + "/info.txt");
+        info << end_frame << "\n" << fps << "\n" << dx << "\n" << Np << "\n" << particle_volume << "\n";
+    }
+    // saveTiming, save_data.cpp:363-373. The phase timers are the engine's per-kernel device times, grouped
+    // as the reference's phases: p2g (sort + P2G with the stress force), euler (grid update), g2p (G2P, MUSL
+    // re-scatter), defgrad (deformation update and return map; in the fused USL kernel it counts with g2p).
+    void saveTiming() {
+        mpm_kernel_timing kt[16];
+        uint32_t cnt = 0;
+        T p2g = 0, euler = 0, g2p = 0, defgrad = 0;
+        if (ctx && mpm_get_kernel_times(ctx, kt, 16, &cnt) == 0) {
+            for (uint32_t i = 0; i < cnt && i < 16; ++i) {
+                const std::string k = kt[i].name;
+                const T sec = kt[i].total_ms * 1e-3;
+                if (k == "k_p2g" || k.rfind("k_bin", 0) == 0 || k == "k_zero_grid") p2g += sec;
+                else if (k == "k_grid_update") euler += sec;
+                else if (k == "k_defgrad") defgrad += sec;
+                else if (k.rfind("k_g2p", 0) == 0 || k.rfind("k_musl", 0) == 0) g2p += sec;
+            }
+        }
+        std::ofstream out(directory + sim_name + "/info_timing.txt");
+        out << step_ << "\n" << runtime_total / ((T)1000) << "\n" << p2g << "\n" << g2p << "\n" << euler << "\n" << defgrad << "\n";
     }
 
     // simulation.cpp:219-273 plus the loop bookkeeping of simulate() (time += dt, step++, frame);
@@ -254,6 +317,8 @@ public:
 private:
     mpm_context* ctx = nullptr;
     std::string directory, sim_name;
+    unsigned int frame = 0;
+    T runtime_total = 0;
     T time_ = 0, dt_ = 0;
     unsigned long long step_ = 0;
 
@@ -301,7 +366,7 @@ private:
         mpm_particle_arrays a = arrays();
         if (mpm_get_particles(ctx, &a)) exit = 1;
     }
-    void absorb(const mpm_step_info& info) { time_ = info.time; step_ = info.step; dt_ = info.dt; }
+    void absorb(const mpm_step_info& info) { time_ = info.time; step_ = info.step; dt_ = info.dt; frame = info.frame; }
     void report(int rc) const { std::cout << "Simulation stopped: " << mpm_status_string((uint32_t)rc) << std::endl; }
 };
 

@@ -224,6 +224,13 @@ class Library:
                 f = getattr(L, name)
                 f.argtypes = args
                 f.restype = C.c_int
+            L.mpm_get_stress_measures.argtypes = [vp] + [C.POINTER(C.c_double)] * 3
+            L.mpm_get_stress_measures.restype = C.c_int
+            for name in ("mpm_write_particles_ply", "mpm_write_grid_ply"):
+                getattr(L, name).argtypes = [vp, C.c_char_p]
+                getattr(L, name).restype = C.c_int
+            L.mpm_batch_autotune.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
+            L.mpm_batch_autotune.restype = C.c_int
             L.mpm_device_bytes.argtypes = [vp]
             L.mpm_device_bytes.restype = C.c_uint64
             L.mpm_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(vp)]
@@ -452,6 +459,13 @@ class Batch:
         rc = self.L.lib.mpm_batch_set_budget(self.h, eng.h, nsteps)
         if rc:
             raise MPMError(rc, "mpm_batch_set_budget")
+
+    def autotune(self, max_steps):
+        chosen, rate = C.c_uint32(), C.c_double()
+        rc = self.L.lib.mpm_batch_autotune(self.h, max_steps, C.byref(chosen), C.byref(rate))
+        if rc:
+            raise MPMError(rc, "mpm_batch_autotune")
+        return chosen.value, rate.value
 
     def step(self, max_steps, flags=0):
         info = BatchInfo()

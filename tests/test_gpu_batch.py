@@ -170,3 +170,28 @@ def test_batch_grid_growth(engine):
     assert rc == 0
     assert_same(e, ref, "thrown scene")
     batch.close()
+
+
+def test_autotune_keeps_results(engine):
+    """The chunk autotuner advances the members by ordinary steps: afterwards every member equals the scene
+    advanced alone by the same number of steps, and the chosen length is one of the candidates."""
+    print("batch autotune", flush=True)
+    scenes = mixed_scenes()[:4]
+    batch = A.Batch(engine, chunk_steps=16)
+    members = []
+    for name, cfg, plates, objs, f in scenes:
+        e = make(engine, cfg, plates, objs, f)
+        batch.add(e, 200)
+        members.append(e)
+    chosen, rate = batch.autotune(180)
+    assert chosen in (4, 8, 16, 32) and rate > 0
+    rc, _ = batch.step(200)  # finish the budgets
+    assert rc == 0
+    for (name, cfg, plates, objs, f), m in zip(scenes, members):
+        info, left = A.Batch.scene_info(m)
+        assert left == 0
+        ref = make(engine, cfg, plates, objs, f)
+        rc, _ = ref.step(200)
+        assert rc == 0
+        assert_same(m, ref, f"autotuned {name}")
+    batch.close()
