@@ -124,10 +124,10 @@ __global__ __launch_bounds__(kBlock) void kb_grid_update(BatchTable T) {
     grid_update_body<D, SPL, SEP, MIBF, TF>(q.P(), q.r->S, q.r->sb, q.r->g, q.r->stg, q.local, q.count);
 }
 
-template <int D, int SPL, class TF>
+template <int D, int SPL, bool FLIP, class TF>
 __global__ __launch_bounds__(kBlock) void kb_g2p(BatchTable T) {
     BatchSlot<TF> q(T, BR_BLOCK);
-    g2p_body<D, SPL, TF>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.local);
+    g2p_body<D, SPL, FLIP, TF>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.local);
 }
 
 template <int D, int SPL, class TF>

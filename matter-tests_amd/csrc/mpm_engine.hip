@@ -309,7 +309,8 @@ static_assert(kHeadBytes % 4 == 0, "StepState head must be whole words");
 BatchClassKey class_key(const mpm_context* c, bool check_cons) {
     BatchClassKey k{};
     k.D = c->cfg.dim; k.SPL = c->cfg.spline_degree; k.split = c->split ? 1 : 0;
-    k.sep = (c->P.flip_ratio >= -1 || check_cons) ? 1 : 0; k.mibf = c->P.use_mibf ? 1 : 0; k.musl = c->P.use_musl ? 1 : 0;
+    k.sep = (c->P.flip_ratio >= -1 || check_cons) ? 1 : 0; k.flip = c->P.flip_ratio >= -1 ? 1 : 0;
+    k.mibf = c->P.use_mibf ? 1 : 0; k.musl = c->P.use_musl ? 1 : 0;
     k.mat = material_of(c->P);
     return k;
 }

@@ -976,6 +976,29 @@ struct P2GContrib {
         TF Bc[3][3];  // D^-1 B[:,k]
         for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) Bc[r][k] = q.B[r + k * D] * aDt;
         const TF muI = (TF)q.muI;
+        // Factored form, used when no node of this wavefront's part of the window can fall under the
+        // reference's 1e-25 weight cut (smallest per-axis weights multiplied, with a factor 10 of margin for
+        // rounding): with w = w0 w1 w2 and grad w = h1 (dw0 w1 w2, w0 dw1 w2, w0 w1 dw2), every value is
+        // w1 w2 A0[a] + w0 w2 C1[b] + w0 w1 C2[c], where A0 carries the velocity, the first-axis affine term
+        // and the first-axis force term, C1 and C2 the other axes' affine and force terms. About 10
+        // operations per node instead of about 24.
+        TF mw0 = TF(1), mw1 = ss.w1[0], mw2 = TF(1);
+#pragma unroll
+        for (int s = 0; s < SS::SPP; ++s) if (a0 + s < W) mw0 = s == 0 ? ss.w0[0] : fmin(mw0, ss.w0[s]);
+#pragma unroll
+        for (int o = 1; o < SS::OBN; ++o) mw1 = fmin(mw1, ss.w1[o]);
+        if (D == 3) {
+            mw2 = ss.w2[0];
+#pragma unroll
+            for (int o = 1; o < W; ++o) mw2 = fmin(mw2, ss.w2[o]);
+        }
+        // Split form only: the plain form keeps the reference's per-node products, so that its sums round as
+        // the reference's do.
+        constexpr bool kFactored = sizeof(TF) == 4;
+        if (kFactored && MPM_SEL(mw0 * mw1 * mw2 > TF(1e-24))) {
+            if constexpr (kFactored) add_factored(ss, q, a0, T, Bc, muI, acc);
+            return ok;
+        }
 #pragma unroll
         for (int s = 0; s < SS::SPP; ++s) {
             if (a0 + s >= W) break;
@@ -1021,6 +1044,66 @@ struct P2GContrib {
             }
         }
         return ok;
+    }
+    // The factored form of add (see there); every node of the wavefront's part of the window has w > 1e-25.
+    template <class SSt>
+    __device__ __forceinline__ void add_factored(const SSt& ss, const Part& q, int a0, const TF (&T)[3][3], const TF (&Bc)[3][3],
+                                                 TF muI, TF (&acc)[SS::SPP][SS::NSH][NV]) const {
+        constexpr int W = BG<D, SPL>::W;
+        constexpr int NC = SEP ? 2 : 1;  // momentum and force separately (SEP), or summed
+        // last-axis terms C2[c] (3D), per node of the axis
+        TF C2[D == 3 ? W : 1][NC][3];
+        if (D == 3) {
+#pragma unroll
+            for (int o = 0; o < W; ++o)
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    const TF m = ss.w2[o] * (Bc[r][2] * ss.pd2[o]), f = T[r][2] * ss.dw2[o];
+                    if (SEP) { C2[o][0][r] = m; C2[o][1][r] = f; } else C2[o][0][r] = m + f;
+                }
+        }
+#pragma unroll
+        for (int s = 0; s < SS::SPP; ++s) {
+            if (a0 + s >= W) break;
+            TF A0[NC][3];
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                const TF m = ss.w0[s] * madd(Bc[r][0], ss.pd0[s], q.v[r]), f = T[r][0] * ss.dw0[s];
+                if (SEP) { A0[0][r] = m; A0[1][r] = f; } else A0[0][r] = m + f;
+            }
+#pragma unroll
+            for (int o1 = 0; o1 < SS::OBN; ++o1) {
+                TF C1[NC][3];
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    const TF m = ss.w1[o1] * (Bc[r][1] * ss.pd1[o1]), f = T[r][1] * ss.dw1[o1];
+                    if (SEP) { C1[0][r] = m; C1[1][r] = f; } else C1[0][r] = m + f;
+                }
+                const TF w01 = ss.w0[s] * ss.w1[o1];
+                if (D == 3) {
+#pragma unroll
+                    for (int o2 = 0; o2 < W; ++o2) {
+                        const TF w12 = ss.w1[o1] * ss.w2[o2], w02 = ss.w0[s] * ss.w2[o2];
+                        TF* a = acc[s][o1 * W + o2];
+                        a[0] += w01 * ss.w2[o2];
+#pragma unroll
+                        for (int c = 0; c < NC; ++c)
+#pragma unroll
+                            for (int r = 0; r < D; ++r)
+                                a[1 + c * D + r] = madd(w12, A0[c][r], madd(w02, C1[c][r], madd(w01, C2[o2][c][r], a[1 + c * D + r])));
+                        if (MIBF) a[NV - 1] = madd(w01 * ss.w2[o2], muI, a[NV - 1]);
+                    }
+                } else {
+                    TF* a = acc[s][o1];
+                    a[0] += w01;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+#pragma unroll
+                        for (int r = 0; r < D; ++r) a[1 + c * D + r] = madd(ss.w1[o1], A0[c][r], madd(ss.w0[s], C1[c][r], a[1 + c * D + r]));
+                    if (MIBF) a[NV - 1] = madd(w01, muI, a[NV - 1]);
+                }
+            }
+        }
     }
 };
 
@@ -1394,11 +1477,95 @@ template <int D, class TF> struct GridFetch {
     __device__ __forceinline__ TF flip(int, int, int, int) const { return TF(0); }
 };
 
-// Gather of G2P (g2p.cpp:35-67) and of the velocity gradient (deformation_update.cpp:217-230) over the
-// non-zero nodes of the window, in the reference's (i,j,k) order.
+// Gather of G2P (g2p.cpp:35-67) or of the velocity gradient (deformation_update.cpp:217-230) over the
+// non-zero nodes of the window, factored by rows. The weight of node (a, b, c) is w0[a] w1[b] w2[c] and
+// its gradient h1 (dw0[a] w1[b] w2[c], w0[a] dw1[b] w2[c], w0[a] w1[b] dw2[c]). Along each row (a, b) of
+// the window the node values are summed first with the last axis's factors, S = sum_c w2[c] v and
+// Sd = sum_c f[c] v (f = w2 (x_2c - x_p,2) for B, dw2 for grad v), and the row then adds
+//   pic += w0 w1 S,  B[:,0] += w0 (x_0a - x_p,0) w1 S,  B[:,1] += w0 w1 (x_1b - x_p,1) S,  B[:,2] += w0 w1 Sd,
+//   grad v[:,0] += h1 dw0 w1 S,  grad v[:,1] += h1 w0 dw1 S,  grad v[:,2] += h1 w0 w1 Sd.
+// The sums are those of the reference's (i,j,k) loops, grouped by row; they agree with the loop order to
+// rounding. About 9 multiply-adds per node instead of about 30.
+// FLIP (compile time, so that kernels without flip carry no flip loads) adds the flip sum.
+// Used by the split form: the plain form keeps the reference's node order (gather_nodes), so that its
+// sums round as the reference's do and no particle at a yield threshold takes the other branch.
+template <int D, int SPL, bool PIC, bool GRAD, bool FLIP, class TF, class Fetch>
+__device__ __forceinline__ void gather_rows(const Stencil<D, SPL>& st, const double* xp, double h1, const Fetch& fetch, bool apic,
+                                            TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
+    static_assert(PIC != GRAD, "one gather per kind");
+    constexpr int C = Stencil<D, SPL>::cnt;
+    constexpr int C2 = D == 3 ? C : 1;
+    TF w[3][C], f[3][C];  // weights; per axis the second factor: w (x_i - x_p) for B, h1 dw for grad v
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            w[d][k] = d < D ? (TF)st.w[d][k] : TF(1);
+            if (d >= D) f[d][k] = TF(0);
+            else if (PIC) f[d][k] = w[d][k] * (TF)(st.xi[d][k] - xp[d]);
+            else f[d][k] = (TF)(st.dw[d][k] * h1);
+        }
+    const bool second = GRAD || apic;
+#pragma unroll
+    for (int a = 0; a < C; ++a) {
+#pragma unroll
+        for (int b = 0; b < C; ++b) {
+            TF S[3] = {0, 0, 0}, Sd[3] = {0, 0, 0}, Sf[3] = {0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < C2; ++c) {
+                TF vi[3];
+                fetch.get(a, b, c, vi);
+                if (D == 3) {
+#pragma unroll
+                    for (int r = 0; r < D; ++r) {
+                        S[r] = madd(w[2][c], vi[r], S[r]);
+                        if (second) Sd[r] = madd(f[2][c], vi[r], Sd[r]);
+                        if (PIC && FLIP) Sf[r] = madd(w[2][c], fetch.flip(a, b, c, r), Sf[r]);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < D; ++r) {
+                        S[r] = vi[r];
+                        if (PIC && FLIP) Sf[r] = fetch.flip(a, b, c, r);
+                    }
+                }
+            }
+            const TF w01 = w[0][a] * w[1][b];
+            // rows are scheduled one after another: hoisting every row's loads ahead of the arithmetic
+            // would keep all window values live at once
+            __builtin_amdgcn_sched_barrier(0);
+            if (PIC) {
+                const TF b0 = f[0][a] * w[1][b], b1 = w[0][a] * f[1][b];
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    pic[r] = madd(w01, S[r], pic[r]);
+                    if (FLIP) fl[r] = madd(w01, Sf[r], fl[r]);
+                    if (apic) {
+                        Bp[r][0] = madd(b0, S[r], Bp[r][0]);
+                        Bp[r][1] = madd(b1, S[r], Bp[r][1]);
+                        if (D == 3) Bp[r][2] = madd(w01, Sd[r], Bp[r][2]);
+                    }
+                }
+            } else {
+                const TF g0 = f[0][a] * w[1][b], g1 = w[0][a] * f[1][b];
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    gv[r][0] = madd(g0, S[r], gv[r][0]);
+                    gv[r][1] = madd(g1, S[r], gv[r][1]);
+                    if (D == 3) gv[r][2] = madd(w01, Sd[r], gv[r][2]);
+                }
+            }
+        }
+    }
+}
+
+// The same gather node by node in the reference's (i,j,k) loop order, with its products and sums
+// (g2p.cpp:35-67, deformation_update.cpp:27-40): the plain form's.
+// The flip sum is switched by a run-time flag here: with a compile-time one the compiler keeps every node's
+// flip values live at once in binary64 (the kernel then runs at one wavefront per SIMD).
 template <int D, int SPL, bool PIC, bool GRAD, class TF, class Fetch>
-__device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* xp, double h1, const Fetch& fetch, bool apic,
-                                       bool useflip, TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
+__device__ __forceinline__ void gather_nodes(const Stencil<D, SPL>& st, const double* xp, double h1, const Fetch& fetch, bool apic,
+                                             bool useflip, TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
     const int c2 = (D == 3) ? Stencil<D, SPL>::cnt : 1;
 #pragma unroll
     for (int a = 0; a < Stencil<D, SPL>::cnt; ++a) {
@@ -1437,6 +1604,14 @@ __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* 
     }
 }
 
+// useflip must equal FLIP; the plain form reads the run-time value.
+template <int D, int SPL, bool PIC, bool GRAD, bool FLIP, class TF, class Fetch>
+__device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* xp, double h1, const Fetch& fetch, bool apic,
+                                       bool useflip, TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
+    if constexpr (sizeof(TF) == 4) gather_rows<D, SPL, PIC, GRAD, FLIP, TF>(st, xp, h1, fetch, apic, pic, fl, Bp, gv);
+    else gather_nodes<D, SPL, PIC, GRAD, TF>(st, xp, h1, fetch, apic, useflip, pic, fl, Bp, gv);
+}
+
 // Block prologue shared by the gather kernels: returns false when the block has nothing to do.
 struct BlockRange { uint32_t b, start, cnt; int bc[3]; };
 template <int D>
@@ -1456,7 +1631,7 @@ __device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& 
 // same formula in musl.cpp:13-25) and the position update (position_update.cpp:12). The deformation
 // update, which reads the particles' positions of the step's start, follows in k_defgrad.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL, class TF>
+template <int D, int SPL, bool FLIP, class TF>
 __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                               const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b) {
     using G = BG<D, SPL>;
@@ -1467,7 +1642,9 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
     const double h1 = P.one_over_dx, dt = S->dt;
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
-    const bool apic = P.flip_ratio < 0, useflip = P.flip_ratio >= -1;
+    const bool apic = P.flip_ratio < 0;
+    constexpr bool useflip = FLIP;  // flip_ratio >= -1
+    const bool flip_rt = P.flip_ratio >= -1;
     load_tile<D, SPL, TF>(s_v, g.v, D, br.bc, nn);
     if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, D, br.bc, nn);
     s_c.load(g.coord, br.bc, nn);
@@ -1483,7 +1660,7 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
         int tl[3];
         if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, s_c.view(br.bc), h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
         TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3];
-        gather<D, SPL, true, false, TF>(st, xp, h1, TileFetch<D, SPL, TF>{s_v, s_f, tl[0], tl[1], tl[2]}, apic, useflip, pic, fl, Bp, gv);
+        gather<D, SPL, true, false, FLIP, TF>(st, xp, h1, TileFetch<D, SPL, TF>{s_v, s_f, tl[0], tl[1], tl[2]}, apic, flip_rt, pic, fl, Bp, gv);
         double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) {
             const double vo = useflip ? (double)src.v[d][p] : 0.0;
@@ -1499,9 +1676,9 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
     err = wave_or_u32(err);
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
-template <int D, int SPL, class TF>
+template <int D, int SPL, bool FLIP, class TF>
 __global__ __launch_bounds__(kBlock) void k_g2p(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
-    g2p_body<D, SPL, TF>(P, S, src, dst, sb, g, blockIdx.x);
+    g2p_body<D, SPL, FLIP, TF>(P, S, src, dst, sb, g, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1527,7 +1704,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
             err |= MPM_ERR_GRID_BOUNDS;
         } else {
             TF gvt[3][3] = {{0}}, pic[3], fl[3], Bp[3][3];
-            gather<D, SPL, false, true, TF>(st, xp, h1, GridFetch<D, TF>{g, nn, st.base[0], st.base[1], st.base[2]}, false, false, pic, fl, Bp, gvt);
+            gather<D, SPL, false, true, false, TF>(st, xp, h1, GridFetch<D, TF>{g, nn, st.base[0], st.base[1], st.base[2]}, false, false, pic, fl, Bp, gvt);
             double gv[3][3];
             for (int r = 0; r < 3; ++r) for (int k = 0; k < 3; ++k) gv[r][k] = (double)gvt[r][k];
             deform_and_project<D, TF, MAT>(P, dt, src, p, dst, i, gv, err, pcount);
