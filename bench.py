@@ -312,10 +312,7 @@ def batched_run(args, count=128):
     tune_steps = 180
     for e in engs:
         b.add(e, tune_steps + args.warmup + args.steps)
-    chunk, tuned_rate = b.autotune(tune_steps)  # these steps are ordinary steps of the scenes
-    rc, tinfo = b.step(tune_steps)  # scenes that took fewer tuning steps catch up, so all start the timed region level
-    if rc:
-        raise A.MPMError(rc, "batched tuning steps")
+    chunk, tuned_rate = b.autotune(tune_steps)  # ordinary steps of the scenes, the same number for each
     if args.warmup:
         rc, _ = b.step(args.warmup)
         if rc:

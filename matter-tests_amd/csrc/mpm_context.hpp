@@ -26,7 +26,8 @@ struct mpm_context {
     uint32_t* bbuf = nullptr;   // block histogram, offsets, cursors
     uint16_t* cellbuf = nullptr; // per-block dual-cell offsets
     char* stgbuf = nullptr;     // per-block node tiles (staging of the scatter kernels)
-    uint32_t cap_blocks = 0;
+    uint32_t cap_blocks = 0;    // allocated blocks (sort arrays, staging)
+    uint32_t act_blocks = 0;    // blocks the block kernels launch and a step may use (<= cap_blocks)
     mpm::StagingT<double> stg{};
     mpm::StagingT<float> stgf{};
     bool flip_formed = false;   // the last enqueued step wrote grid flip

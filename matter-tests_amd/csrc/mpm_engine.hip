@@ -113,34 +113,50 @@ void layout_particles(mpm_context* c, uint32_t n) {
     c->sb.perm = c->ibuf + (size_t)3 * n;
 }
 
+// Nodes per axis by which the grid may grow over `batch` steps: the particle box grows by at most about
+// cfl * dx per side per step (dt <= cfl dx / vmax), plus the remesh snap. A step that outgrows it stops
+// (StepState::need_realloc) and the host grows the grid and resumes, so the margin only sets how often that
+// happens.
+uint32_t growth_nodes(const mpm_context* c, uint32_t batch) {
+    const double cfl = std::min(1.0, std::max(0.25, c->cfg.cfl));
+    return 2 * (uint32_t)std::ceil(cfl * batch) + 6;
+}
+
 int ensure_grid(mpm_context* c, uint32_t batch) {
     const int D = c->cfg.dim;
+    const uint32_t margin = growth_nodes(c, batch);
     uint32_t want_axis = 0; uint64_t want_nodes = 1;
     for (int d = 0; d < D; ++d) {
-        uint32_t a = c->est_n[d] + 2 * (batch + 4);
+        uint32_t a = c->est_n[d] + margin;
         want_axis = std::max(want_axis, a);
         want_nodes *= a;
     }
-    uint64_t want_blocks = 1;
+    uint64_t need = 1;  // blocks of a grid grown by the margin
     const uint32_t be = D == 3 ? BlockEdge<3>::v : BlockEdge<2>::v;
-    for (int d = 0; d < D; ++d) want_blocks *= (c->est_n[d] + 2 * (batch + 4) + be - 1) / be;
-    if (want_blocks > c->cap_blocks) {
+    for (int d = 0; d < D; ++d) need *= (c->est_n[d] + margin + be - 1) / be;
+    if (need > c->cap_blocks) {
         if (c->bbuf) { (void)hipFree(c->bbuf); c->bbuf = nullptr; }
         if (c->cellbuf) { (void)hipFree(c->cellbuf); c->cellbuf = nullptr; }
         if (c->stgbuf) { (void)hipFree(c->stgbuf); c->stgbuf = nullptr; }
-        c->cap_blocks = 0;
+        c->cap_blocks = 0; c->act_blocks = 0;
         const size_t nt = tile_nodes(D, c->cfg.spline_degree);
-        if (want_blocks * nt > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
-        if (hipMalloc(&c->bbuf, (size_t)3 * want_blocks * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
-        if (hipMalloc(&c->cellbuf, (size_t)65 * want_blocks * sizeof(uint16_t)) != hipSuccess) { c->cellbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        if (need * nt > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
+        const uint64_t alloc = std::min<uint64_t>(0xFFFFFFF0ull / nt, need + need / 4);  // room for the active count to grow
+        if (hipMalloc(&c->bbuf, (size_t)3 * alloc * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        if (hipMalloc(&c->cellbuf, (size_t)65 * alloc * sizeof(uint16_t)) != hipSuccess) { c->cellbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
         const size_t tf = c->split ? sizeof(float) : sizeof(double);
-        if (hipMalloc(&c->stgbuf, (size_t)kMaxNV * nt * want_blocks * tf) != hipSuccess) { c->stgbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
-        c->cap_blocks = (uint32_t)want_blocks;
-        c->sb.count = c->bbuf; c->sb.offset = c->bbuf + want_blocks; c->sb.cursor = c->bbuf + 2 * want_blocks;
+        if (hipMalloc(&c->stgbuf, (size_t)kMaxNV * nt * alloc * tf) != hipSuccess) { c->stgbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        c->cap_blocks = (uint32_t)alloc;
+        c->sb.count = c->bbuf; c->sb.offset = c->bbuf + alloc; c->sb.cursor = c->bbuf + 2 * alloc;
         c->sb.cell_off = c->cellbuf;
-        c->stg.base = (double*)c->stgbuf; c->stg.stride = (uint32_t)(nt * want_blocks);
-        c->stgf.base = (float*)c->stgbuf; c->stgf.stride = (uint32_t)(nt * want_blocks);
+        c->stg.base = (double*)c->stgbuf; c->stg.stride = (uint32_t)(nt * alloc);
+        c->stgf.base = (float*)c->stgbuf; c->stgf.stride = (uint32_t)(nt * alloc);
     }
+    // Active blocks: the block kernels launch one workgroup per active block, and a step whose grid has more
+    // blocks stops for a reallocation. They follow the batch's need with some hysteresis (grow to 5/4 of it,
+    // shrink below half), so that batches keep their launch shapes and graphs for a while.
+    if (need > c->act_blocks || 2 * need < c->act_blocks)
+        c->act_blocks = (uint32_t)std::min<uint64_t>(c->cap_blocks, need + need / 4);
     if (want_axis <= c->cap_axis && want_nodes <= c->cap_nodes) return 0;
     if (want_nodes > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
     want_axis = std::max(want_axis, c->cap_axis);
@@ -325,7 +341,7 @@ SceneRef<TF> scene_ref(mpm_context* c) {
     r.P = c->dP; r.S = c->S;
     r.ps[0] = layouts<TF>(c)[0]; r.ps[1] = layouts<TF>(c)[1];
     r.sb = c->sb; r.g = grid_of<TF>(c); r.stg = staging_of<TF>(c);
-    r.n = c->n; r.cap_axis = c->cap_axis; r.cap_nodes = (uint32_t)c->cap_nodes; r.cap_blocks = c->cap_blocks;
+    r.n = c->n; r.cap_axis = c->cap_axis; r.cap_nodes = (uint32_t)c->cap_nodes; r.cap_blocks = c->act_blocks;
     return r;
 }
 
@@ -417,7 +433,7 @@ int rebuild_tables(mpm_batch* b, bool check_cons) {
             mpm_context* c = bc.members[i];
             if (bc.key.split) { SceneRef<float> r = scene_ref<float>(c); std::memcpy(host.data() + ref_off[k] + i * sizeof(r), &r, sizeof(r)); }
             else { SceneRef<double> r = scene_ref<double>(c); std::memcpy(host.data() + ref_off[k] + i * sizeof(r), &r, sizeof(r)); }
-            const uint32_t wg[BR_KINDS] = {(c->n + kBlock - 1) / kBlock, c->cap_blocks, grid_workgroups(c), (c->cap_blocks + kBlock - 1) / kBlock};
+            const uint32_t wg[BR_KINDS] = {(c->n + kBlock - 1) / kBlock, c->act_blocks, grid_workgroups(c), (c->act_blocks + kBlock - 1) / kBlock};
             for (int r = 0; r < BR_KINDS; ++r) st[r * (m + 1) + i + 1] = st[r * (m + 1) + i] + wg[r];
         }
         for (int r = 0; r < BR_KINDS; ++r) bc.wg[r] = st[r * (m + 1) + m];
@@ -680,7 +696,7 @@ int mpm_step(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* inf
         uint32_t xsel = c->Sh->xsel;
         k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u,
                                            (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u, c->cap_axis, (uint32_t)c->cap_nodes,
-                                           c->cap_blocks);
+                                           c->act_blocks);
         for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, (flags & MPM_STEP_CHECK_CONSERVATION) != 0);
         if (hipGetLastError() != hipSuccess) { status |= MPM_ERR_HIP; break; }
         rc = sync_state(c);
@@ -997,11 +1013,12 @@ int mpm_batch_step(mpm_batch* b, uint32_t max_steps, uint32_t flags, mpm_batch_i
         for (size_t i = 0; i < b->scenes.size(); ++i) {
             mpm_context* c = b->scenes[i];
             const uint32_t lim = std::min(chunk, want[i] - done[i]);
-            if (!lim && c->cap_blocks && c->cap_nodes) continue;  // idle member whose buffers exist
+            if (!lim && c->act_blocks && c->cap_nodes) continue;  // idle member whose buffers exist
             const void* before[3] = {c->gbuf, c->stgbuf, c->bbuf};
+            const uint32_t act_before = c->act_blocks;
             const int rc = ensure_grid(c, std::max(lim, 1u));
             if (rc) { status |= rc; break; }
-            if (before[0] != c->gbuf || before[1] != c->stgbuf || before[2] != c->bbuf) b->dirty = true;
+            if (before[0] != c->gbuf || before[1] != c->stgbuf || before[2] != c->bbuf || act_before != c->act_blocks) b->dirty = true;
         }
         if (status) break;
         if (b->dirty || b->check_cons != cons) {

@@ -13,7 +13,7 @@ void enqueue_p2g(mpm_context* c, const PartStateT<TF>& src, const PartStateT<TF>
     hipStream_t s = c->stream;
     const GridPtrsT<TF>& g = grid_of<TF>(c);
     const StagingT<TF>& stg = staging_of<TF>(c);
-    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL, SEP, MIBF, TF><<<c->cap_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg); }
+    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL, SEP, MIBF, TF><<<c->act_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg); }
     { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF, TF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, g, stg); }
 }
 
@@ -51,7 +51,7 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     const uint32_t n = c->n;
     const uint32_t pblocks = (n + kBlock - 1) / kBlock;
     const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 8192);
-    const uint32_t zblocks = (c->cap_blocks + kBlock - 1) / kBlock;
+    const uint32_t zblocks = (c->act_blocks + kBlock - 1) / kBlock;
     { LaunchTimer t(c, KS_SETUP); k_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g); }
     { LaunchTimer t(c, KS_ZERO); k_zero_grid<><<<zblocks, kBlock, 0, s>>>(c->S, c->sb); }
     { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, g, n); }
@@ -65,12 +65,12 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     else enqueue_p2g<D, SPL, false, false, TF>(c, src, dst, gblocks);
     {
         LaunchTimer t(c, KS_G2P);
-        if (P.flip_ratio >= -1) k_g2p<D, SPL, true, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
-        else k_g2p<D, SPL, false, TF><<<c->cap_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
+        if (P.flip_ratio >= -1) k_g2p<D, SPL, true, TF><<<c->act_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
+        else k_g2p<D, SPL, false, TF><<<c->act_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
     }
     if (P.use_musl) {
         constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
-        { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->cap_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
+        { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->act_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
         { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL, TF><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, g, stg); }
     }
     {
