@@ -239,11 +239,13 @@ def main():
                    "blocks_last_step": head["blocks"], "precision": forms[0]},
         "roofline": head["roofline"],
         "step_roofline": head["step_roofline"],
+        "pcie_inclusive": head["pcie_inclusive"],
         "kernels": head["kernels"],
     }
     if "split" in runs and "fp64" in runs:
         xs, xd = runs["split"]["x"], runs["fp64"]["x"]
-        result["fp64"] = {k: runs["fp64"][k] for k in ("value", "ms_per_step", "roofline", "step_roofline", "kernels")}
+        result["fp64"] = {k: runs["fp64"][k] for k in ("value", "ms_per_step", "roofline", "step_roofline", "pcie_inclusive",
+                                                       "kernels")}
         result["split_speedup_vs_fp64"] = runs["split"]["value"] / runs["fp64"]["value"]
         result["split_error_vs_fp64_gpu"] = {
             "position_rel": float(np.linalg.norm(xs - xd) / np.linalg.norm(xd)),
@@ -367,7 +369,9 @@ def timed_run(prec, args):
     cfg, plates, objs, x = S.dp_collapse(n_per_axis=args.n_axis, musl=True, precision=prec)
     n = x.shape[0]
     eng = A.Engine(A.product_library(), cfg, plates, objs)
-    eng.set_particles(x)
+    tu = time.perf_counter()
+    eng.set_particles(x)  # host arrays -> device layout (synchronous)
+    upload_s = time.perf_counter() - tu
     if args.warmup:
         rc, _ = eng.step(args.warmup)
         if rc:
@@ -405,9 +409,16 @@ def timed_run(prec, args):
     step_gbs = bpu * n / (wall / args.steps) / 1e9
     step_roofline = {"bytes_per_particle_update": bpu, "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": step_gbs / HBM_PEAK_GBS}
+    td = time.perf_counter()
+    state = eng.get_particles()  # device layout -> host arrays in the caller's order (synchronous)
+    download_s = time.perf_counter() - td
+    # The same K steps with the host hand-over of the boundary counted: particle state in before them and out
+    # after them. Reported beside the value, never as it.
+    pcie = {"value": n * args.steps / (wall + upload_s + download_s), "upload_s": upload_s, "download_s": download_s,
+            "note": "particles x K / (K steps + mpm_set_particles + mpm_get_particles of the full state)"}
     out = dict(cfg=cfg, n=n, value=n * args.steps / wall, ms_per_step=wall * 1e3 / args.steps, kernels=kernels,
                roofline=roofline, step_roofline=step_roofline, grid_nodes=grid_nodes, active=active, blocks=blocks,
-               x=eng.get_particles()["x"], scene=(plates, objs, x))
+               x=state["x"], scene=(plates, objs, x), pcie_inclusive=pcie)
     del eng
     return out
 
