@@ -58,7 +58,7 @@ enum mpm_precision {
 /* Status / error bits. */
 #define MPM_OK                    0u
 #define MPM_ERR_INVALID_ARGUMENT  (1u << 0)  /* bad pointer, size or enum value                     */
-#define MPM_ERR_UNSUPPORTED       (1u << 1)  /* option outside this engine (pbc, Snow, linear spline) */
+#define MPM_ERR_UNSUPPORTED       (1u << 1)  /* option outside this engine (Snow, linear spline)       */
 #define MPM_ERR_HIP               (1u << 2)  /* a HIP runtime call failed                             */
 #define MPM_ERR_OUT_OF_MEMORY     (1u << 3)
 #define MPM_ERR_PERZYNA_NEGATIVE  (1u << 4)  /* plasticity.cpp:190-193,286-289,546-553 */
@@ -97,7 +97,7 @@ typedef struct mpm_config {
     int32_t use_mibf;
     int32_t use_musl;
     int32_t gravity_special;
-    int32_t pbc;                       /* unsupported: must be 0 */
+    int32_t pbc;                       /* periodic in x over [0, Lx): pbc.cpp, remesh.cpp:14-36 */
     int32_t change_particle_positions; /* position_update.cpp:38-43 */
     uint32_t end_frame;
     uint32_t reserved0;

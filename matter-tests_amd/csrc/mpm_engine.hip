@@ -34,6 +34,8 @@ void derive_params(const mpm_config& c, Params& P) {
     P.dim = c.dim; P.spline_degree = c.spline_degree; P.elastic_model = c.elastic_model; P.plastic_model = c.plastic_model;
     P.hardening_law = c.hardening_law; P.use_pradhana = c.use_pradhana; P.use_mises_q = c.use_mises_q; P.use_mibf = c.use_mibf;
     P.use_musl = c.use_musl; P.gravity_special = c.gravity_special; P.change_particle_positions = c.change_particle_positions;
+    P.pbc = c.pbc ? 1 : 0;
+    P.L[0] = c.Lx; P.L[1] = c.Ly; P.L[2] = c.Lz;
     P.end_frame = c.end_frame;
     // simulate() set-up, simulation.cpp:94-145
     P.dx = c.dx;
@@ -77,7 +79,8 @@ size_t layout_bytes(int D, bool split) {
 int alloc_particles(mpm_context* c, uint32_t n) {
     const int D = c->cfg.dim;
     const size_t need = 2 * layout_bytes(D, c->split) * ((size_t)n + kArrayPad) + 64;
-    const size_t ineed = (size_t)4 * n;  // id x2, key, perm
+    // id x2, key, perm; with periodic boundaries key and perm also hold one periodic copy per particle
+    const size_t ineed = (size_t)(c->cfg.pbc ? 6 : 4) * n;
     if (!(c->pbuf && c->pbuf_bytes >= need)) {
         if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
         if (hipMalloc(&c->pbuf, need) != hipSuccess) { c->pbuf = nullptr; c->pbuf_bytes = 0; return MPM_ERR_OUT_OF_MEMORY; }
@@ -109,8 +112,10 @@ void layout_particles(mpm_context* c, uint32_t n) {
         ps.id = c->ibuf + (size_t)s * n;
         layouts<TF>(c)[s] = ps;
     }
+    const size_t ks = c->cfg.pbc ? (size_t)2 * n : (size_t)n;
     c->sb.key = c->ibuf + (size_t)2 * n;
-    c->sb.perm = c->ibuf + (size_t)3 * n;
+    c->sb.perm = c->sb.key + ks;
+    c->sb.ghosts = c->cfg.pbc ? 1u : 0u;
 }
 
 // Nodes per axis by which the grid may grow over `batch` steps: the particle box grows by at most about
@@ -142,12 +147,14 @@ int ensure_grid(mpm_context* c, uint32_t batch) {
         const size_t nt = tile_nodes(D, c->cfg.spline_degree);
         if (need * nt > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
         const uint64_t alloc = std::min<uint64_t>(0xFFFFFFF0ull / nt, need + need / 4);  // room for the active count to grow
-        if (hipMalloc(&c->bbuf, (size_t)3 * alloc * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        // count, offset, cursor; with periodic boundaries each holds the periodic copies' blocks after the particles'
+        const uint64_t bs = c->cfg.pbc ? 2 * alloc : alloc;
+        if (hipMalloc(&c->bbuf, (size_t)3 * bs * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
         if (hipMalloc(&c->cellbuf, (size_t)65 * alloc * sizeof(uint16_t)) != hipSuccess) { c->cellbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
         const size_t tf = c->split ? sizeof(float) : sizeof(double);
         if (hipMalloc(&c->stgbuf, (size_t)kMaxNV * nt * alloc * tf) != hipSuccess) { c->stgbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
         c->cap_blocks = (uint32_t)alloc;
-        c->sb.count = c->bbuf; c->sb.offset = c->bbuf + alloc; c->sb.cursor = c->bbuf + 2 * alloc;
+        c->sb.count = c->bbuf; c->sb.offset = c->bbuf + bs; c->sb.cursor = c->bbuf + 2 * bs;
         c->sb.cell_off = c->cellbuf;
         c->stg.base = (double*)c->stgbuf; c->stg.stride = (uint32_t)(nt * alloc);
         c->stgf.base = (float*)c->stgbuf; c->stgf.stride = (uint32_t)(nt * alloc);
@@ -559,7 +566,7 @@ int mpm_create(const mpm_config* cfg, mpm_context** out) {
     if (c.dim != 2 && c.dim != 3) return MPM_ERR_INVALID_ARGUMENT;
     if (c.spline_degree != 2 && c.spline_degree != 3) return MPM_ERR_UNSUPPORTED;
     if (c.precision != MPM_PRECISION_FP64 && c.precision != MPM_PRECISION_SPLIT) return MPM_ERR_INVALID_ARGUMENT;
-    if (c.pbc) return MPM_ERR_UNSUPPORTED;
+    if (c.pbc && !(c.Lx > 0 && c.Ly > 0 && (c.dim == 2 || c.Lz > 0))) return MPM_ERR_INVALID_ARGUMENT;
     if (!valid_enum(c.elastic_model, 0, 1) || !valid_enum(c.plastic_model, 0, 10) || !valid_enum(c.hardening_law, 0, 4))
         return MPM_ERR_INVALID_ARGUMENT;
     if (c.plastic_model == MPM_PLASTIC_SNOW) return MPM_ERR_UNSUPPORTED;
@@ -647,8 +654,14 @@ int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in)
     c->n = n;
     rc = c->split ? upload_particles<float>(c, n, in) : upload_particles<double>(c, n, in);
     if (rc) return rc;
-    // initial grid estimate from the particle bounds
-    for (int d = 0; d < D; ++d) {
+    // initial grid estimate from the particle bounds (periodic: the fixed grid of remeshFixed(4), remesh.cpp:14-36)
+    for (int d = 0; d < D && c->cfg.pbc; ++d) {
+        const double L = d == 0 ? c->cfg.Lx : (d == 1 ? c->cfg.Ly : c->cfg.Lz);
+        const double span = (L + 6.0 * c->cfg.dx - (d == 1 ? -c->cfg.dx : -5.0 * c->cfg.dx)) / c->cfg.dx;
+        if (!(span < 4.0e6)) return MPM_ERR_OUT_OF_MEMORY;
+        c->est_n[d] = (uint32_t)span + 2;
+    }
+    for (int d = 0; d < D && !c->cfg.pbc; ++d) {
         double mn = in->x[d], mx = in->x[d];
         for (uint32_t p = 1; p < n; ++p) { double v = in->x[(size_t)p * D + d]; mn = std::min(mn, v); mx = std::max(mx, v); }
         if (c->cfg.grid_reference_point[0] < 1e10) { mn = std::min(mn, c->cfg.grid_reference_point[d]); mx = std::max(mx, c->cfg.grid_reference_point[d]); }
@@ -961,6 +974,7 @@ void mpm_batch_destroy(mpm_batch* b) {
 int mpm_batch_add(mpm_batch* b, mpm_context* c, uint32_t nsteps) {
     if (!b || !c || c->batch) return MPM_ERR_INVALID_ARGUMENT;
     if (!c->ready) return MPM_ERR_NOT_READY;
+    if (c->cfg.pbc) return MPM_ERR_UNSUPPORTED;  // periodic scenes step alone (mpm_step)
     if (c->device != b->device) return MPM_ERR_INVALID_ARGUMENT;
     if (hipSetDevice(b->device) != hipSuccess) return MPM_ERR_HIP;
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -60,7 +60,13 @@ struct SortBufs {
     uint32_t* offset;    // exclusive prefix sum of count
     uint32_t* cursor;    // scatter cursors
     uint16_t* cell_off;  // per block: first sorted slot (relative to offset) of each dual cell, 65 entries
+    // Periodic boundaries (ghosts = 1): particle p near x = 0 or x = Lx also has a periodic copy, binned
+    // under key[n + p] = nblocks + its block (kNoGhost: none). count/offset/cursor then have 2 nblocks
+    // entries, and the copies' slots follow the particles' (perm entries kGhostBit | p).
+    uint32_t ghosts;
 };
+constexpr uint32_t kGhostBit = 0x80000000u;
+constexpr uint32_t kNoGhost = 0xFFFFFFFFu;
 
 template <class TF> struct GridPtrsT {
     TF* mass;        // m_i
@@ -410,11 +416,13 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
                     else { unsigned ef = (unsigned)floor(fmax(0.0, (mni[d] - mn[d]) / dx - 1e-8 * dx)); lo[d] = loi[d] - ef * dx; }
                 }
             }
-            // node counts of arange(lo, hi + dx, dx)
+            // node counts of arange(lo, hi + dx, dx); periodic runs keep the fixed grid of remeshFixed(4)
+            // (remesh.cpp:14-36): x and z from -5 dx to L + 6 dx, y from -dx to Ly + 6 dx
             uint32_t cnt[3] = {1, 1, 1};
             bool fits = true;
             for (int d = 0; d < D; ++d) {
                 double stop = hi[d] + dx;
+                if (P.pbc) { lo[d] = d == 1 ? -dx : -dx * 5.0; stop = P.L[d] + 6.0 * dx; hi[d] = stop - dx; }
                 uint32_t c = 0;
                 for (double val = lo[d]; val < stop; val += dx) { if (++c > S->cap_axis) break; }
                 cnt[d] = c;
@@ -473,7 +481,7 @@ __global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrsT<
 // ----------------------------------------------------------------------------------------------
 __device__ __forceinline__ void zero_grid_body(const StepState* S, const SortBufs& sb, uint32_t bid, uint32_t nbid) {
     if (S->stop) return;
-    const uint32_t nb = S->nblocks;
+    const uint32_t nb = S->nblocks << sb.ghosts;
     for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nb; i += nbid * blockDim.x) sb.count[i] = 0;
 }
 template <int = 0>  // a template, so that every translation unit may instantiate it
@@ -502,7 +510,18 @@ __device__ __forceinline__ uint32_t block_key(const double* xp, const StepState*
     return D == 3 ? (c[0] * S->nb[1] + c[1]) * S->nb[2] + c[2] : c[0] * S->nb[1] + c[1];
 }
 
-// k_bin_count: block of every particle and the histogram (wavefront-aggregated atomics).
+// Periodic copy of a particle at x0 (PBCAddParticles, pbc.cpp:37-83): within 4 dx of x = Lx (strictly
+// inside) the copy sits at -(Lx - x0); otherwise within 4 dx of x = 0 it sits at Lx + x0.
+__device__ __forceinline__ bool periodic_copy(double Lx, double band, double x0, double& gx) {
+    double diff = Lx - x0;
+    if (diff < band && diff > 0) { gx = -diff; return true; }
+    diff = x0;
+    if (diff < band && diff >= 0) { gx = Lx + diff; return true; }
+    return false;
+}
+
+// k_bin_count: block of every particle and the histogram (wavefront-aggregated atomics); with periodic
+// boundaries also the block of each particle's periodic copy (histogram entries nblocks + block).
 template <int D, int SPL, class TF>
 __device__ __forceinline__ void bin_count_body(const Params& P, const StepState* S, const PartStateT<TF>& src, const SortBufs& sb,
                                                const GridPtrsT<TF>& g, uint32_t n, uint32_t bid) {
@@ -510,13 +529,24 @@ __device__ __forceinline__ void bin_count_body(const Params& P, const StepState*
     const uint32_t p = bid * blockDim.x + threadIdx.x;
     const bool active = p < n;
     uint32_t key = 0;
+    double xp[3] = {0, 0, 0};
     if (active) {
-        double xp[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
         key = block_key<D, SPL, TF>(xp, S, g, P.one_over_dx);
         sb.key[p] = key;
     }
     (void)wave_claim(sb.count, key, active);
+    if (sb.ghosts) {
+        double gx = 0.0;
+        const bool has = active && periodic_copy(P.L[0], 4.0 * P.dx, xp[0], gx);
+        uint32_t gkey = 0;
+        if (has) {
+            xp[0] = gx;
+            gkey = S->nblocks + block_key<D, SPL, TF>(xp, S, g, P.one_over_dx);
+        }
+        if (active) sb.key[n + p] = has ? gkey : kNoGhost;
+        (void)wave_claim(sb.count, gkey, has);
+    }
 }
 template <int D, int SPL, class TF>
 __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState* S, PartStateT<TF> src, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
@@ -525,10 +555,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState*
 
 // k_bin_scan: exclusive prefix sum of the block histogram; one workgroup of 1024 threads, each owning
 // a contiguous run of blocks.
-__device__ __forceinline__ void bin_scan_body(const StepState* S, const SortBufs& sb) {
+__device__ __forceinline__ void bin_scan_body(StepState* S, const SortBufs& sb) {
     if (S->stop) return;
     __shared__ uint32_t part[1024];
-    const uint32_t nb = S->nblocks;
+    const uint32_t nb = S->nblocks << sb.ghosts;
     const uint32_t per = (nb + 1023) / 1024;
     const uint32_t b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
     uint32_t s = 0;
@@ -542,19 +572,23 @@ __device__ __forceinline__ void bin_scan_body(const StepState* S, const SortBufs
         __syncthreads();
     }
     uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-    for (uint32_t b = b0; b < b1; ++b) { sb.offset[b] = run; sb.cursor[b] = run; run += sb.count[b]; }
+    for (uint32_t b = b0; b < b1; ++b) {
+        if (sb.ghosts && b == S->nblocks) S->n_ghosts = part[1023] - run;  // entries from here on are copies
+        sb.offset[b] = run; sb.cursor[b] = run; run += sb.count[b];
+    }
 }
 template <int = 0>  // a template, so that every translation unit may instantiate it
-__global__ __launch_bounds__(1024) void k_bin_scan(const StepState* S, SortBufs sb) { bin_scan_body(S, sb); }
+__global__ __launch_bounds__(1024) void k_bin_scan(StepState* S, SortBufs sb) { bin_scan_body(S, sb); }
 
-// k_bin_scatter: counting-sort permutation (the order inside a block is fixed later by k_p2g).
+// k_bin_scatter: counting-sort permutation (the order inside a block is fixed later by k_p2g). With
+// periodic boundaries it runs over 2n entries: entry n + p is the periodic copy of p, if any.
 __device__ __forceinline__ void bin_scatter_body(const StepState* S, const SortBufs& sb, uint32_t n, uint32_t bid) {
     if (S->stop) return;
-    const uint32_t p = bid * blockDim.x + threadIdx.x;
-    const bool active = p < n;
-    const uint32_t key = active ? sb.key[p] : 0u;
-    const uint32_t pos = wave_claim(sb.cursor, key, active);
-    if (active) sb.perm[pos] = p;
+    const uint32_t e = bid * blockDim.x + threadIdx.x;
+    const uint32_t key = e < (n << sb.ghosts) ? sb.key[e] : kNoGhost;
+    const bool active = key != kNoGhost;
+    const uint32_t pos = wave_claim(sb.cursor, active ? key : 0u, active);
+    if (active) sb.perm[pos] = e < n ? e : (kGhostBit | (e - n));
 }
 template <int = 0>  // a template, so that every translation unit may instantiate it
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, SortBufs sb, uint32_t n) { bin_scatter_body(S, sb, n, blockIdx.x); }
@@ -641,7 +675,9 @@ __device__ __forceinline__ int tile_index(int t0, int t1, int t2) {
 // on, so that the 64 lanes of a scatter wavefront (one per cell) read consecutive slots at every step and
 // the next step's reads of this layout are coalesced. Particles whose window is outside the block (their
 // window leaves the grid) go last and are reported. L.cstart keeps the cell-major offsets (cell counts).
-template <int D, int SPL, int NV, class TF>
+// GHOST: the block's periodic copies (perm entries kGhostBit | p, at their copy positions); they are only
+// ordered in LDS for the scatter, and nothing is written back.
+template <int D, int SPL, int NV, class TF, bool GHOST = false>
 __device__ __forceinline__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, const Params& P, const StepState* S, const PartStateT<TF>& src,
                                 const PartStateT<TF>& dst, const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b, uint32_t start,
                                 uint32_t cnt, const int* bc) {
@@ -656,7 +692,8 @@ __device__ __forceinline__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, c
     for (uint32_t k = tid; k < cnt; k += blockDim.x) {
         const uint32_t p = sb.perm[start + k];
         double xp[3] = {0, 0, 0};
-        for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
+        for (int d = 0; d < D; ++d) xp[d] = src.x[d][p & ~kGhostBit];
+        if (GHOST) { double gx = xp[0]; (void)periodic_copy(P.L[0], 4.0 * P.dx, xp[0], gx); xp[0] = gx; }
         int wb[3];
         int cell = G::NC;
         if (window_base<D, SPL>(wb, xp, org, nn, L.coords.view(bc), h1, true)) {
@@ -715,6 +752,7 @@ __device__ __forceinline__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, c
     }
     for (uint32_t k = L.cstart[G::NC] + tid; k < cnt; k += blockDim.x) L.p[k] = L.u.sort.tmp[k];  // outside bucket last
     __syncthreads();
+    if (GHOST) return err;
     for (uint32_t k = tid; k < cnt; k += blockDim.x) {
         const uint32_t p = L.p[k];
         sb.perm[start + k] = p;
@@ -779,18 +817,11 @@ __device__ __forceinline__ void add_partials(ScatterLDS<D, SPL, NV, TF>& L, int 
 // contrib.load(p, i, part) fetches source particle p (sorted slot i); contrib.add(part, a0, half, acc, base)
 // adds it to acc for slices a0.. and second-axis half `half` and returns false if its window leaves the grid.
 template <int D, int SPL, int NV, class TF, class Contrib>
-__device__ __forceinline__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, uint32_t b,
-                              uint32_t start) {
+__device__ __forceinline__ void accumulate_cells(const ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, uint32_t start,
+                                                 TF (&acc)[ScatterShape<D, SPL, NV, TF>::SPP][ScatterShape<D, SPL, NV, TF>::NSH][NV]) {
     using SS = ScatterShape<D, SPL, NV, TF>;
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int grp = wave % SS::NG, half = wave / SS::NG, a0 = grp * SS::SPP;
-    TF acc[SS::SPP][SS::NSH][NV];
-#pragma unroll
-    for (int s = 0; s < SS::SPP; ++s)
-#pragma unroll
-        for (int o = 0; o < SS::NSH; ++o)
-#pragma unroll
-            for (int v = 0; v < NV; ++v) acc[s][o][v] = TF(0);
     {  // rank-major layout (see order_block): at rank j the cells with more than j particles are consecutive
         const uint32_t n_c = L.cstart[lane + 1] - L.cstart[lane];
         const uint32_t n_max = wave_max_u32(n_c);
@@ -809,6 +840,15 @@ __device__ __forceinline__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, con
             }
         }
     }
+}
+
+// The wavefronts' partial sums into the zeroed tile, one wavefront after another, and the tile to staging.
+template <int D, int SPL, int NV, class TF>
+__device__ __forceinline__ void finish_tile(ScatterLDS<D, SPL, NV, TF>& L, const StagingT<TF>& stg, uint32_t b,
+                                            const TF (&acc)[ScatterShape<D, SPL, NV, TF>::SPP][ScatterShape<D, SPL, NV, TF>::NSH][NV]) {
+    using SS = ScatterShape<D, SPL, NV, TF>;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int grp = wave % SS::NG, half = wave / SS::NG, a0 = grp * SS::SPP;
     zero_tile<D, SPL, NV, TF>(L);
     for (int w = 0; w < SS::WAVES; ++w) {
         __syncthreads();
@@ -818,18 +858,41 @@ __device__ __forceinline__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, con
     write_tile<D, SPL, NV, TF>(L, stg, b);
 }
 
+template <int D, int SPL, int NV, class TF, class Acc>
+__device__ __forceinline__ void zero_acc(Acc& acc) {
+    using SS = ScatterShape<D, SPL, NV, TF>;
+#pragma unroll
+    for (int s = 0; s < SS::SPP; ++s)
+#pragma unroll
+        for (int o = 0; o < SS::NSH; ++o)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[s][o][v] = TF(0);
+}
+
+template <int D, int SPL, int NV, class TF, class Contrib>
+__device__ __forceinline__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, uint32_t b,
+                              uint32_t start) {
+    using SS = ScatterShape<D, SPL, NV, TF>;
+    TF acc[SS::SPP][SS::NSH][NV];
+    zero_acc<D, SPL, NV, TF>(acc);
+    accumulate_cells<D, SPL, NV, TF>(L, contrib, start, acc);
+    finish_tile<D, SPL, NV, TF>(L, stg, b, acc);
+}
+
 // Tile of a block with more than kMaxBP particles: every particle adds its window with LDS atomics.
+// Sorted slots start .. start+cnt-1, then gstart .. gstart+gcnt-1 (the block's periodic copies, if any).
 template <int D, int SPL, int NV, class TF, class Contrib>
 __device__ __forceinline__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, const SortBufs& sb,
-                                         uint32_t b, uint32_t start, uint32_t cnt, const int* bc) {
+                                         uint32_t b, uint32_t start, uint32_t cnt, const int* bc, uint32_t gstart = 0, uint32_t gcnt = 0) {
     using G = BG<D, SPL>;
     using SS = ScatterShape<D, SPL, NV, TF>;
     const int tid = threadIdx.x;
+    __syncthreads();
     zero_tile<D, SPL, NV, TF>(L);
     __syncthreads();
     uint32_t err = 0;
-    for (uint32_t k = tid; k < cnt; k += blockDim.x) {
-        const uint32_t i = start + k, p = sb.perm[i];
+    for (uint32_t k = tid; k < cnt + gcnt; k += blockDim.x) {
+        const uint32_t i = k < cnt ? start + k : gstart + (k - cnt), p = sb.perm[i];
         typename Contrib::Part q;
         contrib.load(p, i, q);
         bool ok = true;
@@ -953,12 +1016,16 @@ struct P2GContrib {
     double org[3];
     uint32_t nn[3];
     double h1, aD, kf;
+    double Lx, band;  // periodic interval and copy band (periodic copies only)
     bool apic;
-    __device__ __forceinline__ void load(uint32_t p, uint32_t, Part& q) const {
+    // Source particle p, or (kGhostBit | p) its periodic copy: the same state at the copy's position.
+    __device__ __forceinline__ void load(uint32_t e, uint32_t, Part& q) const {
+        const uint32_t p = e & ~kGhostBit;
         for (int d = 0; d < D; ++d) { q.x[d] = src.x[d][p]; q.v[d] = src.v[d][p]; }
         for (int k = 0; k < D * D; ++k) q.B[k] = apic ? src.B[k][p] : TF(0);
         for (int k = 0; k < SymPack<D>::n; ++k) q.t6[k] = src.tau[k][p];
         q.muI = MIBF ? src.muI[p] : 0.0;
+        if (e & kGhostBit) { double gx = q.x[0]; (void)periodic_copy(Lx, band, q.x[0], gx); q.x[0] = gx; }
     }
     __device__ __forceinline__ bool add(const Part& q, int a0, int half, TF (&acc)[SS::SPP][SS::NSH][NV], int* base) const {
         constexpr int W = BG<D, SPL>::W;
@@ -1167,16 +1234,23 @@ struct MuslContrib {
 // (explicit_euler_update.cpp:15-168). The reference adds the force only where m_i > 0; nodes with
 // m_i = 0 are never updated or read, so the force is scattered unconditionally.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL, bool SEP, bool MIBF, class TF>
+// PBC: the variant for periodic contexts, which also scatters the block's periodic copies.
+template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC = false>
 __device__ __forceinline__ void p2g_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                          const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
     constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
+    using SS = ScatterShape<D, SPL, NV, TF>;
     __shared__ ScatterLDS<D, SPL, NV, TF> L;
     if (S->stop) return;
-    if (b >= S->nblocks) return;
+    const uint32_t nbk = S->nblocks;
+    if (b >= nbk) return;
     const uint32_t cnt = sb.count[b];
-    if (cnt == 0) return;
+    // periodic copies binned into this block (added to the tile after the block's particles; PBCAddParticles
+    // appends them after the particles too, pbc.cpp:37-83)
+    const uint32_t gcnt = PBC ? sb.count[nbk + b] : 0u;
+    if (cnt == 0 && gcnt == 0) return;
     const uint32_t start = sb.offset[b];
+    const uint32_t gstart = gcnt ? sb.offset[nbk + b] : 0u;
     int bc[3];
     block_coords<D>(b, S, bc);
     P2GContrib<D, SPL, SEP, MIBF, TF> cf;
@@ -1187,18 +1261,34 @@ __device__ __forceinline__ void p2g_body(const Params& P, StepState* S, const Pa
     __syncthreads();
     cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
     cf.kf = S->dt * P.particle_volume / P.particle_mass;
+    cf.Lx = P.L[0]; cf.band = 4.0 * P.dx;
     uint32_t err = 0;
-    if (cnt > kMaxBP) {
-        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) dst.id[start + k] = src.id[sb.perm[start + k]];
-        err |= scatter_block_atomic<D, SPL, NV, TF>(L, cf, stg, sb, b, start, cnt, bc);
-    } else {
+    if (cnt > kMaxBP || gcnt > kMaxBP) {
+        if (cnt > kMaxBP) {
+            for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) dst.id[start + k] = src.id[sb.perm[start + k]];
+        } else if (cnt) {
+            err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
+        }
+        err |= scatter_block_atomic<D, SPL, NV, TF>(L, cf, stg, sb, b, start, cnt, bc, gstart, gcnt);
+    } else if (gcnt == 0) {
         err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
         scatter_block<D, SPL, NV, TF>(L, cf, stg, b, start);
+    } else {
+        TF acc[SS::SPP][SS::NSH][NV];
+        zero_acc<D, SPL, NV, TF>(acc);
+        if (cnt) {
+            err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
+            accumulate_cells<D, SPL, NV, TF>(L, cf, start, acc);
+            __syncthreads();
+        }
+        err |= order_block<D, SPL, NV, TF, true>(L, P, S, src, dst, sb, g, b, gstart, gcnt, bc);
+        accumulate_cells<D, SPL, NV, TF>(L, cf, gstart, acc);
+        finish_tile<D, SPL, NV, TF>(L, stg, b, acc);
     }
-    if (S->check_conservation) {  // sum of m_p v_p (simulation.cpp:292-317)
+    if (S->check_conservation) {  // sum of m_p v_p (simulation.cpp:292-317), periodic copies included
         double pm[3] = {0, 0, 0};
-        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
-            const uint32_t p = sb.perm[start + k];
+        for (uint32_t k = threadIdx.x; k < cnt + gcnt; k += blockDim.x) {
+            const uint32_t p = sb.perm[k < cnt ? start + k : gstart + (k - cnt)] & ~kGhostBit;
             for (int d = 0; d < D; ++d) pm[d] += P.particle_mass * src.v[d][p];
         }
         for (int d = 0; d < D; ++d) pm[d] = wave_sum_f64(pm[d]);
@@ -1207,10 +1297,10 @@ __device__ __forceinline__ void p2g_body(const Params& P, StepState* S, const Pa
     err = wave_or_u32(err);
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
-template <int D, int SPL, bool SEP, bool MIBF, class TF>
+template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
 __global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB))
 void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
-    p2g_body<D, SPL, SEP, MIBF, TF>(P, S, src, dst, sb, g, stg, blockIdx.x);
+    p2g_body<D, SPL, SEP, MIBF, TF, PBC>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }
 
 // k_musl_scatter: the MUSL re-scatter, in the particle order that k_p2g fixed for the step.
@@ -1253,8 +1343,11 @@ void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> d
 
 // Sum over the blocks that cover node gx (at most two per axis), in increasing block order, of the
 // staged tile values.
+// with_copies: the tiles of the P2G scatter, which also exist for blocks that hold only periodic copies
+// (the MUSL re-scatter has none).
 template <int D, int SPL, int NV, class TF>
-__device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& sb, const StagingT<TF>& stg, const int* gx, double (&s)[NV]) {
+__device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& sb, const StagingT<TF>& stg, const int* gx, double (&s)[NV],
+                                            bool with_copies = false) {
     using G = BG<D, SPL>;
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) {
@@ -1268,7 +1361,7 @@ __device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& 
         for (int b1 = lo[1]; b1 <= hi[1]; ++b1)
             for (int b2 = lo[2]; b2 <= hi[2]; ++b2) {
                 const uint32_t bl = D == 3 ? ((uint32_t)b0 * S->nb[1] + (uint32_t)b1) * S->nb[2] + (uint32_t)b2 : (uint32_t)b0 * S->nb[1] + (uint32_t)b1;
-                if (sb.count[bl] == 0) continue;
+                if (sb.count[bl] == 0 && !(with_copies && sb.count[S->nblocks + bl] != 0)) continue;
                 const int q = tile_index<D, SPL>(gx[0] - G::E * b0, gx[1] - G::E * b1, D == 3 ? gx[2] - G::E * b2 : 0);
                 const size_t at = (size_t)bl * G::NT + q;
 #pragma unroll
@@ -1297,7 +1390,7 @@ __device__ __forceinline__ void grid_update_body(const Params& P, StepState* S, 
         if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
         else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
         double s[NV];
-        sum_staging<D, SPL, NV, TF>(S, sb, stg, gx, s);
+        sum_staging<D, SPL, NV, TF>(S, sb, stg, gx, s, sb.ghosts != 0);
         const double sw = s[0];
         V<D> v;
         if (sw > 0) { for (int d = 0; d < D; ++d) v[d] = s[1 + d] / sw; }
@@ -1667,6 +1760,10 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
             vnew[d] = (double)(TF)blend_velocity(P.flip_ratio, vo, (double)fl[d], (double)pic[d]);
             xnew[d] = xp[d] + dt * (double)pic[d];
         }
+        if (P.pbc) {  // position_update.cpp:26-33
+            if (xnew[0] > P.L[0]) xnew[0] = xnew[0] - P.L[0];
+            else if (xnew[0] < 0) xnew[0] = P.L[0] + xnew[0];
+        }
         if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
         for (int d = 0; d < D; ++d) { dst.v[d][i] = (TF)vnew[d]; dst.x[d][i] = xnew[d]; }
         for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : TF(0);
@@ -1768,7 +1865,7 @@ __device__ __forceinline__ void finish_body(const Params& P, StepState* S, uint3
     const double dt = S->dt;
     for (int k = 0; k < P.n_plates; ++k) plate_move<D>(S->plates[k], dt, P.frame_dt, S->time);
     if (S->check_conservation) {
-        double pm = P.particle_mass * n_particles;
+        double pm = P.particle_mass * (n_particles + (P.pbc ? S->n_ghosts : 0u));  // Np includes the copies here
         if (fabs(S->cons_grid_mass - pm) > 1e-10 * pm) S->err |= MPM_ERR_CONSERVATION;
         double dn = 0, pn = 0;
         for (int d = 0; d < D; ++d) { double e = S->cons_grid_mom[d] - S->cons_part_mom[d]; dn += e * e; pn += S->cons_part_mom[d] * S->cons_part_mom[d]; }

@@ -132,6 +132,7 @@ int mpm_server_submit(mpm_server* s, const mpm_request_desc* q, uint64_t* id) {
     if ((q->n_plates && !q->plates) || (q->n_objects && !q->objects)) return MPM_ERR_INVALID_ARGUMENT;
     const int D = q->cfg->dim;
     if ((D != 2 && D != 3) || !(q->cfg->dx > 0)) return MPM_ERR_INVALID_ARGUMENT;
+    if (q->cfg->pbc) return MPM_ERR_UNSUPPORTED;  // periodic scenes step alone (mpm_step), not in a batch
     std::unique_ptr<Request> r(new (std::nothrow) Request());
     if (!r) return MPM_ERR_OUT_OF_MEMORY;
     r->id = s->reqs.size();

@@ -13,7 +13,11 @@ void enqueue_p2g(mpm_context* c, const PartStateT<TF>& src, const PartStateT<TF>
     hipStream_t s = c->stream;
     const GridPtrsT<TF>& g = grid_of<TF>(c);
     const StagingT<TF>& stg = staging_of<TF>(c);
-    { LaunchTimer t(c, KS_P2G); k_p2g<D, SPL, SEP, MIBF, TF><<<c->act_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg); }
+    {
+        LaunchTimer t(c, KS_P2G);
+        if (c->sb.ghosts) k_p2g<D, SPL, SEP, MIBF, TF, true><<<c->act_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg);
+        else k_p2g<D, SPL, SEP, MIBF, TF, false><<<c->act_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg);
+    }
     { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF, TF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, g, stg); }
 }
 
@@ -56,7 +60,8 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     { LaunchTimer t(c, KS_ZERO); k_zero_grid<><<<zblocks, kBlock, 0, s>>>(c->S, c->sb); }
     { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, g, n); }
     { LaunchTimer t(c, KS_BIN_SCAN); k_bin_scan<><<<1, 1024, 0, s>>>(c->S, c->sb); }
-    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<><<<pblocks, kBlock, 0, s>>>(c->S, c->sb, n); }
+    const uint32_t sblocks = (uint32_t)((((uint64_t)n << c->sb.ghosts) + kBlock - 1) / kBlock);  // + periodic copies
+    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<><<<sblocks, kBlock, 0, s>>>(c->S, c->sb, n); }
     const bool sep = P.flip_ratio >= -1 || check_cons;
     c->flip_formed = sep;
     if (sep && P.use_mibf) enqueue_p2g<D, SPL, true, true, TF>(c, src, dst, gblocks);

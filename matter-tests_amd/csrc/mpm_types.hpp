@@ -29,11 +29,13 @@ struct DevObject {
 struct Params {
     int32_t dim, spline_degree, elastic_model, plastic_model, hardening_law;
     int32_t use_pradhana, use_mises_q, use_mibf, use_musl, gravity_special, change_particle_positions;
+    int32_t pbc;     // periodic in x (pbc.cpp, remesh.cpp:14-36, position_update.cpp:26-33)
     int32_t n_objects, n_plates;
     uint32_t end_frame;
     double dx, one_over_dx, apicDinverse, particle_mass, particle_volume;
     double cfl, dt_max, frame_dt, final_time, wave_speed, flip_ratio, gravity_time;
     double gravity_final[3], grid_reference_point[3];
+    double L[3];     // Lx, Ly, Lz: the periodic interval [0, Lx) and the fixed grid's extent
     double E, nu, mu, lambda, K, fac_Q, stress_tolerance, xi;
     double q_prefac, d_prefac, e_mu_prefac, f_mu_prefac, rma_prefac;
     double q_max, q_min, p_min, M, q_cohesion, perzyna_exp, perzyna_visc, beta, p0;
@@ -73,7 +75,7 @@ struct StepState {
     uint32_t need_realloc;  // the grid of the next step exceeds capacity
     uint32_t steps_done, frame_ended, finished;
     uint32_t step_limit;    // stop once steps_done reaches it (a scene's step budget in a batch)
-    uint32_t pad1;
+    uint32_t n_ghosts;      // periodic copies binned this step (k_bin_scan)
     uint32_t stop_at_frame_end, check_conservation;
     uint32_t cap_axis, cap_nodes;
     uint32_t xsel;          // which particle-state buffer is current (toggles per executed step)

@@ -39,6 +39,7 @@ template <int D, int SPL> struct OracleImpl : OracleBase {
         sim.use_pradhana = c->use_pradhana; sim.use_mises_q = c->use_mises_q; sim.use_mibf = c->use_mibf;
         sim.use_musl = c->use_musl; sim.gravity_special = c->gravity_special;
         sim.change_particle_positions = c->change_particle_positions;
+        sim.pbc = c->pbc != 0;
         for (int d = 0; d < D; ++d) { sim.grid_reference_point(d) = c->grid_reference_point[d]; sim.gravity(d) = c->gravity[d]; }
         sim.fps = c->fps; sim.cfl = c->cfl; sim.cfl_elastic = c->cfl_elastic; sim.flip_ratio = c->flip_ratio;
         sim.rho = c->rho; sim.gravity_time = c->gravity_time;
@@ -173,7 +174,6 @@ template <int D, int SPL> struct OracleImpl : OracleBase {
 };
 
 OracleBase* make(const mpm_config* c) {
-    if (c->pbc) return nullptr;
     if (c->dim == 3 && c->spline_degree == 2) return new OracleImpl<3, 2>(c);
     if (c->dim == 3 && c->spline_degree == 3) return new OracleImpl<3, 3>(c);
     if (c->dim == 2 && c->spline_degree == 2) return new OracleImpl<2, 2>(c);

@@ -256,6 +256,7 @@ public:
     Grid<D> grid;
     unsigned Nx = 0, Ny = 0, Nz = 1, grid_nodes = 0;
     T min_init[3] = {0, 0, 0}, max_init[3] = {0, 0, 0}, low_init[3] = {0, 0, 0}, high_init[3] = {0, 0, 0};
+    unsigned num_add_pbc_particles = 0;  // simulation.hpp:68
 
     inline unsigned ind(unsigned i, unsigned j, unsigned k) const { return (i * Ny + j) * Nz + k; }  // simulation.hpp:197-199
     inline unsigned ind(unsigned i, unsigned j) const { return (i * Ny + j); }                         // simulation.hpp:201-203
@@ -325,14 +326,20 @@ public:
         while (frame < end_frame) { bool fe; if (!loopIteration(fe)) break; }
     }
 
-    // advanceStep(), simulation.cpp:219-273 (pbc is unsupported)
+    // advanceStep(), simulation.cpp:219-273
     void advanceStep() {
         updateDt();
-        if (current_time_step == 0) remeshFixedInit(2, 2, 2); else remeshFixedCont();
+        if (pbc) {
+            if (current_time_step == 0) remeshFixed(4);
+        } else {
+            if (current_time_step == 0) remeshFixedInit(2, 2, 2); else remeshFixedCont();
+        }
+        if (pbc) PBCAddParticles(4);
         resizeGrid();
         P2G();
         if (check_conservation) { checkMassConservation(); checkMomentumConservation(); }
         explicitEulerUpdate();
+        if (pbc) PBCDelParticles();
         G2P();
         if (use_musl) MUSL();
         deformationUpdate();
@@ -366,6 +373,52 @@ public:
         Nx = grid.x.size(); Ny = grid.y.size(); Nz = (D == 3) ? grid.z.size() : 1;
         grid_nodes = Nx * Ny * Nz;
         grid.xc = grid.x[0]; grid.yc = grid.y[0]; if (D == 3) grid.zc = grid.z[0];
+    }
+    // remeshFixed, remesh.cpp:14-36: the fixed grid of a periodic run, set once at the first step.
+    void remeshFixed(unsigned extra_nodes) {
+        grid.x = arange(-dx * (T)(1 + extra_nodes), Lx + (T)(2 + extra_nodes) * dx, dx);
+        grid.y = arange(-dx, Ly + (T)(2 + extra_nodes) * dx, dx);
+        if (D == 3) grid.z = arange(-dx * (T)(1 + extra_nodes), Lz + (T)(2 + extra_nodes) * dx, dx);
+        setGridSize();
+    }
+    // PBCAddParticles, pbc.cpp:37-83: a particle within safety_factor*dx of x = Lx (strictly inside) gets a
+    // copy at x - Lx, written -(Lx - x); otherwise one within that distance of x = 0 gets a copy at Lx + x.
+    // The copies are appended in particle order (the reference's OpenMP loop appends in thread order).
+    // The copy carries x, v, F and Bmat as in the reference, and also muI: the reference leaves muI
+    // unextended and its MIBF P2G reads muI[p] past the end for the copies (p2g.cpp:46,61), so the copy of
+    // the source value is this restatement's definition of that read.
+    void PBCAddParticles(unsigned safety_factor) {
+        num_add_pbc_particles = 0;
+        const unsigned n0 = Np;
+        const T band = (T)safety_factor * dx;
+        for (unsigned p = 0; p < n0; ++p) {
+            TV part_x = particles.x[p];
+            T diff = Lx - part_x(0);
+            bool add = false;
+            if (diff < band && diff > 0) { part_x(0) = -diff; add = true; }
+            else {
+                diff = part_x(0);
+                if (diff < band && diff >= 0) { part_x(0) = Lx + diff; add = true; }
+            }
+            if (!add) continue;
+            particles.x.push_back(part_x);
+            particles.v.push_back(particles.v[p]);
+            particles.F.push_back(particles.F[p]);
+            particles.Bmat.push_back(particles.Bmat[p]);
+            particles.muI.push_back(particles.muI[p]);
+            ++num_add_pbc_particles;
+        }
+        Np = (unsigned)particles.x.size();
+    }
+    // PBCDelParticles, pbc.cpp:85-101
+    void PBCDelParticles() {
+        const size_t k = num_add_pbc_particles;
+        particles.x.resize(particles.x.size() - k);
+        particles.v.resize(particles.v.size() - k);
+        particles.F.resize(particles.F.size() - k);
+        particles.Bmat.resize(particles.Bmat.size() - k);
+        particles.muI.resize(particles.muI.size() - k);
+        Np = (unsigned)particles.x.size();
     }
     // remeshFixedInit, remesh.cpp:38-182
     void remeshFixedInit(unsigned sfx, unsigned sfy, unsigned sfz) {
@@ -1292,12 +1345,16 @@ public:
         return err;
     }
 
-    // positionUpdate, position_update.cpp:6-46 (pbc unsupported)
+    // positionUpdate, position_update.cpp:6-46
     void positionUpdate() {
         #pragma omp parallel for num_threads(n_threads)
         for (long p = 0; p < (long)Np; ++p) {
             particles.x[p] = particles.x[p] + dt * particles.pic[p];
             if (!use_musl) velocityUpdate(p);
+            if (pbc) {  // position_update.cpp:26-33
+                if (particles.x[p](0) > Lx) particles.x[p](0) = particles.x[p](0) - Lx;
+                else if (particles.x[p](0) < 0) particles.x[p](0) = Lx + particles.x[p](0);
+            }
             if (change_particle_positions) {
                 if (particles.x[p](0) > 0.8) { particles.x[p](0) = -0.1; particles.x[p](1) += 0.27; }
             }

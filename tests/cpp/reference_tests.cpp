@@ -1,6 +1,6 @@
 // The reference's physics tests (tests/tests.cpp:13-736) written against the C++ host API of the engine
 // (matter-tests_amd/host/matter.hpp). Each test keeps the reference's parameters and tolerance; particle
-// set-up uses the jittered-lattice sampleParticles of matter.hpp in place of tph_poisson.
+// set-up uses the Poisson-disk sampleParticles of matter.hpp (Bridson's algorithm) in place of tph_poisson.
 //
 //   reference_tests [Suite.Name ...]     runs the named tests (all when none is given); exit code 0 = pass
 #include <cstdio>

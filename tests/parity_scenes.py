@@ -149,6 +149,44 @@ def gravity_ramp(dim=2, n=20):
     return cfg, plates, objs, f
 
 
+def periodic_layer(dim=2, spline=2, musl=False, flip=-2.0, cells_x=16, copies=1, model="DP", mibf=0, seed=11,
+                   cells_y=6, cells_z=4, ppc_axis=2, precision="fp64"):
+    """Layer of material filling the periodic interval 0 <= x < Lx (Lx = cells_x dx) on a SlipFree floor,
+    moving in +x at about 5 m/s under gravity, with seeded velocity and deformation perturbations, run with
+    periodic boundaries in x (pbc = 1; pbc.cpp, remesh.cpp:14-36, position_update.cpp:26-33).
+    copies > 1: the same layer repeated `copies` times along x without periodic boundaries (the unrolled
+    periodic problem), on a grid anchored at the origin so that its nodes sit at multiples of dx, as the
+    periodic grid's do. Returns (cfg, plates, objects, fields, Lx)."""
+    dx = 0.01
+    h = dx / ppc_axis
+    counts = [ppc_axis * cells_x, ppc_axis * cells_y] + ([ppc_axis * cells_z] if dim == 3 else [])
+    base = _block(dim, counts, h, [0.0] * dim, seed=seed)
+    rng = np.random.default_rng(seed)
+    npart = base.shape[0]
+    v = 0.2 * rng.standard_normal((npart, dim))
+    v[:, 0] += 5.0
+    F = np.tile(np.eye(dim), (npart, 1, 1)) + 0.01 * rng.standard_normal((npart, dim, dim))
+    Lx = cells_x * dx
+    cfg = base_cfg(dim, dx, ppc_axis, spline_degree=spline, plastic_model=model, M=math.tan(math.radians(30)),
+                   use_musl=int(musl), flip_ratio=flip, gravity=[0.0, -9.81, 0.0], end_frame=1, fps=1.0,
+                   grid_reference_point=[0.0, 0.0, 0.0], use_mibf=mibf, precision=precision)
+    if copies == 1:
+        cfg.pbc = 1
+        cfg.Lx, cfg.Ly, cfg.Lz = Lx, counts[1] * h + 2 * dx, (counts[2] * h if dim == 3 else 1.0)
+        x = base
+    else:
+        shift = np.zeros(dim)
+        shift[0] = Lx
+        x = np.concatenate([base + j * shift for j in range(copies)])
+        v, F = np.tile(v, (copies, 1)), np.tile(F, (copies, 1, 1))
+    return cfg, [A.make_plate(0.0, "bottom", "SlipFree", 0.3)], [], {"x": x, "v": v, "F": F}, Lx
+
+
+def periodic(dim=2, spline=2, musl=False, flip=-2.0, **kw):
+    cfg, plates, objs, f, _ = periodic_layer(dim, spline, musl, flip, **kw)
+    return cfg, plates, objs, f
+
+
 SCENES = {
     "dp3_usl_apic": lambda: dp_collapse(3, musl=False),
     "dp3_musl_apic": lambda: dp_collapse(3, musl=True),
@@ -186,6 +224,10 @@ SCENES = {
     "obj_box_rotated": lambda: object_scene("BoxRotated", "SlipFree"),
     "moving_plates": lambda: moving_plates(),
     "gravity_ramp_2d": lambda: gravity_ramp(),
+    "periodic2d_usl_apic": lambda: periodic(2),
+    "periodic2d_musl_picflip": lambda: periodic(2, musl=True, flip=0.95),
+    "periodic3d_usl_cubic_aflip_mibf": lambda: periodic(3, spline=3, flip=-1.0, mibf=1, cells_x=12),
+    "periodic3d_musl_apic_vm": lambda: periodic(3, musl=True, model="VM", cells_x=12),
 }
 
 

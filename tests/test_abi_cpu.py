@@ -63,9 +63,9 @@ def test_invalid_arguments_are_rejected_without_a_device():
     cfg.dim = 4
     assert lib.mpm_create(C.byref(cfg), C.byref(h)) == A.ERR["INVALID_ARGUMENT"]
     cfg.dim = 3
-    cfg.pbc = 1
-    assert lib.mpm_create(C.byref(cfg), C.byref(h)) == A.ERR["UNSUPPORTED"]
-    cfg.pbc = 0
+    cfg.pbc, cfg.Lx = 1, 0.0  # a periodic run needs its interval [0, Lx)
+    assert lib.mpm_create(C.byref(cfg), C.byref(h)) == A.ERR["INVALID_ARGUMENT"]
+    cfg.pbc, cfg.Lx = 0, 1.0
     cfg.plastic_model = A.PLASTIC["Snow"]
     assert lib.mpm_create(C.byref(cfg), C.byref(h)) == A.ERR["UNSUPPORTED"]
     cfg.plastic_model = A.PLASTIC["DP"]
