@@ -17,6 +17,7 @@ scenes (`batched`) and a served Poisson request stream (`served`, with p50/p99 s
     python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--no-cpu-baseline] [--extra c3,batched,served]
 """
 import argparse
+import ctypes as C
 import json
 import math
 import os
@@ -124,10 +125,26 @@ def cpu_baseline(cfg, plates, objs, x, steps=3):
     e.set_particles(x)
     info = A.StepInfo()
     lib.lib.oracle_time_steps(e.h, 1, info)
+    t0, t1 = (C.c_double * 4)(), (C.c_double * 4)()
+    lib.lib.oracle_phase_times(e.h, t0)
     secs = lib.lib.oracle_time_steps(e.h, steps, info)
+    lib.lib.oracle_phase_times(e.h, t1)
     n = x.shape[0]
+    phases = {k: (t1[i] - t0[i]) * 1e3 / steps for i, k in enumerate(("p2g", "euler", "g2p", "defgrad"))}
     return {"value": n * steps / secs, "unit": "particle-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps (after 1 warm-up step) of the same {n}-particle scene, {secs:.2f} s"}
+            "sample": f"{steps} steps (after 1 warm-up step) of the same {n}-particle scene, {secs:.2f} s",
+            "phase_ms_per_step": phases, "host_cpu": host_cpu(), "host_cores": os.cpu_count()}
+
+
+def host_cpu():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 COPY_BYTES = 256 * 1024 * 1024

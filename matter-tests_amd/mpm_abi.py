@@ -253,6 +253,8 @@ class Library:
             L.oracle_set_threads.argtypes = [vp, C.c_int]
             L.oracle_time_steps.argtypes = [vp, C.c_uint32, C.POINTER(StepInfo)]
             L.oracle_time_steps.restype = C.c_double
+            L.oracle_phase_times.argtypes = [vp, C.POINTER(C.c_double)]
+            L.oracle_phase_times.restype = None
             L.oracle_avg_data.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
             L.oracle_bulk_modulus.argtypes = [vp]
             L.oracle_bulk_modulus.restype = C.c_double

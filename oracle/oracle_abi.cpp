@@ -22,6 +22,7 @@ struct OracleBase {
     virtual void set_threads(int) = 0;
     virtual int avg_data(double*, double*, double*) = 0;
     virtual double bulk_modulus() const = 0;
+    virtual void phase_times(double*) const = 0;
 };
 
 template <int D, int SPL> struct OracleImpl : OracleBase {
@@ -171,6 +172,9 @@ template <int D, int SPL> struct OracleImpl : OracleBase {
         return 0;
     }
     double bulk_modulus() const override { return sim.calculateBulkModulus(); }
+    void phase_times(double* t) const override {
+        t[0] = sim.runtime_p2g; t[1] = sim.runtime_euler; t[2] = sim.runtime_g2p; t[3] = sim.runtime_defgrad;
+    }
 };
 
 OracleBase* make(const mpm_config* c) {
@@ -308,5 +312,9 @@ double oracle_time_steps(void* h, uint32_t nsteps, mpm_step_info* info) {
     auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
 }
+
+// Accumulated phase timers of advanceStep in seconds: P2G, explicitEulerUpdate, G2P, deformationUpdate
+// (simulation.cpp:237-266; the reference prints them at the end of simulate(), :205-209).
+void oracle_phase_times(void* h, double* out) { ((OracleBase*)h)->phase_times(out); }
 
 }  // extern "C"

@@ -24,6 +24,7 @@
 #pragma once
 #include "oracle_math.hpp"
 #include "../include/mpm.h"
+#include <chrono>
 #include <memory>
 #include <string>
 #include <cstdio>
@@ -257,6 +258,9 @@ public:
     unsigned Nx = 0, Ny = 0, Nz = 1, grid_nodes = 0;
     T min_init[3] = {0, 0, 0}, max_init[3] = {0, 0, 0}, low_init[3] = {0, 0, 0}, high_init[3] = {0, 0, 0};
     unsigned num_add_pbc_particles = 0;  // simulation.hpp:68
+    // phase timers of advanceStep (simulation.hpp:154-157, timer.hpp), in seconds
+    double runtime_p2g = 0, runtime_euler = 0, runtime_g2p = 0, runtime_defgrad = 0;
+    static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
     inline unsigned ind(unsigned i, unsigned j, unsigned k) const { return (i * Ny + j) * Nz + k; }  // simulation.hpp:197-199
     inline unsigned ind(unsigned i, unsigned j) const { return (i * Ny + j); }                         // simulation.hpp:201-203
@@ -336,13 +340,21 @@ public:
         }
         if (pbc) PBCAddParticles(4);
         resizeGrid();
+        double t = now_s();
         P2G();
+        runtime_p2g += now_s() - t;
         if (check_conservation) { checkMassConservation(); checkMomentumConservation(); }
+        t = now_s();
         explicitEulerUpdate();
+        runtime_euler += now_s() - t;
         if (pbc) PBCDelParticles();
+        t = now_s();
         G2P();
+        runtime_g2p += now_s() - t;
         if (use_musl) MUSL();
+        t = now_s();
         deformationUpdate();
+        runtime_defgrad += now_s() - t;
         positionUpdate();
         moveObjects();
     }
