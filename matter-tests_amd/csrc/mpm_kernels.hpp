@@ -1778,6 +1778,16 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Params P, StepState* S, PartStat
     g2p_body<D, SPL, FLIP, TF>(P, S, src, dst, sb, g, blockIdx.x);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2. For a kernel that walks the
+// block-sorted particles with one thread each, this maps the workgroups that land on one XCD to one
+// contiguous range of particles (an x-slab of the domain, with the same particle count on every XCD), so
+// that each L2 holds the grid nodes of one slab instead of the whole grid.
+__device__ __forceinline__ uint32_t xcd_slab(uint32_t w, uint32_t nwg) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t q = nwg / kXcd, rem = nwg % kXcd, x = w % kXcd, i = w / kXcd;
+    return x < rem ? x * (q + 1) + i : rem * (q + 1) + (x - rem) * q + i;
+}
+
 // ----------------------------------------------------------------------------------------------
 // k_defgrad (MUSL): deformationUpdate (deformation_update.cpp:16-46) on the re-scattered velocity.
 // One thread per sorted particle slot (the return map is long and register-heavy, so this kernel is
@@ -1816,7 +1826,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
 }
 template <int D, int SPL, class TF, int MAT>
 __global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
-    defgrad_body<D, SPL, TF, MAT>(P, S, src, dst, sb, g, n, blockIdx.x);
+    defgrad_body<D, SPL, TF, MAT>(P, S, src, dst, sb, g, n, xcd_slab(blockIdx.x, gridDim.x));
 }
 
 // ----------------------------------------------------------------------------------------------
