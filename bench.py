@@ -11,10 +11,11 @@ Prints one JSON line: throughput in particle-updates/s (particles x steps / devi
 the timed steps), the roofline of the dominant kernel from per-kernel HIP-event durations over the timed
 region, the split form's position error against the plain form, the parity of the same configuration
 against the CPU oracle at a small size, the oracle timed on the host cores on a bounded sample of the same
-workload (a reported baseline, not a target), and the extra workloads: the inclined flow (`c3`), 128 batched
-scenes (`batched`) and a served Poisson request stream (`served`, with p50/p99 step and request latency).
+workload (a reported baseline, not a target), and the extra workloads: the 2D column collapse run whole on the
+GPU and on the oracle (`c1`), the inclined flow (`c3`), 128 batched scenes (`batched`) and a served Poisson request
+stream (`served`, with p50/p99 step and request latency).
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--no-cpu-baseline] [--extra c3,batched,served]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--no-cpu-baseline] [--extra c1,c3,batched,served]
 """
 import argparse
 import ctypes as C
@@ -223,8 +224,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 256 MiB device copy first (counter runs)")
     ap.add_argument("--precision", choices=["both", "split", "fp64"], default="both",
                     help="forms to run; the first one run (split when both) gives the headline value")
-    ap.add_argument("--extra", default="c3,batched,served",
-                    help="comma-separated extra workloads reported beside the headline: c3 (inclined flow), batched "
+    ap.add_argument("--extra", default="c1,c3,batched,served",
+                    help="comma-separated extra workloads reported beside the headline: c1 (the 2D column collapse, "
+                         "GPU and oracle, 200 steps), c3 (inclined flow), batched "
                          "(128 scenes), served (Poisson request stream); empty for none")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-kernel HBM traffic summary of two rocprofv3 --pmc runs and exit")
@@ -270,6 +272,8 @@ def main():
             "after_steps": args.warmup + args.steps, "particles": n,
             "note": "same seeded input, both forms on the GPU; the plain form is held to the CPU oracle at 1e-10"}
     extra = [w for w in args.extra.split(",") if w]
+    if "c1" in extra:
+        result["c1"] = column2d_run(args)
     if "c3" in extra:
         result["c3"] = incline_run(args)
     if "batched" in extra:
@@ -284,6 +288,41 @@ def main():
         cfg64.precision = A.PRECISION["fp64"]
         result["cpu_baseline"] = cpu_baseline(cfg64, plates, objs, x)
     print(json.dumps(result), flush=True)
+
+
+def column2d_run(args, steps=200):
+    """The 2D plane-strain column collapse that the reference runs on the CPU: 0.5 x 0.5 m, dx = 0.01 m, 4 particles
+    per cell (10,000), quadratic, USL, APIC, Drucker-Prager 30 deg, NoSlip floor, 200 steps from rest, plain FP64.
+    The whole run on the GPU (one mpm_step call) and on the oracle (min(16, host cores) threads), and the
+    difference of the final states."""
+    cfg, plates, objs, x = S.column_2d()
+    n = x.shape[0]
+    out = {"workload": "2D plane-strain column collapse, DP 30 deg, APIC, USL, NoSlip floor, 200 steps",
+           "particles": n, "precision": "fp64", "steps": steps, "unit": "particle-updates/s"}
+    eng = A.Engine(A.product_library(), cfg, plates, objs)
+    eng.set_particles(x)
+    t0 = time.perf_counter()
+    rc, info = eng.step(steps)
+    t1 = time.perf_counter()
+    if rc or info.steps_done != steps:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], "2D column steps")
+    out.update(value=n * steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps)
+    g = eng.get_particles()
+    lib = A.oracle_library()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ocfg = type(cfg).from_buffer_copy(cfg)
+    ocfg.n_threads = threads
+    oe = A.Engine(lib, ocfg, plates, objs)
+    oe.set_particles(x)
+    oinfo = A.StepInfo()
+    secs = lib.lib.oracle_time_steps(oe.h, steps, oinfo)
+    o = oe.get_particles()
+    out["oracle"] = {"value": n * steps / secs, "cores": threads, "seconds": secs}
+    out["final_state_vs_oracle"] = {
+        "x_rel": float(np.linalg.norm(g["x"] - o["x"]) / np.linalg.norm(o["x"])),
+        "x_max_abs_m": float(np.max(np.abs(g["x"] - o["x"]))),
+        "time_equal": bool(info.time == oinfo.time), "plastic_count": [int(info.plastic_count), int(oinfo.plastic_count)]}
+    return out
 
 
 def incline_run(args):
