@@ -268,6 +268,12 @@ typedef struct mpm_kernel_timing {
  * runtime_p2g/euler/g2p/defgrad, simulation.hpp:154-158) with one timer per kernel on the context's stream. */
 int mpm_set_profiling(mpm_context* ctx, int enable);
 
+/* Enables HIP-event timing for the one kernel called `name` (e.g. "k_defgrad") and resets the accumulators:
+ * a timed run then carries two events per step instead of two per launch. No counterpart in the reference
+ * (its phase timers are always on); for callers that measure one kernel inside a throughput run.
+ * MPM_ERR_INVALID_ARGUMENT for an unknown name. */
+int mpm_set_profiling_kernel(mpm_context* ctx, const char* name);
+
 /* Copies the accumulated timings (one entry per kernel that ran) into out[0..capacity). */
 int mpm_get_kernel_times(mpm_context* ctx, mpm_kernel_timing* out, uint32_t capacity, uint32_t* count);
 

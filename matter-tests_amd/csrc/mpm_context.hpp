@@ -44,6 +44,7 @@ struct mpm_context {
     uint32_t est_n[3] = {1, 1, 1};
     // per-kernel HIP-event timing (mpm_set_profiling)
     bool profiling = false;
+    uint32_t prof_mask = ~0u;  // kernels (bit KS_*) timed while profiling
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, size_t>> ev_pending;  // (kernel id, index of start event)
     double kt_ms[16] = {0};
@@ -77,7 +78,7 @@ enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P, KS_MUSL_SCATTER, KS_
 // Records an event pair around one launch when profiling; returns the launch's slot.
 struct LaunchTimer {
     mpm_context* c; int id; size_t slot = 0; bool on;
-    LaunchTimer(mpm_context* c_, int id_) : c(c_), id(id_), on(c_->profiling) {
+    LaunchTimer(mpm_context* c_, int id_) : c(c_), id(id_), on(c_->profiling && ((c_->prof_mask >> id_) & 1u)) {
         if (!on) return;
         slot = c->ev_pending.size() * 2;
         while (c->ev_pool.size() < slot + 2) { hipEvent_t e; if (hipEventCreate(&e) != hipSuccess) { on = false; return; } c->ev_pool.push_back(e); }

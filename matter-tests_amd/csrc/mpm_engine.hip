@@ -908,6 +908,18 @@ int mpm_set_profiling(mpm_context* c, int enable) {
     c->ev_pending.clear();
     for (int k = 0; k < KS_COUNT; ++k) { c->kt_ms[k] = 0; c->kt_n[k] = 0; }
     c->profiling = enable != 0;
+    c->prof_mask = ~0u;
+    return 0;
+}
+
+int mpm_set_profiling_kernel(mpm_context* c, const char* name) {
+    if (!c || !name) return MPM_ERR_INVALID_ARGUMENT;
+    int id = -1;
+    for (int k = 0; k < KS_COUNT; ++k) if (std::strcmp(name, kKernelNames[k]) == 0) id = k;
+    if (id < 0) return MPM_ERR_INVALID_ARGUMENT;
+    const int rc = mpm_set_profiling(c, 1);
+    if (rc) return rc;
+    c->prof_mask = 1u << id;
     return 0;
 }
 

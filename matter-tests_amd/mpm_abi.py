@@ -210,6 +210,7 @@ class Library:
             L.mpm_status_string.restype = C.c_char_p
             L.mpm_config_defaults.argtypes = [C.POINTER(Config)]
             L.mpm_set_profiling.argtypes = [vp, C.c_int]
+            L.mpm_set_profiling_kernel.argtypes = [vp, C.c_char_p]
             L.mpm_get_binning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]
             L.mpm_get_kernel_times.argtypes = [vp, C.POINTER(KernelTiming), C.c_uint32, C.POINTER(C.c_uint32)]
             L.mpm_abi_version.restype = C.c_int
@@ -405,6 +406,10 @@ class Engine:
 
     def set_profiling(self, enable=True):
         self._check(self.L.lib.mpm_set_profiling(self.h, 1 if enable else 0), "set_profiling")
+
+    def set_profiling_kernel(self, name):
+        """Times only the kernel `name` (two HIP events per step instead of two per launch)."""
+        self._check(self.L.lib.mpm_set_profiling_kernel(self.h, name.encode()), "set_profiling_kernel")
 
     def kernel_times(self):
         """{kernel name: (total_ms, launches)} accumulated since set_profiling(True)."""
