@@ -1568,7 +1568,9 @@ template <int D, class TF> struct GridFetch {
         const uint32_t node = node_index<D>(b0 + a, b1 + b, D == 3 ? b2 + c : 0, n);
         for (int d = 0; d < D; ++d) vi[d] = g.v[d][node];
     }
-    __device__ __forceinline__ TF flip(int, int, int, int) const { return TF(0); }
+    __device__ __forceinline__ TF flip(int a, int b, int c, int d) const {
+        return g.flip[d][node_index<D>(b0 + a, b1 + b, D == 3 ? b2 + c : 0, n)];
+    }
 };
 
 // Gather of G2P (g2p.cpp:35-67) or of the velocity gradient (deformation_update.cpp:217-230) over the
@@ -1720,6 +1722,38 @@ __device__ __forceinline__ bool block_range(const StepState* S, const SortBufs& 
     return true;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2. For a kernel that walks the
+// block-sorted particles with one thread each, this maps the workgroups that land on one XCD to one
+// contiguous range of particles (an x-slab of the domain, with the same particle count on every XCD), so
+// that each L2 holds the grid nodes of one slab instead of the whole grid.
+__device__ __forceinline__ uint32_t xcd_slab(uint32_t w, uint32_t nwg) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t q = nwg / kXcd, rem = nwg % kXcd, x = w % kXcd, i = w / kXcd;
+    return x < rem ? x * (q + 1) + i : rem * (q + 1) + (x - rem) * q + i;
+}
+
+// Kernels with one thread per sorted particle slot (k_defgrad) gather from LDS copies of the node
+// tiles of the blocks that their workgroup's slots lie in. Slots are in block order, so these are the blocks
+// from the first slot's to the last slot's; at 8 particles per cell a block holds about 440 particles and a
+// workgroup of 256 slots spans one or two. SlotTiles::n tiles (at most 4, and at most about 32 KB of LDS, so
+// that four workgroups fit on a CU) are copied; a particle whose block is beyond them gathers from the grid
+// arrays, with the same values, weights and order.
+template <int D, int SPL, class TF, int SETS> struct SlotTiles {
+    static constexpr int bytes = SETS * 3 * BG<D, SPL>::NT * (int)sizeof(TF);
+    static constexpr int n = (32768 / bytes) < 1 ? 1 : ((32768 / bytes) > 4 ? 4 : (32768 / bytes));
+};
+// First block of the slots base .. min(n, base + blockDim.x) - 1 and the number of blocks, up to cap, from it.
+__device__ __forceinline__ void slot_block_span(const SortBufs& sb, uint32_t base, uint32_t n, uint32_t cap, uint32_t& blo, uint32_t& nt) {
+    __shared__ uint32_t s_blo, s_nt;
+    if (threadIdx.x == 0) {
+        const uint32_t lo = sb.key[sb.perm[base]], hi = sb.key[sb.perm[min(n, base + blockDim.x) - 1]];
+        s_blo = lo;
+        s_nt = min(cap, hi - lo + 1);
+    }
+    __syncthreads();
+    blo = s_blo; nt = s_nt;
+}
+
 // ----------------------------------------------------------------------------------------------
 // k_g2p: G2P (g2p.cpp:17-77), the particle velocity update (USL: position_update.cpp:14-23; MUSL: the
 // same formula in musl.cpp:13-25) and the position update (position_update.cpp:12). The deformation
@@ -1779,48 +1813,28 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Params P, StepState* S, PartStat
     g2p_body<D, SPL, FLIP, TF>(P, S, src, dst, sb, g, blockIdx.x);
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2. For a kernel that walks the
-// block-sorted particles with one thread each, this maps the workgroups that land on one XCD to one
-// contiguous range of particles (an x-slab of the domain, with the same particle count on every XCD), so
-// that each L2 holds the grid nodes of one slab instead of the whole grid.
-__device__ __forceinline__ uint32_t xcd_slab(uint32_t w, uint32_t nwg) {
-    constexpr uint32_t kXcd = 8;
-    const uint32_t q = nwg / kXcd, rem = nwg % kXcd, x = w % kXcd, i = w / kXcd;
-    return x < rem ? x * (q + 1) + i : rem * (q + 1) + (x - rem) * q + i;
-}
-
 // ----------------------------------------------------------------------------------------------
 // k_defgrad: deformationUpdate (deformation_update.cpp:16-46) with the return map, on the velocity of the
 // step's grid update (USL) or of the MUSL re-scatter. One thread per sorted particle slot (the return map is
-// long and register-heavy, so the kernel keeps one particle per thread and no loop). The 256 slots of a
-// workgroup lie in a few consecutive blocks (about 440 particles per block at 8 per cell), and the
-// workgroup first copies the node-velocity tiles of up to kDefgradTiles of them to LDS; a particle gathers
-// grad v from its block's tile, or from the grid arrays when its block is beyond those (a gather from the
-// grid arrays alone took about 40 % of the kernel). Both read the same values with the same weights in the
-// same order.
+// long and register-heavy, so the kernel keeps one particle per thread and no loop); grad v is gathered from
+// the LDS tiles of the workgroup's blocks (SlotTiles; a gather from the grid arrays alone took about 40 % of
+// the kernel).
 // ----------------------------------------------------------------------------------------------
-constexpr int kDefgradTiles = 4;
-
 template <int D, int SPL, class TF, int MAT>
 __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                              const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t n, uint32_t bid) {
     using G = BG<D, SPL>;
-    __shared__ TF s_v[kDefgradTiles][3 * G::NT];
-    __shared__ BlockCoordLDS<D, SPL> s_c[kDefgradTiles];
-    __shared__ uint32_t s_blo, s_nt;
+    constexpr int NTL = SlotTiles<D, SPL, TF, 1>::n;
+    __shared__ TF s_v[NTL][3 * G::NT];
+    __shared__ BlockCoordLDS<D, SPL> s_c[NTL];
     if (S->stop) return;
     const uint32_t base = bid * blockDim.x;
     if (base >= n) return;
     const uint32_t i = base + threadIdx.x;
     const bool active = i < n;
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
-    if (threadIdx.x == 0) {  // blocks of the first and last slot (slots are in block order)
-        const uint32_t blo = sb.key[sb.perm[base]], bhi = sb.key[sb.perm[min(n, base + blockDim.x) - 1]];
-        s_blo = blo;
-        s_nt = min((uint32_t)kDefgradTiles, bhi - blo + 1);
-    }
-    __syncthreads();
-    const uint32_t blo = s_blo, nt = s_nt;
+    uint32_t blo, nt;
+    slot_block_span(sb, base, n, (uint32_t)NTL, blo, nt);
     for (uint32_t t = 0; t < nt; ++t) {
         if (sb.count[blo + t] == 0) continue;
         int bc[3];
