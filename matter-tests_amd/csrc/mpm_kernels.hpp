@@ -554,28 +554,60 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState*
     bin_count_body<D, SPL, TF>(P, S, src, sb, g, n, blockIdx.x);
 }
 
-// k_bin_scan: exclusive prefix sum of the block histogram; one workgroup of 1024 threads, each owning
-// a contiguous run of blocks.
+// k_bin_scan: exclusive prefix sum of the block histogram; one workgroup of 1024 threads, each owning a
+// contiguous run of blocks. The run's counts are read kScanLoads at a time as independent loads (one memory
+// latency per group instead of one per block), the run sums are scanned within each wavefront by shuffles and
+// across the 16 wavefronts by the first one.
+constexpr int kScanLoads = 16;
+__device__ __forceinline__ uint32_t run_sum(const uint32_t* count, uint32_t b0, uint32_t b1) {
+    uint32_t s = 0;
+    for (uint32_t c0 = b0; c0 < b1; c0 += kScanLoads) {
+        uint32_t v[kScanLoads];
+#pragma unroll
+        for (int k = 0; k < kScanLoads; ++k) v[k] = c0 + k < b1 ? count[c0 + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < kScanLoads; ++k) s += v[k];
+    }
+    return s;
+}
 __device__ __forceinline__ void bin_scan_body(StepState* S, const SortBufs& sb) {
     if (S->stop) return;
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t wsum[16];
     const uint32_t nb = S->nblocks << sb.ghosts;
     const uint32_t per = (nb + 1023) / 1024;
-    const uint32_t b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
-    uint32_t s = 0;
-    for (uint32_t b = b0; b < b1; ++b) s += sb.count[b];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan of the run sums
-        const uint32_t t = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += t;
-        __syncthreads();
+    const uint32_t b0 = min(nb, threadIdx.x * per), b1 = min(nb, b0 + per);
+    const uint32_t s = run_sum(sb.count, b0, b1);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t incl = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
     }
-    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-    for (uint32_t b = b0; b < b1; ++b) {
-        if (sb.ghosts && b == S->nblocks) S->n_ghosts = part[1023] - run;  // entries from here on are copies
-        sb.offset[b] = run; sb.cursor[b] = run; run += sb.count[b];
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+        uint32_t v = lane < 16 ? wsum[lane] : 0u;
+        for (int o = 1; o < 16; o <<= 1) {
+            const uint32_t t = __shfl_up(v, o, 64);
+            if (lane >= o) v += t;
+        }
+        if (lane < 16) wsum[lane] = v;  // inclusive over the wavefronts
+    }
+    __syncthreads();
+    const uint32_t total = wsum[15];
+    uint32_t run = (wid ? wsum[wid - 1] : 0u) + incl - s;
+    for (uint32_t c0 = b0; c0 < b1; c0 += kScanLoads) {
+        uint32_t v[kScanLoads];
+#pragma unroll
+        for (int k = 0; k < kScanLoads; ++k) v[k] = c0 + k < b1 ? sb.count[c0 + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < kScanLoads; ++k) {
+            const uint32_t b = c0 + k;
+            if (b < b1) {
+                if (sb.ghosts && b == S->nblocks) S->n_ghosts = total - run;  // entries from here on are copies
+                sb.offset[b] = run; sb.cursor[b] = run; run += v[k];
+            }
+        }
     }
 }
 template <int = 0>  // a template, so that every translation unit may instantiate it
