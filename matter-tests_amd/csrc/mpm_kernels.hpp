@@ -649,7 +649,11 @@ __global__ __launch_bounds__(kBlock) void k_binning(Params P, const StepState* S
 // ----------------------------------------------------------------------------------------------
 template <int D, int SPL, int NV, class TF> struct ScatterShape {
     using G = BG<D, SPL>;
-    static constexpr int REGS = 40;                                    // 32-bit registers for partial sums
+    // 32-bit registers for the partial sums. The quadratic P2G scatter (NV > D) takes enough for one
+    // wavefront to cover all or most of a particle's window slices, which saves the repeated stencil and
+    // particle loads of the other wavefronts (C2 split: k_p2g 289 -> 234 us at 229 VGPRs); the MUSL
+    // re-scatter (NV == D) and the cubic scatters were measured as fast or faster with 40.
+    static constexpr int REGS = (NV > D && SPL == 2) ? 108 : 40;
     static constexpr int WORDS = NV * (int)(sizeof(TF) / 4);
     static constexpr int SPP0 = REGS / (G::NS * WORDS);
     static constexpr int SPP = SPP0 < 1 ? 1 : (SPP0 > G::W ? G::W : SPP0);   // slices per wavefront
