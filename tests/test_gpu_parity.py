@@ -161,3 +161,34 @@ def test_binning_bit_exact(engine, oracle, name):
     same = np.all(x == xo, axis=1)
     refo = (np.floor((xo - origin) * h1) - 1).astype(np.int64)
     np.testing.assert_array_equal(ref[same], refo[same])
+
+
+@pytest.mark.parametrize("dim,precision", [(3, "fp64"), (2, "fp64"), (3, "split")])
+def test_sparse_blocks(engine, oracle, dim, precision):
+    """About one particle per block: the 256 particles of a k_defgrad workgroup lie in about 256 blocks, more
+    than it copies to LDS, so most of them gather grad v from the grid arrays instead of a tile. Held to the
+    oracle at the FP64 parity bar (plain form) or the split form's position bar."""
+    print(f"[parity] sparse blocks dim={dim} {precision}", flush=True)
+    dx = 0.01
+    edge = 4 if dim == 3 else 8                      # cells per block edge
+    n = 12 if dim == 3 else 20
+    sp = (edge + 0.5) * dx
+    idx = np.indices([n] * dim).reshape(dim, -1).T.astype(np.float64)
+    x = 0.02 + idx * sp + 0.1 * dx * np.sin(np.arange(idx.size).reshape(idx.shape))
+    cfg = PS.base_cfg(dim, dx, 2, plastic_model="DP", M=np.tan(np.radians(30)), flip_ratio=-2.0,
+                      gravity=[0.0, -9.81, 0.0], end_frame=1, fps=1.0, precision=precision)
+    v = 0.5 * np.cos(np.arange(x.size).reshape(x.shape))
+    plates = [A.make_plate(0.0, "bottom", "NoSlip")]
+    ge, oe = A.Engine(engine, cfg, plates), A.Engine(oracle, cfg, plates)
+    ge.set_particles(x, v=v)
+    oe.set_particles(x, v=v)
+    rg, ig = ge.step(5)
+    ro, io = oe.step(5)
+    assert rg == ro == 0, (A.status_names(rg), A.status_names(ro))
+    if precision == "fp64":
+        compare_state(ge, oe, f"sparse blocks {dim}D", io.dt)
+    else:
+        gp, op = ge.get_particles(), oe.get_particles()
+        err = float(np.linalg.norm(gp["x"] - op["x"]) / np.linalg.norm(op["x"]))
+        assert err < 1e-5, err
+        assert rel_err(gp["F"], op["F"], 1.0) < 1e-5
