@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void kb_grid_update(BatchTable T) {
 }
 
 template <int D, int SPL, bool FLIP, class TF>
-__global__ __launch_bounds__(kBlock) void kb_g2p(BatchTable T) {
+__global__ __launch_bounds__(kG2PThreads) void kb_g2p(BatchTable T) {
     BatchSlot<TF> q(T, BR_BLOCK);
     g2p_body<D, SPL, FLIP, TF>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.local);
 }

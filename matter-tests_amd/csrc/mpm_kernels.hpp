@@ -1795,6 +1795,10 @@ __device__ __forceinline__ void slot_block_span(const SortBufs& sb, uint32_t bas
 // same formula in musl.cpp:13-25) and the position update (position_update.cpp:12). The deformation
 // update, which reads the particles' positions of the step's start, follows in k_defgrad.
 // ----------------------------------------------------------------------------------------------
+// Threads of a k_g2p workgroup (one workgroup per block, each thread a particle at a time): one wavefront
+// leaves fewer threads idle in a block's last pass (a block holds about 440 particles at 8 per cell).
+constexpr int kG2PThreads = 64;
+
 template <int D, int SPL, bool FLIP, class TF>
 __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                               const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b) {
@@ -1845,7 +1849,7 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
 template <int D, int SPL, bool FLIP, class TF>
-__global__ __launch_bounds__(kBlock) void k_g2p(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
+__global__ __launch_bounds__(kG2PThreads) void k_g2p(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
     g2p_body<D, SPL, FLIP, TF>(P, S, src, dst, sb, g, blockIdx.x);
 }
 

@@ -70,8 +70,8 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     else enqueue_p2g<D, SPL, false, false, TF>(c, src, dst, gblocks);
     {
         LaunchTimer t(c, KS_G2P);
-        if (P.flip_ratio >= -1) k_g2p<D, SPL, true, TF><<<c->act_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
-        else k_g2p<D, SPL, false, TF><<<c->act_blocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g);
+        if (P.flip_ratio >= -1) k_g2p<D, SPL, true, TF><<<c->act_blocks, kG2PThreads, 0, s>>>(P, c->S, src, dst, c->sb, g);
+        else k_g2p<D, SPL, false, TF><<<c->act_blocks, kG2PThreads, 0, s>>>(P, c->S, src, dst, c->sb, g);
     }
     if (P.use_musl) {
         constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
@@ -105,8 +105,8 @@ void enqueue_batch_step(const BatchClass& bc, hipStream_t s) {
     else if (k.sep) enqueue_batch_p2g<D, SPL, true, false, TF>(bc, s);
     else if (k.mibf) enqueue_batch_p2g<D, SPL, false, true, TF>(bc, s);
     else enqueue_batch_p2g<D, SPL, false, false, TF>(bc, s);
-    if (k.flip) kb_g2p<D, SPL, true, TF><<<bc.wg[BR_BLOCK], kBlock, 0, s>>>(T);
-    else kb_g2p<D, SPL, false, TF><<<bc.wg[BR_BLOCK], kBlock, 0, s>>>(T);
+    if (k.flip) kb_g2p<D, SPL, true, TF><<<bc.wg[BR_BLOCK], kG2PThreads, 0, s>>>(T);
+    else kb_g2p<D, SPL, false, TF><<<bc.wg[BR_BLOCK], kG2PThreads, 0, s>>>(T);
     if (k.musl) {
         constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
         kb_musl_scatter<D, SPL, TF><<<bc.wg[BR_BLOCK], TBM, 0, s>>>(T);
