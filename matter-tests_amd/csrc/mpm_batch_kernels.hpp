@@ -1,8 +1,9 @@
 // mpm_batch_kernels.hpp — batched launches: many independent scenes advanced by one launch per kernel.
 //
 // The scenes of a batch that share dimension, spline degree, transfer type and the compile-time switches
-// of the step (separate force, MIBF, material, MUSL) form a class. Each kernel of the step is launched once
-// per class over the concatenated workgroup ranges of its scenes: workgroup w belongs to the scene s with
+// of the step (separate force, MIBF, material, MUSL, periodic boundaries) form a class. Each kernel of the
+// step is launched once per class over the concatenated workgroup ranges of its scenes: workgroup w belongs
+// to the scene s with
 // start[s] <= w < start[s + 1] (a binary search over the class's range table, uniform across the
 // workgroup), and runs the same body as the single-scene kernel on that scene's constants, loop state and
 // buffers with its local workgroup index. Every scene keeps its own StepState, so dt, grid geometry,
@@ -112,10 +113,10 @@ __global__ __launch_bounds__(kBlock) void kb_bin_scatter(BatchTable T) {
     bin_scatter_body(q.r->S, q.r->sb, q.r->n, q.local);
 }
 
-template <int D, int SPL, bool SEP, bool MIBF, class TF>
+template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
 __global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB)) void kb_p2g(BatchTable T) {
     BatchSlot<TF> q(T, BR_BLOCK);
-    p2g_body<D, SPL, SEP, MIBF, TF>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.r->stg, q.local);
+    p2g_body<D, SPL, SEP, MIBF, TF, PBC>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.r->stg, q.local);
 }
 
 template <int D, int SPL, bool SEP, bool MIBF, class TF>

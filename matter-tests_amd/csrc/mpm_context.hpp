@@ -116,10 +116,10 @@ inline int material_of(const Params& P) {
 
 // Compile-time switches shared by the scenes of one batched launch.
 struct BatchClassKey {
-    int D, SPL, split, sep, flip, mibf, musl, mat;
+    int D, SPL, split, sep, flip, mibf, musl, mat, pbc;
     bool operator==(const BatchClassKey& o) const {
         return D == o.D && SPL == o.SPL && split == o.split && sep == o.sep && flip == o.flip && mibf == o.mibf && musl == o.musl &&
-               mat == o.mat;
+               mat == o.mat && pbc == o.pbc;
     }
 };
 

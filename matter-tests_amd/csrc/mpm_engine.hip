@@ -335,6 +335,7 @@ BatchClassKey class_key(const mpm_context* c, bool check_cons) {
     k.sep = (c->P.flip_ratio >= -1 || check_cons) ? 1 : 0; k.flip = c->P.flip_ratio >= -1 ? 1 : 0;
     k.mibf = c->P.use_mibf ? 1 : 0; k.musl = c->P.use_musl ? 1 : 0;
     k.mat = material_of(c->P);
+    k.pbc = c->cfg.pbc ? 1 : 0;
     return k;
 }
 
@@ -440,7 +441,9 @@ int rebuild_tables(mpm_batch* b, bool check_cons) {
             mpm_context* c = bc.members[i];
             if (bc.key.split) { SceneRef<float> r = scene_ref<float>(c); std::memcpy(host.data() + ref_off[k] + i * sizeof(r), &r, sizeof(r)); }
             else { SceneRef<double> r = scene_ref<double>(c); std::memcpy(host.data() + ref_off[k] + i * sizeof(r), &r, sizeof(r)); }
-            const uint32_t wg[BR_KINDS] = {(c->n + kBlock - 1) / kBlock, c->act_blocks, grid_workgroups(c), (c->act_blocks + kBlock - 1) / kBlock};
+            // particle kernels cover the periodic copies' entries too (k_bin_scatter); the others return early there
+            const uint32_t wg[BR_KINDS] = {(uint32_t)((((uint64_t)c->n << c->sb.ghosts) + kBlock - 1) / kBlock), c->act_blocks,
+                                           grid_workgroups(c), (c->act_blocks + kBlock - 1) / kBlock};
             for (int r = 0; r < BR_KINDS; ++r) st[r * (m + 1) + i + 1] = st[r * (m + 1) + i] + wg[r];
         }
         for (int r = 0; r < BR_KINDS; ++r) bc.wg[r] = st[r * (m + 1) + m];
@@ -986,7 +989,6 @@ void mpm_batch_destroy(mpm_batch* b) {
 int mpm_batch_add(mpm_batch* b, mpm_context* c, uint32_t nsteps) {
     if (!b || !c || c->batch) return MPM_ERR_INVALID_ARGUMENT;
     if (!c->ready) return MPM_ERR_NOT_READY;
-    if (c->cfg.pbc) return MPM_ERR_UNSUPPORTED;  // periodic scenes step alone (mpm_step)
     if (c->device != b->device) return MPM_ERR_INVALID_ARGUMENT;
     if (hipSetDevice(b->device) != hipSuccess) return MPM_ERR_HIP;
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -50,7 +50,8 @@ uint64_t estimate_bytes(const mpm_config& c, uint32_t n, const uint32_t* span, u
     const bool split = c.precision == MPM_PRECISION_SPLIT;
     const size_t tf = split ? sizeof(float) : sizeof(double);
     const size_t nd = D + D * D + 6, nt = D + D * D + (D == 3 ? 6 : 3);
-    uint64_t bytes = 2 * (nd * sizeof(double) + nt * tf) * ((uint64_t)n + 1024) + (uint64_t)16 * n;
+    const uint64_t pbc = c.pbc ? 1 : 0;  // periodic: key and perm and the block arrays also hold the copies
+    uint64_t bytes = 2 * (nd * sizeof(double) + nt * tf) * ((uint64_t)n + 1024) + (uint64_t)(16 + 8 * pbc) * n;
     uint64_t nodes = 1, blocks = 1;
     const uint32_t E = D == 3 ? 4 : 8;
     for (int d = 0; d < D; ++d) {
@@ -60,7 +61,7 @@ uint64_t estimate_bytes(const mpm_config& c, uint32_t n, const uint32_t* span, u
     }
     const uint32_t W = c.spline_degree == 2 ? 3 : 4, T = E + W - 1;
     const uint64_t NT = D == 3 ? (uint64_t)T * T * T : (uint64_t)T * T;
-    bytes += (uint64_t)(1 + 2 * D + 1) * nodes * tf + blocks * (12 + 130 + mpm::kMaxNV * NT * tf);
+    bytes += (uint64_t)(1 + 2 * D + 1) * nodes * tf + blocks * (12 * (1 + pbc) + 130 + mpm::kMaxNV * NT * tf);
     return bytes;
 }
 
@@ -132,7 +133,6 @@ int mpm_server_submit(mpm_server* s, const mpm_request_desc* q, uint64_t* id) {
     if ((q->n_plates && !q->plates) || (q->n_objects && !q->objects)) return MPM_ERR_INVALID_ARGUMENT;
     const int D = q->cfg->dim;
     if ((D != 2 && D != 3) || !(q->cfg->dx > 0)) return MPM_ERR_INVALID_ARGUMENT;
-    if (q->cfg->pbc) return MPM_ERR_UNSUPPORTED;  // periodic scenes step alone (mpm_step), not in a batch
     std::unique_ptr<Request> r(new (std::nothrow) Request());
     if (!r) return MPM_ERR_OUT_OF_MEMORY;
     r->id = s->reqs.size();
@@ -146,7 +146,13 @@ int mpm_server_submit(mpm_server* s, const mpm_request_desc* q, uint64_t* id) {
     const size_t per[10] = {(size_t)D, (size_t)D, (size_t)D * D, (size_t)D * D, 1, 1, 1, 1, 1, 1};
     for (int k = 0; k < 10; ++k) if (src[k]) r->f[k].assign(src[k], src[k] + per[k] * q->n);
     uint32_t span[3] = {0, 0, 0};
-    for (int d = 0; d < D; ++d) {
+    for (int d = 0; d < D && q->cfg->pbc; ++d) {  // the fixed grid of a periodic run (remeshFixed(4))
+        const double L = d == 0 ? q->cfg->Lx : (d == 1 ? q->cfg->Ly : q->cfg->Lz);
+        const double sp = L / q->cfg->dx + (d == 1 ? 7.0 : 11.0);
+        if (!(sp < 4.0e6) || !(L > 0)) return MPM_ERR_INVALID_ARGUMENT;
+        span[d] = (uint32_t)sp;
+    }
+    for (int d = 0; d < D && !q->cfg->pbc; ++d) {
         double mn = a.x[d], mx = a.x[d];
         for (uint32_t p = 1; p < q->n; ++p) { const double v = a.x[(size_t)p * D + d]; mn = std::min(mn, v); mx = std::max(mx, v); }
         const double sp = (mx - mn) / q->cfg->dx;

@@ -88,7 +88,8 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
 template <int D, int SPL, bool SEP, bool MIBF, class TF>
 void enqueue_batch_p2g(const BatchClass& bc, hipStream_t s) {
     constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB;
-    kb_p2g<D, SPL, SEP, MIBF, TF><<<bc.wg[BR_BLOCK], TB, 0, s>>>(bc.T);
+    if (bc.key.pbc) kb_p2g<D, SPL, SEP, MIBF, TF, true><<<bc.wg[BR_BLOCK], TB, 0, s>>>(bc.T);
+    else kb_p2g<D, SPL, SEP, MIBF, TF, false><<<bc.wg[BR_BLOCK], TB, 0, s>>>(bc.T);
     kb_grid_update<D, SPL, SEP, MIBF, TF><<<bc.wg[BR_GRID], kBlock, 0, s>>>(bc.T);
 }
 

@@ -8,7 +8,7 @@ test_gpu_parity.py. Here:
   against the oracle;
 * the conservation checks count the periodic copies, as the reference's Np does while they exist;
 * the split form against the oracle, with positions compared along the periodic axis modulo Lx;
-* batches and the server refuse periodic scenes (they step alone).
+* in a batch, and as requests of the server, a periodic scene's state equals the scene advanced alone.
 """
 import numpy as np
 import pytest
@@ -99,18 +99,70 @@ def test_split_form_against_oracle(engine, oracle):
     assert err < 1e-5
 
 
-def test_batches_refuse_periodic_scenes(engine):
-    cfg, plates, objs, f = PS.SCENES["periodic2d_usl_apic"]()
-    e = make(engine, cfg, plates, objs, f)
-    b = A.Batch(engine)
-    with pytest.raises(A.MPMError) as ex:
-        b.add(e, 5)
-    assert ex.value.code == A.ERR["UNSUPPORTED"]
-    b.close()
-    srv = A.Server(engine)
-    with pytest.raises(A.MPMError) as ex:
-        srv.submit(cfg, plates, objs, f["x"], 5, 0.0)
-    assert ex.value.code == A.ERR["UNSUPPORTED"]
+def periodic_mix():
+    """Periodic scenes of several launch classes (2D/3D, USL/MUSL, quadratic/cubic, MIBF, split form) and one
+    scene without periodic boundaries that shares the other switches of a periodic one."""
+    out = [(n, *PS.SCENES[n]()) for n in ("periodic2d_usl_apic", "periodic2d_musl_picflip", "periodic3d_musl_apic_vm",
+                                          "periodic3d_usl_cubic_aflip_mibf")]
+    cfg, plates, objs, f, _ = PS.periodic_layer(dim=3, cells_x=12, precision="split")
+    out.append(("periodic3d_split", cfg, plates, objs, f))
+    cfg, plates, objs, f, _ = PS.periodic_layer(dim=2, copies=2)  # the same layer unrolled, no periodic boundaries
+    out.append(("unrolled2d", cfg, plates, objs, f))
+    return out
+
+
+FIELDS = ("x", "v", "F", "Bmat", "eps_pl_dev", "eps_pl_vol", "delta_gamma", "muI", "viscosity")
+
+
+@pytest.mark.parametrize("use_graph", [True, False], ids=["graph", "direct"])
+def test_batched_periodic_equals_alone(engine, use_graph):
+    """A batch of periodic scenes (and one without): each member's state equals the scene advanced alone,
+    bit for bit, as for every other batch member."""
+    print(f"periodic batch vs alone, graph={use_graph}", flush=True)
+    scenes = periodic_mix()
+    steps = 12
+    alone = []
+    for name, cfg, plates, objs, f in scenes:
+        e = make(engine, cfg, plates, objs, f)
+        rc, info = e.step(steps)
+        assert rc == 0, (name, A.status_names(rc))
+        alone.append((e, info))
+    batch = A.Batch(engine, chunk_steps=5, use_graph=use_graph)
+    members = []
+    for name, cfg, plates, objs, f in scenes:
+        e = make(engine, cfg, plates, objs, f)
+        batch.add(e, steps)
+        members.append(e)
+    rc, binfo = batch.step(1000)
+    assert rc == 0, A.status_names(rc)
+    assert binfo.n_active == 0
+    for (name, *_), m, (a, ia) in zip(scenes, members, alone):
+        im, left = A.Batch.scene_info(m)
+        assert left == 0 and (im.time, im.step, im.plastic_count) == (ia.time, ia.step, ia.plastic_count), name
+        pm, pa = m.get_particles(), a.get_particles()
+        for k in FIELDS:
+            assert np.array_equal(pm[k], pa[k], equal_nan=True), f"{name}: {k} differs between batched and alone"
+        gm, ga = m.get_grid(), a.get_grid()
+        assert np.array_equal(gm["mass"], ga["mass"]) and np.array_equal(gm["v"], ga["v"]), name
+    batch.close()
+
+
+def test_served_periodic_equals_alone(engine):
+    """Periodic requests arriving among others: each result equals the scene advanced alone, bit for bit."""
+    print("periodic served vs alone", flush=True)
+    scenes = periodic_mix()
+    srv = A.Server(engine, chunk_steps=4, keep_results=True)
+    reqs = [(cfg, p, o, f, 10 + 3 * i, 0.002 * i) for i, (_, cfg, p, o, f) in enumerate(scenes)]
+    ids = [srv.submit(cfg, p, o, f["x"], steps, t, **{k: v for k, v in f.items() if k != "x"})
+           for cfg, p, o, f, steps, t in reqs]
+    rc, st = srv.run(60.0)
+    assert rc == 0 and st.completed == len(reqs) and st.failed == 0, st.as_dict()
+    for rid, (cfg, p, o, f, steps, _) in zip(ids, reqs):
+        got, info, _ = srv.result(rid)
+        e = make(engine, cfg, p, o, f)
+        rc, rinfo = e.step(steps)
+        assert rc == 0 and info.time == rinfo.time
+        ref = e.get_particles()
+        for k in FIELDS:
+            assert np.array_equal(got[k], ref[k], equal_nan=True), f"request {rid}: {k} differs from the scene alone"
     srv.close()
-    rc, _ = e.step(5)  # the scene itself steps alone
-    assert rc == 0
