@@ -1555,11 +1555,18 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
 // every particle of the block gathers from it.
 // ----------------------------------------------------------------------------------------------
 
-// Copies ncomp node arrays over the block's tile into LDS ([component][node]); nodes outside the grid
-// read as zero (no particle window reaches them).
-template <int D, int SPL, class TF>
-__device__ __forceinline__ void load_tile(TF* s, TF* const* src, int ncomp, const int* bc, const uint32_t* n) {
+// Node values in an LDS tile, in one of two layouts: component-major ([component][node]; k_g2p) or
+// node-major (NM: the D components of a node side by side, padded to TileStride<D>, 4 in 3D and 2 in 2D;
+// k_defgrad), where a gather reads a node with one wide load. Measured: node-major took k_defgrad's plain
+// form from 390 to 346 us on C2, but made k_g2p slower (253 -> 283 us), so each kernel keeps its own.
+template <int D> struct TileStride { static constexpr int v = D == 3 ? 4 : 2; };
+
+// Copies the D node arrays src[0..D) over the block's tile into LDS; nodes outside the grid read as zero
+// (no particle window reaches them).
+template <int D, int SPL, class TF, bool NM = false>
+__device__ __forceinline__ void load_tile(TF* s, TF* const* src, const int* bc, const uint32_t* n) {
     using G = BG<D, SPL>;
+    constexpr int ST = TileStride<D>::v;
     for (int q = threadIdx.x; q < G::NT; q += blockDim.x) {
         int t[3] = {0, 0, 0};
         if (D == 3) { t[2] = q % G::T; const int r = q / G::T; t[1] = r % G::T; t[0] = r / G::T; }
@@ -1568,7 +1575,12 @@ __device__ __forceinline__ void load_tile(TF* s, TF* const* src, int ncomp, cons
         int gx[3] = {0, 0, 0};
         for (int d = 0; d < D; ++d) { gx[d] = G::E * bc[d] + t[d]; in = in && gx[d] < (int)n[d]; }
         const uint32_t node = in ? node_index<D>(gx[0], gx[1], gx[2], n) : 0u;
-        for (int c = 0; c < ncomp; ++c) s[c * G::NT + q] = in ? src[c][node] : TF(0);
+        if constexpr (NM) {
+            for (int c = 0; c < D; ++c) s[q * ST + c] = in ? src[c][node] : TF(0);
+            for (int c = D; c < ST; ++c) s[q * ST + c] = TF(0);
+        } else {
+            for (int c = 0; c < D; ++c) s[c * G::NT + q] = in ? src[c][node] : TF(0);
+        }
     }
 }
 
@@ -1585,15 +1597,26 @@ __device__ __forceinline__ bool gather_stencil(Stencil<D, SPL>& st, int* tl, con
 }
 
 // Node values for gathers: from the LDS tile of a block (tile-local window base t0..t2) or from the grid.
-template <int D, int SPL, class TF> struct TileFetch {
+template <int D, int SPL, class TF, bool NM = false> struct TileFetch {
     const TF* v; const TF* f;
     int t0, t1, t2;
     __device__ __forceinline__ void get(int a, int b, int c, TF* vi) const {
         const int q = tile_index<D, SPL>(t0 + a, t1 + b, t2 + c);
-        for (int d = 0; d < D; ++d) vi[d] = v[d * BG<D, SPL>::NT + q];
+        if constexpr (!NM) {
+            for (int d = 0; d < D; ++d) vi[d] = v[d * BG<D, SPL>::NT + q];
+        } else if constexpr (D == 3 && sizeof(TF) == 4) {
+            const float4 w = *reinterpret_cast<const float4*>(v + q * 4);
+            vi[0] = w.x; vi[1] = w.y; vi[2] = w.z;
+        } else if constexpr (D == 2 && sizeof(TF) == 4) {
+            const float2 w = *reinterpret_cast<const float2*>(v + q * 2);
+            vi[0] = w.x; vi[1] = w.y;
+        } else {
+            for (int d = 0; d < D; ++d) vi[d] = v[q * TileStride<D>::v + d];
+        }
     }
     __device__ __forceinline__ TF flip(int a, int b, int c, int d) const {
-        return f[d * BG<D, SPL>::NT + tile_index<D, SPL>(t0 + a, t1 + b, t2 + c)];
+        const int q = tile_index<D, SPL>(t0 + a, t1 + b, t2 + c);
+        return NM ? f[q * TileStride<D>::v + d] : f[d * BG<D, SPL>::NT + q];
     }
 };
 template <int D, class TF> struct GridFetch {
@@ -1775,7 +1798,7 @@ __device__ __forceinline__ uint32_t xcd_slab(uint32_t w, uint32_t nwg) {
 // that four workgroups fit on a CU) are copied; a particle whose block is beyond them gathers from the grid
 // arrays, with the same values, weights and order.
 template <int D, int SPL, class TF, int SETS> struct SlotTiles {
-    static constexpr int bytes = SETS * 3 * BG<D, SPL>::NT * (int)sizeof(TF);
+    static constexpr int bytes = SETS * TileStride<D>::v * BG<D, SPL>::NT * (int)sizeof(TF);
     static constexpr int n = (32768 / bytes) < 1 ? 1 : ((32768 / bytes) > 4 ? 4 : (32768 / bytes));
 };
 // First block of the slots base .. min(n, base + blockDim.x) - 1 and the number of blocks, up to cap, from it.
@@ -1813,8 +1836,8 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
     const bool apic = P.flip_ratio < 0;
     constexpr bool useflip = FLIP;  // flip_ratio >= -1
     const bool flip_rt = P.flip_ratio >= -1;
-    load_tile<D, SPL, TF>(s_v, g.v, D, br.bc, nn);
-    if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, D, br.bc, nn);
+    load_tile<D, SPL, TF>(s_v, g.v, br.bc, nn);
+    if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, br.bc, nn);
     s_c.load(g.coord, br.bc, nn);
     __syncthreads();
     uint32_t err = 0;
@@ -1865,7 +1888,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
                                              const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t n, uint32_t bid) {
     using G = BG<D, SPL>;
     constexpr int NTL = SlotTiles<D, SPL, TF, 1>::n;
-    __shared__ TF s_v[NTL][3 * G::NT];
+    __shared__ __attribute__((aligned(16))) TF s_v[NTL][TileStride<D>::v * G::NT];
     __shared__ BlockCoordLDS<D, SPL> s_c[NTL];
     if (S->stop) return;
     const uint32_t base = bid * blockDim.x;
@@ -1879,7 +1902,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
         if (sb.count[blo + t] == 0) continue;
         int bc[3];
         block_coords<D>(blo + t, S, bc);
-        load_tile<D, SPL, TF>(s_v[t], g.v, D, bc, nn);
+        load_tile<D, SPL, TF, true>(s_v[t], g.v, bc, nn);
         s_c[t].load(g.coord, bc, nn);
     }
     __syncthreads();
@@ -1898,7 +1921,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
             int bc[3], tl[3];
             block_coords<D>(blo + t, S, bc);
             ok = gather_stencil<D, SPL>(st, tl, xp, org, nn, s_c[t].view(bc), h1, bc);
-            if (ok) gather<D, SPL, false, true, false, TF>(st, xp, h1, TileFetch<D, SPL, TF>{s_v[t], s_v[t], tl[0], tl[1], tl[2]}, false, false, pic, fl, Bp, gvt);
+            if (ok) gather<D, SPL, false, true, false, TF>(st, xp, h1, TileFetch<D, SPL, TF, true>{s_v[t], s_v[t], tl[0], tl[1], tl[2]}, false, false, pic, fl, Bp, gvt);
         } else {
             ok = make_stencil<D, SPL>(st, xp, org, nn, GlobalCoords{g.coord, nn}, h1, false);
             if (ok) gather<D, SPL, false, true, false, TF>(st, xp, h1, GridFetch<D, TF>{g, nn, st.base[0], st.base[1], st.base[2]}, false, false, pic, fl, Bp, gvt);
