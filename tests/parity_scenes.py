@@ -200,6 +200,7 @@ SCENES = {
     "vmvisc_newton": lambda: model_block("VMVisc", xi=0.5, perzyna_visc=0.01, perzyna_exp=2.0),
     "dpvisc": lambda: model_block("DPVisc", perzyna_visc=0.01),
     "dpvisc_newton_mibf": lambda: model_block("DPVisc", perzyna_visc=0.01, perzyna_exp=2.0, use_mibf=1),
+    "dp_apic_mibf": lambda: model_block("DP", use_mibf=1, flip=-2.0),  # APIC + MIBF: 5 values per node in P2G
     "dpmui": lambda: model_block("DPMui", q_cohesion=20.0),
     "mcc_nohard": lambda: model_block("MCC", hardening="NoHard"),
     "mcc_expoexpl": lambda: model_block("MCC", hardening="ExpoExpl"),
