@@ -128,12 +128,14 @@ template <int D> struct V { double v[D]; MPM_HD double& operator[](int i) { retu
 template <int D> MPM_HD void set_identity(M<D>& a) { for (int i = 0; i < D * D; ++i) a.m[i] = 0.0; for (int i = 0; i < D; ++i) a(i, i) = 1.0; }
 // (A*B)(r,c) = sum_k A(r,k) B(k,c), summed in k order.
 template <int D> MPM_HD M<D> mul(const M<D>& A, const M<D>& B) {
+    #pragma clang fp contract(fast)  // products and sums fuse into FMAs: fewer VALU instructions, one rounding each
     M<D> R;
     for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { double s = A(r, 0) * B(0, c); for (int k = 1; k < D; ++k) s += A(r, k) * B(k, c); R(r, c) = s; }
     return R;
 }
 // U * diag(d) * V^T evaluated as (U*diag(d)) * V^T.
 template <int D> MPM_HD M<D> udvt(const M<D>& U, const V<D>& d, const M<D>& W) {
+    #pragma clang fp contract(fast)
     M<D> UD; for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) UD(r, c) = U(r, c) * d[c];
     M<D> R;
     for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { double s = UD(r, 0) * W(c, 0); for (int k = 1; k < D; ++k) s += UD(r, k) * W(c, k); R(r, c) = s; }
@@ -171,6 +173,7 @@ MPM_HD PlaneRot sym_jacobi(double x, double y, double z) {  // annihilates y in 
 // One two-sided Jacobi step on the pair (p, q); compile-time indices keep X, U, W in registers.
 template <int D, int p, int q>
 MPM_HD void jacobi_pair(M<D>& X, M<D>& U, M<D>& W, double& dmax, bool& rotated, double tol) {
+    #pragma clang fp contract(fast)
     const double thr = fmax(2.2250738585072014e-308, tol * dmax);
     if (!(fabs(X(p, q)) > thr || fabs(X(q, p)) > thr)) return;
     rotated = true;
@@ -286,6 +289,7 @@ template <int D> struct SymPack { static constexpr int n = D == 3 ? 6 : 3; };
 // Hencky: tau = U diag(2 mu ln s + lambda tr(ln s)) U^T, the product HenckyPiola(F) * F^T of
 // simulation.hpp:231-238 (P = U diag((2 mu ln s + lambda tr ln s)/s) W^T, F^T = W diag(s) U^T).
 template <int D> MPM_HD void hencky_tau(const M<D>& U, const V<D>& lnS, double mu, double lambda, double* tau) {
+    #pragma clang fp contract(fast)
     double tr = lnS[0]; for (int i = 1; i < D; ++i) tr += lnS[i];
     V<D> d; for (int i = 0; i < D; ++i) d[i] = 2.0 * mu * lnS[i] + lambda * tr;
     auto T = [&](int r, int c) { double s = U(r, 0) * d[0] * U(c, 0); for (int k = 1; k < D; ++k) s += U(r, k) * d[k] * U(c, k); return s; };
