@@ -1544,6 +1544,7 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
         V<D> ls;
         if (MAT != MPM_PLASTIC_NONE) {
             V<D> h; for (int k = 0; k < D; ++k) h[k] = log_d(fmax(fabs(s[k]), 1e-4));
+            const V<D> h0 = h;
             RMResult r = return_map<D, MAT>(P, dt, h, ps);
             err |= r.err;
             pcount += r.plastic_incr;
@@ -1552,7 +1553,11 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
                 F = udvt<D>(U, e, W);
                 for (int k = 0; k < D; ++k) ls[k] = h[k];
             } else {
-                for (int k = 0; k < D; ++k) ls[k] = log_d(fabs(s[k]));
+                // ln|s| is the h computed above unless a singular value is below the clamp 1e-4
+                bool small = false;
+                for (int k = 0; k < D; ++k) small = small || !(fabs(s[k]) >= 1e-4);
+                if (small) { for (int k = 0; k < D; ++k) ls[k] = log_d(fabs(s[k])); }
+                else { for (int k = 0; k < D; ++k) ls[k] = h0[k]; }
             }
         } else {
             for (int k = 0; k < D; ++k) ls[k] = log_d(fabs(s[k]));
