@@ -235,8 +235,8 @@ def main():
         print(summarize_pmc(*args.summarize_pmc))
         return
     if args.gpus != 1:
-        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding "
-              "is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported (sharding the particles "
+              "over several GPUs is designed but not built or measured)", file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy:
