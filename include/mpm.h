@@ -69,6 +69,7 @@ enum mpm_precision {
 #define MPM_ERR_NONFINITE         (1u << 9)  /* NaN/Inf reached the particle state */
 #define MPM_ERR_CONSERVATION      (1u << 10) /* simulation.cpp:312-316,328-332 */
 #define MPM_ERR_NOT_READY         (1u << 11) /* step before particles were set */
+#define MPM_ERR_SCRATCH_LIMIT     (1u << 12) /* mpm_step_noalloc: the grid does not fit the device scratch */
 #define MPM_WARN_SUPERSONIC       (1u << 16) /* update_dt.cpp:14-17 (warning only; not an exit) */
 
 #define MPM_ERR_EXIT_MASK 0x0000FFFFu /* bits that map to the reference's `exit = 1` */
@@ -208,6 +209,19 @@ int mpm_set_particles(mpm_context* ctx, uint32_t n, const mpm_particle_arrays* i
  * advance, as simulation.cpp:175-177), or, with MPM_STEP_STOP_AT_FRAME_END, after the step
  * that completes a frame. Returns the OR of MPM_ERR_* bits raised (warnings excluded). */
 int mpm_step(mpm_context* ctx, uint32_t nsteps, uint32_t flags, mpm_step_info* info);
+
+/* mpm_step without device allocation. No counterpart in the reference: for callers that cannot allocate
+ * device memory while they step (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same steps, results and return value as mpm_step, but the grid, block arrays and staging
+ * tiles of the steps live in a static device array of MPM_NOALLOC_SCRATCH_BYTES per GPU (held until the
+ * process ends) instead of the context's own grid, and no hipMalloc, hipFree or allocating library call is
+ * made. The particle state is the context's (allocated by mpm_set_particles). A step whose grid (grown by a
+ * margin of 8 steps) does not fit returns MPM_ERR_SCRATCH_LIMIT before it runs; in 3D with the quadratic
+ * spline the array holds a grid of about 60 nodes per axis (FP64) or 75 (split), margin included. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU race (one host thread per GPU), and mpm_get_grid after this call reads the array, whose
+ * contents the next mpm_step_noalloc on that GPU replaces. A later mpm_step returns to the context's own grid. */
+#define MPM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mpm_step_noalloc(mpm_context* ctx, uint32_t nsteps, uint32_t flags, mpm_step_info* info);
 
 /* Downloads particle state in the order of mpm_set_particles. Replaces reading
  * `sim.particles` (tests/tests.cpp:49,141-143,359). */

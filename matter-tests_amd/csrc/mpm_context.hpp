@@ -6,6 +6,20 @@
 #include <utility>
 #include <mutex>
 
+// Device buffers of the node grid and of the per-block arrays of the step, with their capacities (nodes,
+// nodes per axis, blocks). A context owns one store; mpm_step_noalloc binds a store carved from the device's
+// static scratch instead.
+struct GridStore {
+    char* gbuf = nullptr;        // node arrays: mass, friction, v[dim], flip[dim]
+    double* cbuf = nullptr;      // node coordinates, cap_axis per axis
+    uint32_t* bbuf = nullptr;    // block histogram, offsets, cursors
+    uint16_t* cellbuf = nullptr; // per-block dual-cell offsets
+    char* stgbuf = nullptr;      // per-block node tiles (staging of the scatter kernels)
+    uint64_t cap_nodes = 0;
+    uint32_t cap_axis = 0;
+    uint32_t cap_blocks = 0;
+};
+
 struct mpm_context {
     mpm_config cfg;
     mpm::Params P;
@@ -23,19 +37,19 @@ struct mpm_context {
     mpm::PartStateT<double> ps[2]{};  // plain form
     mpm::PartStateT<float> pf[2]{};   // split form
     mpm::SortBufs sb{};
-    uint32_t* bbuf = nullptr;   // block histogram, offsets, cursors
-    uint16_t* cellbuf = nullptr; // per-block dual-cell offsets
-    char* stgbuf = nullptr;     // per-block node tiles (staging of the scatter kernels)
-    uint32_t cap_blocks = 0;    // allocated blocks (sort arrays, staging)
+    // grid and block arrays: the store the context allocated, and the one its views (sb.count/offset/cursor/
+    // cell_off, stg, gp, ...) point into now (own, or the device scratch after mpm_step_noalloc)
+    GridStore own{};
+    bool grid_bound = false;    // the views point into a store
+    bool on_scratch = false;    // ... and that store is the device scratch
+    uint64_t grid_gen = 0;      // incremented whenever the views change (batch tables cache them)
+    uint32_t cap_blocks = 0;    // blocks of the bound store (sort arrays, staging)
     uint32_t act_blocks = 0;    // blocks the block kernels launch and a step may use (<= cap_blocks)
+    uint64_t cap_nodes = 0;     // nodes of the bound store
+    uint32_t cap_axis = 0;      // nodes per axis of the bound store's coordinate arrays
     mpm::StagingT<double> stg{};
     mpm::StagingT<float> stgf{};
     bool flip_formed = false;   // the last enqueued step wrote grid flip
-    // grid
-    char* gbuf = nullptr;
-    double* cbuf = nullptr;
-    uint64_t cap_nodes = 0;
-    uint32_t cap_axis = 0;
     mpm::GridPtrsT<double> gp{};
     mpm::GridPtrsT<float> gf{};
     // step state

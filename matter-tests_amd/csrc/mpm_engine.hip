@@ -18,6 +18,16 @@
 
 using namespace mpm;
 
+// ---- Scratch of mpm_step_noalloc -----------------------------------------------------------------------
+// A file-scope device array that holds the grid, block arrays and staging tiles of a no-allocation step.
+// Every GPU that loads the library holds its own copy until the process ends. Host code must not pass the
+// array's name to a kernel or a copy: that is a host-side stand-in, not a device address. The device address
+// comes from hipGetSymbolAddress, taken once per device with that device current (scratch_address).
+// Nothing in it has to start at zero: every step clears its block histogram (k_zero_grid) and writes its
+// grid, cell offsets and staging tiles before it reads them.
+constexpr size_t kScratchBytes = MPM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 namespace {
 
 constexpr uint32_t kBatch = 32;
@@ -127,66 +137,155 @@ uint32_t growth_nodes(const mpm_context* c, uint32_t batch) {
     return 2 * (uint32_t)std::ceil(cfl * batch) + 6;
 }
 
-int ensure_grid(mpm_context* c, uint32_t batch) {
+// Grid that the next `batch` steps may need: nodes per axis (largest axis), nodes, and blocks of the grid
+// grown by the margin.
+struct GridNeed { uint32_t axis; uint64_t nodes, blocks; };
+GridNeed grid_need(const mpm_context* c, uint32_t batch) {
     const int D = c->cfg.dim;
     const uint32_t margin = growth_nodes(c, batch);
-    uint32_t want_axis = 0; uint64_t want_nodes = 1;
-    for (int d = 0; d < D; ++d) {
-        uint32_t a = c->est_n[d] + margin;
-        want_axis = std::max(want_axis, a);
-        want_nodes *= a;
-    }
-    uint64_t need = 1;  // blocks of a grid grown by the margin
+    GridNeed nd{0, 1, 1};
     const uint32_t be = D == 3 ? BlockEdge<3>::v : BlockEdge<2>::v;
-    for (int d = 0; d < D; ++d) need *= (c->est_n[d] + margin + be - 1) / be;
-    if (need > c->cap_blocks) {
-        if (c->bbuf) { (void)hipFree(c->bbuf); c->bbuf = nullptr; }
-        if (c->cellbuf) { (void)hipFree(c->cellbuf); c->cellbuf = nullptr; }
-        if (c->stgbuf) { (void)hipFree(c->stgbuf); c->stgbuf = nullptr; }
-        c->cap_blocks = 0; c->act_blocks = 0;
-        const size_t nt = tile_nodes(D, c->cfg.spline_degree);
-        if (need * nt > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
-        const uint64_t alloc = std::min<uint64_t>(0xFFFFFFF0ull / nt, need + need / 4);  // room for the active count to grow
-        // count, offset, cursor; with periodic boundaries each holds the periodic copies' blocks after the particles'
-        const uint64_t bs = c->cfg.pbc ? 2 * alloc : alloc;
-        if (hipMalloc(&c->bbuf, (size_t)3 * bs * sizeof(uint32_t)) != hipSuccess) { c->bbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
-        if (hipMalloc(&c->cellbuf, (size_t)65 * alloc * sizeof(uint16_t)) != hipSuccess) { c->cellbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
-        const size_t tf = c->split ? sizeof(float) : sizeof(double);
-        if (hipMalloc(&c->stgbuf, (size_t)kMaxNV * nt * alloc * tf) != hipSuccess) { c->stgbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
-        c->cap_blocks = (uint32_t)alloc;
-        c->sb.count = c->bbuf; c->sb.offset = c->bbuf + bs; c->sb.cursor = c->bbuf + 2 * bs;
-        c->sb.cell_off = c->cellbuf;
-        c->stg.base = (double*)c->stgbuf; c->stg.stride = (uint32_t)(nt * alloc);
-        c->stgf.base = (float*)c->stgbuf; c->stgf.stride = (uint32_t)(nt * alloc);
+    for (int d = 0; d < D; ++d) {
+        const uint32_t a = c->est_n[d] + margin;
+        nd.axis = std::max(nd.axis, a);
+        nd.nodes *= a;
+        nd.blocks *= (a + be - 1) / be;
     }
-    // Active blocks: the block kernels launch one workgroup per active block, and a step whose grid has more
-    // blocks stops for a reallocation. They follow the batch's need with some hysteresis (grow to 5/4 of it,
-    // shrink below half), so that batches keep their launch shapes and graphs for a while.
-    if (need > c->act_blocks || 2 * need < c->act_blocks)
-        c->act_blocks = (uint32_t)std::min<uint64_t>(c->cap_blocks, need + need / 4);
-    if (want_axis <= c->cap_axis && want_nodes <= c->cap_nodes) return 0;
-    if (want_nodes > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
-    want_axis = std::max(want_axis, c->cap_axis);
-    want_nodes = std::max(want_nodes, c->cap_nodes);
-    if (c->gbuf) { (void)hipFree(c->gbuf); c->gbuf = nullptr; }
-    if (c->cbuf) { (void)hipFree(c->cbuf); c->cbuf = nullptr; }
-    const int nf = 1 + 2 * D + 1;  // mass, v, flip, fric
+    return nd;
+}
+
+// Bytes of the parts of a store: block arrays (count, offset, cursor; with periodic boundaries each also
+// holds the periodic copies' blocks after the particles'), dual-cell offsets, staging tiles, node arrays
+// (mass, friction, v, flip) and node coordinates.
+struct StoreBytes { size_t bbuf, cellbuf, stgbuf, gbuf, cbuf; };
+StoreBytes store_bytes(const mpm_context* c, uint64_t blocks, uint64_t nodes, uint32_t axis) {
+    const int D = c->cfg.dim;
     const size_t tf = c->split ? sizeof(float) : sizeof(double);
-    if (hipMalloc(&c->gbuf, (size_t)nf * want_nodes * tf) != hipSuccess) { c->gbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
-    if (hipMalloc(&c->cbuf, (size_t)3 * want_axis * sizeof(double)) != hipSuccess) { c->cbuf = nullptr; c->cap_nodes = 0; return MPM_ERR_OUT_OF_MEMORY; }
+    const uint64_t bs = c->cfg.pbc ? 2 * blocks : blocks;
+    StoreBytes b;
+    b.bbuf = (size_t)3 * bs * sizeof(uint32_t);
+    b.cellbuf = (size_t)65 * blocks * sizeof(uint16_t);
+    b.stgbuf = (size_t)kMaxNV * tile_nodes(D, c->cfg.spline_degree) * blocks * tf;
+    b.gbuf = (size_t)(1 + 2 * D + 1) * nodes * tf;
+    b.cbuf = (size_t)3 * axis * sizeof(double);
+    return b;
+}
+
+// Points the context's views (sort arrays, staging, grid) into store s.
+void bind_store(mpm_context* c, const GridStore& s) {
+    const int D = c->cfg.dim;
+    const size_t nt = tile_nodes(D, c->cfg.spline_degree);
+    const uint64_t bs = c->cfg.pbc ? 2ull * s.cap_blocks : (uint64_t)s.cap_blocks;
+    c->sb.count = s.bbuf; c->sb.offset = s.bbuf + bs; c->sb.cursor = s.bbuf + 2 * bs;
+    c->sb.cell_off = s.cellbuf;
+    c->stg.base = (double*)s.stgbuf; c->stg.stride = (uint32_t)(nt * s.cap_blocks);
+    c->stgf.base = (float*)s.stgbuf; c->stgf.stride = (uint32_t)(nt * s.cap_blocks);
     auto carve = [&](auto& g) {
         using TF = std::remove_pointer_t<decltype(g.mass)>;
-        TF* q = (TF*)c->gbuf;
-        auto take = [&]() { TF* r = q; q += want_nodes; return r; };
+        TF* q = (TF*)s.gbuf;
+        auto take = [&]() { TF* r = q; q += s.cap_nodes; return r; };
         g.mass = take();
         g.fric = take();
         for (int d = 0; d < D; ++d) g.v[d] = take();
         for (int d = 0; d < D; ++d) g.flip[d] = take();
-        for (int d = 0; d < 3; ++d) g.coord[d] = c->cbuf + (size_t)d * want_axis;
+        for (int d = 0; d < 3; ++d) g.coord[d] = s.cbuf + (size_t)d * s.cap_axis;
     };
     carve(c->gp);
     carve(c->gf);
-    c->cap_axis = want_axis; c->cap_nodes = want_nodes;
+    c->cap_blocks = s.cap_blocks; c->cap_nodes = s.cap_nodes; c->cap_axis = s.cap_axis;
+    c->act_blocks = std::min(c->act_blocks, c->cap_blocks);
+    c->grid_bound = true;
+    c->grid_gen++;
+}
+
+void unbind_store(mpm_context* c) {
+    c->grid_bound = false; c->on_scratch = false;
+    c->cap_blocks = 0; c->act_blocks = 0; c->cap_nodes = 0; c->cap_axis = 0;
+    c->grid_gen++;
+}
+
+// Active blocks: the block kernels launch one workgroup per active block, and a step whose grid has more
+// blocks stops for a reallocation. They follow the batch's need with some hysteresis (grow to 5/4 of it,
+// shrink below half), so that batches keep their launch shapes and graphs for a while.
+void set_active_blocks(mpm_context* c, uint64_t need) {
+    if (need > c->act_blocks || 2 * need < c->act_blocks)
+        c->act_blocks = (uint32_t)std::min<uint64_t>(c->cap_blocks, need + need / 4);
+}
+
+// Grows the context's own store to the grid of the next `batch` steps (reallocating) and binds it.
+int ensure_grid(mpm_context* c, uint32_t batch) {
+    const GridNeed nd = grid_need(c, batch);
+    GridStore& o = c->own;
+    bool changed = !c->grid_bound || c->on_scratch;
+    if (nd.blocks > o.cap_blocks) {
+        if (o.bbuf) { (void)hipFree(o.bbuf); o.bbuf = nullptr; }
+        if (o.cellbuf) { (void)hipFree(o.cellbuf); o.cellbuf = nullptr; }
+        if (o.stgbuf) { (void)hipFree(o.stgbuf); o.stgbuf = nullptr; }
+        o.cap_blocks = 0;
+        unbind_store(c);
+        const size_t nt = tile_nodes(c->cfg.dim, c->cfg.spline_degree);
+        if (nd.blocks * nt > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
+        const uint64_t alloc = std::min<uint64_t>(0xFFFFFFF0ull / nt, nd.blocks + nd.blocks / 4);  // room for the active count to grow
+        const StoreBytes sz = store_bytes(c, alloc, 0, 0);
+        if (hipMalloc(&o.bbuf, sz.bbuf) != hipSuccess) { o.bbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        if (hipMalloc(&o.cellbuf, sz.cellbuf) != hipSuccess) { o.cellbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        if (hipMalloc(&o.stgbuf, sz.stgbuf) != hipSuccess) { o.stgbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        o.cap_blocks = (uint32_t)alloc;
+        changed = true;
+    }
+    if (!(nd.axis <= o.cap_axis && nd.nodes <= o.cap_nodes)) {
+        if (nd.nodes > 0xFFFFFFF0ull) return MPM_ERR_OUT_OF_MEMORY;
+        const uint32_t want_axis = std::max(nd.axis, o.cap_axis);
+        const uint64_t want_nodes = std::max(nd.nodes, o.cap_nodes);
+        if (o.gbuf) { (void)hipFree(o.gbuf); o.gbuf = nullptr; }
+        if (o.cbuf) { (void)hipFree(o.cbuf); o.cbuf = nullptr; }
+        o.cap_nodes = 0; o.cap_axis = 0;
+        unbind_store(c);
+        const StoreBytes sz = store_bytes(c, 0, want_nodes, want_axis);
+        if (hipMalloc(&o.gbuf, sz.gbuf) != hipSuccess) { o.gbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        if (hipMalloc(&o.cbuf, sz.cbuf) != hipSuccess) { o.cbuf = nullptr; return MPM_ERR_OUT_OF_MEMORY; }
+        o.cap_axis = want_axis; o.cap_nodes = want_nodes;
+        changed = true;
+    }
+    if (changed) { bind_store(c, o); c->on_scratch = false; }
+    set_active_blocks(c, nd.blocks);
+    return 0;
+}
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on device dev (current), taken once per device. 0 on success.
+int scratch_address(int dev, char** out) {
+    if (dev < 0 || dev >= kMaxScratchDevices) return MPM_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MPM_ERR_HIP;
+        g_scratch_ptr[dev] = (char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Carves a store for the grid of the next `batch` steps out of the scratch at `base` and binds it; no
+// device memory is allocated. MPM_ERR_SCRATCH_LIMIT when that grid does not fit.
+int bind_scratch(mpm_context* c, uint32_t batch, char* base) {
+    const GridNeed nd = grid_need(c, batch);
+    if (nd.nodes > 0xFFFFFFF0ull || nd.blocks * tile_nodes(c->cfg.dim, c->cfg.spline_degree) > 0xFFFFFFF0ull) return MPM_ERR_SCRATCH_LIMIT;
+    const StoreBytes sz = store_bytes(c, nd.blocks, nd.nodes, nd.axis);
+    auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_cell = align(sz.bbuf), o_stg = o_cell + align(sz.cellbuf), o_g = o_stg + align(sz.stgbuf),
+                 o_c = o_g + align(sz.gbuf), end = o_c + align(sz.cbuf);
+    if (end > kScratchBytes) return MPM_ERR_SCRATCH_LIMIT;
+    GridStore s;
+    s.bbuf = (uint32_t*)base; s.cellbuf = (uint16_t*)(base + o_cell); s.stgbuf = base + o_stg;
+    s.gbuf = base + o_g; s.cbuf = (double*)(base + o_c);
+    s.cap_blocks = (uint32_t)nd.blocks; s.cap_nodes = nd.nodes; s.cap_axis = nd.axis;
+    bind_store(c, s);
+    c->on_scratch = true;
+    c->act_blocks = s.cap_blocks;
     return 0;
 }
 
@@ -596,11 +695,11 @@ void mpm_destroy(mpm_context* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->pbuf) (void)hipFree(c->pbuf);
     if (c->ibuf) (void)hipFree(c->ibuf);
-    if (c->bbuf) (void)hipFree(c->bbuf);
-    if (c->cellbuf) (void)hipFree(c->cellbuf);
-    if (c->stgbuf) (void)hipFree(c->stgbuf);
-    if (c->gbuf) (void)hipFree(c->gbuf);
-    if (c->cbuf) (void)hipFree(c->cbuf);
+    if (c->own.bbuf) (void)hipFree(c->own.bbuf);
+    if (c->own.cellbuf) (void)hipFree(c->own.cellbuf);
+    if (c->own.stgbuf) (void)hipFree(c->own.stgbuf);
+    if (c->own.gbuf) (void)hipFree(c->own.gbuf);
+    if (c->own.cbuf) (void)hipFree(c->own.cbuf);
     if (c->batch) (void)mpm_batch_remove(c->batch, c);
     if (c->S) (void)hipFree(c->S);
     if (c->Sh) (void)hipHostFree(c->Sh);
@@ -697,17 +796,20 @@ int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in)
     return 0;
 }
 
-int mpm_step(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* info) {
-    if (!c) return MPM_ERR_INVALID_ARGUMENT;
-    if (info) std::memset(info, 0, sizeof(*info));
-    if (!c->ready) return MPM_ERR_NOT_READY;
-    if (c->batch) return MPM_ERR_INVALID_ARGUMENT;  // a batch member advances with its batch
-    if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
+}  // extern "C"
+
+namespace {
+// The loop of mpm_step: batches of steps enqueued without host synchronisation, the grid grown between
+// batches. scratch == nullptr: the context's own store (allocated as needed). Otherwise the grid lives in
+// the device scratch at `scratch`, batches are kNoAllocBatch steps (a smaller growth margin), and nothing
+// is allocated.
+constexpr uint32_t kNoAllocBatch = 8;
+int step_loop(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* info, char* scratch) {
     uint32_t remaining = nsteps, done = 0, status = 0, reallocs = 0;
     const int D = c->cfg.dim;
     while (remaining > 0) {
-        uint32_t batch = std::min(remaining, kBatch);
-        int rc = ensure_grid(c, batch);
+        uint32_t batch = std::min(remaining, scratch ? kNoAllocBatch : kBatch);
+        int rc = scratch ? bind_scratch(c, batch, scratch) : ensure_grid(c, batch);
         if (rc) { status |= rc; break; }
         uint32_t xsel = c->Sh->xsel;
         k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u,
@@ -739,6 +841,35 @@ int mpm_step(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* inf
     }
     return (int)(status & MPM_ERR_EXIT_MASK);
 }
+}  // namespace
+
+extern "C" {
+
+int mpm_step(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* info) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    if (info) std::memset(info, 0, sizeof(*info));
+    if (!c->ready) return MPM_ERR_NOT_READY;
+    if (c->batch) return MPM_ERR_INVALID_ARGUMENT;  // a batch member advances with its batch
+    if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
+    return step_loop(c, nsteps, flags, info, nullptr);
+}
+
+int mpm_step_noalloc(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* info) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    if (info) std::memset(info, 0, sizeof(*info));
+    if (!c->ready) return MPM_ERR_NOT_READY;
+    if (c->batch) return MPM_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
+    char* base = nullptr;
+    const int rc = scratch_address(c->device, &base);
+    if (rc) { if (info) info->status = (uint32_t)rc; return rc; }
+    // per-kernel timing would create HIP events: it is off for the call
+    const bool prof = c->profiling;
+    c->profiling = false;
+    const int r = step_loop(c, nsteps, flags, info, base);
+    c->profiling = prof;
+    return r;
+}
 
 uint32_t mpm_num_particles(const mpm_context* c) { return c ? c->n : 0; }
 
@@ -761,7 +892,7 @@ int mpm_get_grid(mpm_context* c, mpm_grid_info* info, double* x, double* y, doub
         info->Nx = S.n[0]; info->Ny = S.n[1]; info->Nz = D == 3 ? S.n[2] : 1; info->grid_nodes = nn;
         info->xc = S.origin[0]; info->yc = S.origin[1]; info->zc = D == 3 ? S.origin[2] : 0.0;
     }
-    if (nn == 0 || !c->gbuf) return 0;
+    if (nn == 0 || !c->grid_bound) return 0;
     double* axes[3] = {x, y, z};
     for (int d = 0; d < D; ++d) if (axes[d]) HIP_TRY(hipMemcpy(axes[d], c->gp.coord[d], sizeof(double) * S.n[d], hipMemcpyDeviceToHost));
     return c->split ? download_grid<float>(c, nn, mass, v, flip, friction) : download_grid<double>(c, nn, mass, v, flip, friction);
@@ -947,8 +1078,9 @@ const char* mpm_status_string(uint32_t s) {
     static const char* names[] = {"invalid argument", "unsupported option", "HIP runtime error", "out of memory",
                                   "negative Perzyna delta_gamma", "Newton iteration limit", "Newton derivative too small",
                                   "MCC return map found no admissible step", "stencil outside the grid", "non-finite state",
-                                  "conservation check failed", "particles not set"};
-    for (int b = 0; b < 12; ++b) if (s & (1u << b)) return names[b];
+                                  "conservation check failed", "particles not set",
+                                  "grid of the no-allocation step exceeds the device scratch"};
+    for (int b = 0; b < 13; ++b) if (s & (1u << b)) return names[b];
     if (s & MPM_WARN_SUPERSONIC) return "particle faster than the elastic wave speed";
     return "unknown status";
 }
@@ -1042,11 +1174,12 @@ int mpm_batch_step(mpm_batch* b, uint32_t max_steps, uint32_t flags, mpm_batch_i
             mpm_context* c = b->scenes[i];
             const uint32_t lim = std::min(chunk, want[i] - done[i]);
             if (!lim && c->act_blocks && c->cap_nodes) continue;  // idle member whose buffers exist
-            const void* before[3] = {c->gbuf, c->stgbuf, c->bbuf};
+            // the views and launch shapes a class table caches: any rebinding of the store bumps grid_gen
+            const uint64_t gen_before = c->grid_gen;
             const uint32_t act_before = c->act_blocks;
             const int rc = ensure_grid(c, std::max(lim, 1u));
             if (rc) { status |= rc; break; }
-            if (before[0] != c->gbuf || before[1] != c->stgbuf || before[2] != c->bbuf || act_before != c->act_blocks) b->dirty = true;
+            if (gen_before != c->grid_gen || act_before != c->act_blocks) b->dirty = true;
         }
         if (status) break;
         if (b->dirty || b->check_cons != cons) {
@@ -1136,12 +1269,9 @@ int mpm_scene_info(const mpm_context* c, mpm_step_info* info, uint32_t* budget_l
 
 uint64_t mpm_device_bytes(const mpm_context* c) {
     if (!c) return 0;
-    const int D = c->cfg.dim;
-    const size_t tf = c->split ? sizeof(float) : sizeof(double);
     uint64_t bytes = c->pbuf_bytes + c->ibuf_n * sizeof(uint32_t);
-    bytes += (uint64_t)3 * c->cap_blocks * sizeof(uint32_t) + (uint64_t)65 * c->cap_blocks * sizeof(uint16_t);
-    bytes += (uint64_t)kMaxNV * tile_nodes(D, c->cfg.spline_degree) * c->cap_blocks * tf;
-    bytes += (uint64_t)(1 + 2 * D + 1) * c->cap_nodes * tf + (uint64_t)3 * c->cap_axis * sizeof(double);
+    const StoreBytes sz = store_bytes(c, c->own.cap_blocks, c->own.cap_nodes, c->own.cap_axis);  // the scratch is not the context's
+    bytes += sz.bbuf + sz.cellbuf + sz.stgbuf + sz.gbuf + sz.cbuf;
     bytes += sizeof(StepState) + (c->dP ? sizeof(Params) : 0);
     return bytes;
 }

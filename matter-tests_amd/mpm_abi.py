@@ -24,7 +24,7 @@ PRECISION = dict(fp64=0, split=1)
 
 ERR = dict(INVALID_ARGUMENT=1 << 0, UNSUPPORTED=1 << 1, HIP=1 << 2, OUT_OF_MEMORY=1 << 3,
            PERZYNA_NEGATIVE=1 << 4, NEWTON_MAXITER=1 << 5, NEWTON_SMALL_DERIV=1 << 6, MCC_NO_STEP=1 << 7,
-           GRID_BOUNDS=1 << 8, NONFINITE=1 << 9, CONSERVATION=1 << 10, NOT_READY=1 << 11,
+           GRID_BOUNDS=1 << 8, NONFINITE=1 << 9, CONSERVATION=1 << 10, NOT_READY=1 << 11, SCRATCH_LIMIT=1 << 12,
            WARN_SUPERSONIC=1 << 16)
 STEP_STOP_AT_FRAME_END = 1
 STEP_CHECK_CONSERVATION = 2
@@ -230,6 +230,8 @@ class Library:
             for name in ("mpm_write_particles_ply", "mpm_write_grid_ply"):
                 getattr(L, name).argtypes = [vp, C.c_char_p]
                 getattr(L, name).restype = C.c_int
+            L.mpm_step_noalloc.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(StepInfo)]
+            L.mpm_step_noalloc.restype = C.c_int
             L.mpm_batch_autotune.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
             L.mpm_batch_autotune.restype = C.c_int
             L.mpm_device_bytes.argtypes = [vp]
@@ -367,6 +369,12 @@ class Engine:
         rc = self.L.fn("step")(self.h, nsteps, flags, C.byref(info))
         if rc and raise_on_error:
             raise MPMError(rc, "step")
+        return rc, info
+
+    def step_noalloc(self, nsteps=1, flags=0):
+        """mpm_step_noalloc: the same steps with the grid in the device's static scratch (no allocation)."""
+        info = StepInfo()
+        rc = self.L.lib.mpm_step_noalloc(self.h, nsteps, flags, C.byref(info))
         return rc, info
 
     def get_particles(self):
