@@ -247,7 +247,9 @@ __device__ __forceinline__ bool stencil_base(int* b, const double* xp, const dou
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        int v = (int)floor((xp[d] - org[d]) * h1) - 1;
+        // a NaN or out-of-range coordinate becomes an index outside every grid (no int overflow in v + 3)
+        const double f = floor((xp[d] - org[d]) * h1);
+        int v = f < 2.0e9 ? (f > -2.0e9 ? (int)f - 1 : -0x3FFFFFFF) : 0x3FFFFFFF;
         if (clamp && v < 0) v = 0;
         if (v < 0 || v + 3 >= (int)n[d]) ok = false;
         b[d] = v;
@@ -394,6 +396,11 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
             }
             double mn[3], mx[3], lo[3], hi[3];
             for (int d = 0; d < D; ++d) { mn[d] = key_val(S->xmin_key[d]); mx[d] = key_val(S->xmax_key[d]); }
+            // A NaN or infinite velocity or position anywhere shows up in these reductions (a NaN's order key
+            // sorts above +inf or below -inf). The reference steps on with it (dt falls back to dt_max and the
+            // remesh snap's fmax drops the NaN); here the run stops before the step.
+            bool state_finite = isfinite(max_speed);
+            for (int d = 0; d < D; ++d) state_finite = state_finite && isfinite(mn[d]) && isfinite(mx[d]);
             double mni[3], mxi[3], loi[3], hii[3];
             const double dx = P.dx;
             if (S->step == 0) {
@@ -436,7 +443,7 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
             for (int d = 0; d < D; ++d) nb[d] = (cnt[d] + BlockEdge<D>::v - 1) / BlockEdge<D>::v;
             const unsigned long long nblocks = (unsigned long long)nb[0] * nb[1] * nb[2];
             if (nblocks > S->cap_blocks) fits = false;
-            bool finite = true;
+            bool finite = state_finite;
             for (int d = 0; d < D; ++d) if (!(lo[d] == lo[d]) || !(hi[d] == hi[d]) || !isfinite(lo[d]) || !isfinite(hi[d]) || cnt[d] < 1) finite = false;
             if (!(dt == dt)) finite = false;
             if (!finite) {

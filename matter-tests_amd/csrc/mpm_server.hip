@@ -87,10 +87,12 @@ int prepare(Request& r, int device) {
                          &a.viscosity, &a.muI};
     for (int k = 0; k < 10; ++k) *slot[k] = r.f[k].empty() ? nullptr : r.f[k].data();
     if ((rc = mpm_set_particles(c, r.n, &a))) { mpm_destroy(c); return rc; }
-    for (auto& v : r.f) std::vector<double>().swap(v);  // the device holds the state now
-    r.ctx = c;
+    r.ctx = c;  // the inputs stay until the request is done: a run stopped by its time limit re-queues it
     return 0;
 }
+
+// A request that has reached RQ_DONE or RQ_FAILED no longer needs its input copies.
+void release_inputs(Request& r) { for (auto& v : r.f) std::vector<double>().swap(v); }
 
 }  // namespace
 
@@ -243,6 +245,7 @@ int mpm_server_run(mpm_server* s, double time_limit_s, mpm_server_stats* stats) 
             order[next]->est_bytes > s->opt.hbm_budget_bytes) {
             Request* r = order[next++];
             r->state = RQ_FAILED; r->status |= MPM_ERR_OUT_OF_MEMORY; ++failed;
+            release_inputs(*r);
             continue;
         }
         // prepared scenes join the batch
@@ -254,9 +257,13 @@ int mpm_server_run(mpm_server* s, double time_limit_s, mpm_server_stats* stats) 
         for (Request* r : ready) {
             --in_flight;
             pending_bytes -= std::min(pending_bytes, r->est_bytes);
-            if (r->state == RQ_FAILED) { ++failed; continue; }
+            if (r->state == RQ_FAILED) { ++failed; release_inputs(*r); continue; }
             const int arc = mpm_batch_add(batch, r->ctx, r->nsteps);
-            if (arc) { r->status |= (uint32_t)arc; r->state = RQ_FAILED; mpm_destroy(r->ctx); r->ctx = nullptr; ++failed; continue; }
+            if (arc) {
+                r->status |= (uint32_t)arc; r->state = RQ_FAILED; mpm_destroy(r->ctx); r->ctx = nullptr; ++failed;
+                release_inputs(*r);
+                continue;
+            }
             r->state = RQ_RESIDENT;
             r->t_resident = now_s();
             queue_ms.push_back(1e3 * (r->t_resident - r->arrival));
@@ -319,6 +326,7 @@ int mpm_server_run(mpm_server* s, double time_limit_s, mpm_server_stats* stats) 
             }
             mpm_destroy(r->ctx);
             r->ctx = nullptr;
+            release_inputs(*r);
             if (bad) { r->state = RQ_FAILED; ++failed; }
             else { r->state = RQ_DONE; ++completed; req_ms.push_back(1e3 * (r->t_done - r->arrival)); }
         }
@@ -331,10 +339,16 @@ int mpm_server_run(mpm_server* s, double time_limit_s, mpm_server_stats* stats) 
     }
     cv_prep.notify_all();
     worker.join();
-    // scenes prepared or resident when the run stopped early
-    for (Request* r : prepared) if (r->ctx) { mpm_destroy(r->ctx); r->ctx = nullptr; r->state = RQ_QUEUED; }
-    for (Request* r : resident) { (void)mpm_batch_remove(batch, r->ctx); mpm_destroy(r->ctx); r->ctx = nullptr; r->state = RQ_QUEUED; }
-    for (size_t i = next; i < total; ++i) if (order[i]->state == RQ_PREPARING) order[i]->state = RQ_QUEUED;
+    // Requests interrupted by the time limit (admitted and waiting for preparation, prepared, or resident) go
+    // back to the queue with their inputs and start from the beginning in a later run; a request whose
+    // preparation failed stays failed.
+    for (Request* r : prepared) if (r->ctx) { mpm_destroy(r->ctx); r->ctx = nullptr; }
+    for (Request* r : resident) { (void)mpm_batch_remove(batch, r->ctx); mpm_destroy(r->ctx); r->ctx = nullptr; }
+    for (Request* r : order) {
+        if (r->state != RQ_PREPARING && r->state != RQ_READY && r->state != RQ_RESIDENT) continue;
+        r->state = RQ_QUEUED; r->status = 0; r->info = mpm_step_info{};
+        r->t_admit = r->t_resident = r->t_done = -1.0;
+    }
     mpm_batch_destroy(batch);
     if (stats) {
         stats->submitted = (uint32_t)total; stats->completed = completed; stats->failed = failed;

@@ -272,6 +272,8 @@ class Library:
             L.oracle_spline_dN.restype = C.c_double
             L.oracle_arange.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_int]
             L.oracle_max_threads.restype = C.c_int
+            L.oracle_object_eval.argtypes = [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+            L.oracle_object_eval.restype = C.c_int
         for name, args in (("add_plate", [vp, C.POINTER(Plate)]), ("add_object", [vp, C.POINTER(Object)]),
                            ("set_particles", [vp, C.c_uint32, C.POINTER(ParticleArrays)]),
                            ("step", [vp, C.c_uint32, C.c_uint32, C.POINTER(StepInfo)]),

@@ -23,6 +23,7 @@ struct OracleBase {
     virtual int avg_data(double*, double*, double*) = 0;
     virtual double bulk_modulus() const = 0;
     virtual void phase_times(double*) const = 0;
+    virtual int object_eval(uint32_t, const double*, int*, double*) const = 0;
 };
 
 template <int D, int SPL> struct OracleImpl : OracleBase {
@@ -175,6 +176,14 @@ template <int D, int SPL> struct OracleImpl : OracleBase {
     void phase_times(double* t) const override {
         t[0] = sim.runtime_p2g; t[1] = sim.runtime_euler; t[2] = sim.runtime_g2p; t[3] = sim.runtime_defgrad;
     }
+    int object_eval(uint32_t k, const double* X, int* inside, double* normal) const override {
+        if (k >= sim.objects.size()) return MPM_ERR_INVALID_ARGUMENT;
+        Vec<D> x; for (int d = 0; d < D; ++d) x(d) = X[d];
+        *inside = sim.objects[k]->inside(x) ? 1 : 0;
+        const Vec<D> n = sim.objects[k]->normal(x);
+        for (int d = 0; d < D; ++d) normal[d] = n(d);
+        return 0;
+    }
 };
 
 OracleBase* make(const mpm_config* c) {
@@ -316,5 +325,10 @@ double oracle_time_steps(void* h, uint32_t nsteps, mpm_step_info* info) {
 // Accumulated phase timers of advanceStep in seconds: P2G, explicitEulerUpdate, G2P, deformationUpdate
 // (simulation.cpp:237-266; the reference prints them at the end of simulate(), :205-209).
 void oracle_phase_times(void* h, double* out) { ((OracleBase*)h)->phase_times(out); }
+
+// inside() and normal() of analytic object k of the handle (ObjectGeneral, object_general.hpp:19-21) at X.
+int oracle_object_eval(void* h, uint32_t k, const double* X, int* inside, double* normal) {
+    return ((OracleBase*)h)->object_eval(k, X, inside, normal);
+}
 
 }  // extern "C"

@@ -472,7 +472,12 @@ public:
     }
 
     // Base index of the stencil: floor((x - origin)/dx) - 1 (p2g.cpp:21-24 clamps at 0, g2p.cpp:29-32 does not).
-    inline long baseIndex(T xp, T origin) const { return (long)std::floor((xp - origin) * one_over_dx) - 1; }
+    // A NaN or out-of-range coordinate maps to an index far outside every grid, which stencilInside rejects
+    // (the reference converts it unchecked and reads out of bounds).
+    inline long baseIndex(T xp, T origin) const {
+        const T f = std::floor((xp - origin) * one_over_dx);
+        return f < 2.0e9 ? (f > -2.0e9 ? (long)f - 1 : -(1L << 40)) : (1L << 40);
+    }
     inline bool stencilInside(long b[3]) const {
         long n[3] = {(long)Nx, (long)Ny, (long)Nz};
         for (int d = 0; d < D; ++d) if (b[d] < 0 || b[d] + 3 >= n[d]) return false;

@@ -166,3 +166,73 @@ def bulk_modulus(lib, dim=3, spacing=0.02, end_frame=100):
     p_avg = -np.trace(kirch) / dim
     measured = p_avg / (-math.log(Javg))
     return abs(measured - K) / K
+
+
+# ---------------------------------------------------------------------------------------------------------
+# The two cases whose outcome depends on the particle sample, driven by an explicit sample: the host sampler's
+# output (tests/cpp/sampler_check.cpp prints "N dx particle_volume particle_mass" and then the coordinates),
+# so that the CPU oracle and the HIP engine run the same particles as tests/cpp/reference_tests.cpp.
+def read_sample(text):
+    lines = text.strip().splitlines()
+    n, dx, vol, mass = lines[0].split()
+    x = np.array([[float(v) for v in ln.split()] for ln in lines[1:]], dtype=np.float64)
+    assert x.shape[0] == int(n)
+    return x, float(dx), float(vol), float(mass)
+
+
+def host_sample(exe, r, L, ppc, rho, seed=42):
+    """Runs the sampler_check program `exe` (3D build) and returns (x, dx, particle_volume, particle_mass)."""
+    import subprocess
+    out = subprocess.run([exe, repr(r), repr(L[0]), repr(L[1]), repr(L[2]), repr(ppc), repr(rho), str(seed)],
+                         capture_output=True, text=True, check=True).stdout
+    return read_sample(out)
+
+
+def mibf_slope(lib, sample, mibf, mises, n_threads=1):
+    """One run of BoundaryTest.MIBF (tests/tests.cpp:308-474) on a given sample of the 0.1 x 0.05 x 0.05 block:
+    DPVisc with zero viscosity, 32 deg slope, SlipFree floor with Coulomb mu = tan 30 deg, 0.5 s. The three
+    runs of the test: plate friction (mibf = mises = False), MIBF grid friction (mibf), von Mises q with
+    M = sqrt(3) tan 30 deg (mises). Returns the maximum x."""
+    x0, dx, vol, mass = sample
+    x = x0.copy()
+    x[:, 0] -= 0.5 * 0.1
+    x[:, 1] += 0.5 * dx
+    friction = math.tan(math.radians(30.0))
+    theta = math.radians(32.0)
+    cfg = A.default_config(dim=3, dx=dx, particle_volume=vol, particle_mass=mass, rho=1000.0, end_frame=1, fps=2.0,
+                           cfl=0.5, flip_ratio=-0.95, E=1e5, nu=0.3, n_threads=n_threads,
+                           gravity=[G * math.sin(theta), -G * math.cos(theta), 0.0], grid_reference_point=[0, 0, 0],
+                           plastic_model="DPVisc", use_pradhana=0, use_mises_q=int(mises), use_mibf=int(mibf),
+                           M=math.sqrt(3.0) * friction if mises else friction, q_cohesion=0.0, perzyna_exp=1.0,
+                           perzyna_visc=0.0)
+    e = A.Engine(lib, cfg, [A.make_plate(0.0, "bottom", "SlipFree", friction)])
+    e.set_particles(x)
+    run_to_end(e)
+    return float(e.get_particles()["x"][:, 0].max())
+
+
+def bulk_modulus_on_sample(lib, sample, n_threads=1):
+    """ElasticityTest.BulkModulus (tests/tests.cpp:476-541) on a given sample of the 1 x 1 x 0.2 slab: returns
+    (measured K, lambda + 2 mu / 3, J average), the measured K from computeAvgData's Kirchhoff average
+    (save_data.cpp:264-292) as the reference test computes it."""
+    x, dx, vol, mass = sample
+    cfg = A.default_config(dim=3, dx=dx, particle_volume=vol, particle_mass=mass, rho=10000.0, end_frame=100, fps=1.0,
+                           cfl=0.6, flip_ratio=-0.95, E=1e6, nu=0.3, n_threads=n_threads)
+    vel = 0.001
+    plates = [A.make_plate(0 - 0.5 * dx, "bottom", "NoSlip", 0.0, v_object=(0.0, vel, 0.0), vmin_factor=10.0, load_factor=1.0),
+              A.make_plate(1.0 + 0.5 * dx, "top", "NoSlip", 0.0, v_object=(0.0, -vel, 0.0), vmin_factor=10.0, load_factor=1.0)]
+    e = A.Engine(lib, cfg, plates)
+    e.set_particles(x)
+    run_to_end(e)
+    p = e.get_particles()
+    E, nu, dim = 1e6, 0.3, 3
+    lam, mu = nu * E / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    K = lam + 2 * mu / dim
+    F = p["F"]
+    w, Q = np.linalg.eigh(F @ F.transpose(0, 2, 1))
+    lnV = np.einsum("nij,nj,nkj->nik", Q, 0.5 * np.log(w), Q)
+    tau = 2 * mu * lnV + lam * np.trace(lnV, axis1=1, axis2=2)[:, None, None] * np.eye(dim)
+    J = np.linalg.det(F) * np.exp(p["eps_pl_vol"])
+    kirch = (tau * J[:, None, None]).sum(axis=0) / J.sum()
+    Javg = J.sum() / F.shape[0]
+    return -np.trace(kirch) / dim / (-math.log(Javg)), K, Javg

@@ -14,10 +14,18 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "cpp", "device_svd_check.cpp")
 
 
+# The engine compiles with -ffp-contract=off and lets products and sums fuse only inside the functions marked
+# `#pragma clang fp contract(fast)` (the Jacobi rotations, 3x3 products, Hencky stress). The host build uses
+# the same compiler family and flags, with FMA instructions enabled, so that those functions round here as
+# they do on the device (g++ ignores the clang pragma and would check an unfused variant).
+CLANG = "/opt/rocm/llvm/bin/clang++"
+
+
 @pytest.fixture(scope="module")
 def prog(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("svd") / "device_svd_check")
-    subprocess.run(["g++", "-O2", "-std=c++17", SRC, "-o", exe], check=True)
+    assert os.path.exists(CLANG), f"{CLANG} is missing"
+    subprocess.run([CLANG, "-O2", "-std=c++17", "-mfma", "-ffp-contract=off", SRC, "-o", exe], check=True)
     return exe
 
 

@@ -80,3 +80,28 @@ def test_admission_rejects_what_cannot_fit(engine):
     rc, st = srv.run(60.0)
     assert rc == 0 and st.completed == len(reqs) and st.peak_resident == 1
     srv.close()
+
+
+def test_time_limit_requeues_then_a_second_run_serves_all(engine):
+    """A run stopped by its time limit returns the requests it had admitted (waiting for preparation,
+    prepared or resident) to the queue with their inputs; a second run serves them from the start, and every
+    result equals the scene advanced alone."""
+    print("served: time limit, then resume", flush=True)
+    reqs = requests()
+    cfg, p, o, f = PS.dp_collapse(dim=3, musl=True, n=40)  # 64,000 particles: still running at the limit
+    reqs.append((cfg, p, o, f, 60, 0.0))
+    srv = A.Server(engine, chunk_steps=4, keep_results=True)
+    ids = [srv.submit(cfg, p, o, f["x"], steps, t, **{k: v for k, v in f.items() if k != "x"})
+           for cfg, p, o, f, steps, t in reqs]
+    rc1, st1 = srv.run(0.001)
+    assert rc1 == 0 and st1.failed == 0 and st1.completed < len(reqs)
+    rc2, st2 = srv.run(60.0)
+    print(st1.completed, st2.completed, flush=True)
+    assert rc2 == 0 and st2.failed == 0 and st1.completed + st2.completed == len(reqs)
+    for rid, (cfg, p, o, f, steps, _) in zip(ids, reqs):
+        got, info, lat = srv.result(rid)
+        ref, rinfo = alone(engine, cfg, p, o, f, steps)
+        assert info.steps_done == steps and info.time == rinfo.time
+        for k in FIELDS:
+            assert np.array_equal(got[k], ref[k], equal_nan=True), f"request {rid}: {k} differs from the scene alone"
+    srv.close()

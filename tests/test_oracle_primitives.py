@@ -16,12 +16,10 @@ import numpy as np
 import pytest
 
 import mpm_abi as A
+import returnmap as RM
+from returnmap import consts, dptr, pq, project
 
 rng = np.random.default_rng(20240611)
-
-
-def dptr(a):
-    return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
 # ---------------------------------------------------------------- B-splines (src/tools.hpp:108-188)
@@ -163,41 +161,8 @@ def test_neohookean_kirchhoff(oracle):
 
 
 # ---------------------------------------------------------------- return maps (plasticity.cpp)
-def project(oracle, cfg, F, dt=1e-4, scalars=None):
-    dim = cfg.dim
-    f = np.asarray(F, dtype=float).flatten(order="F").copy()
-    out = np.zeros(dim * dim)
-    sc = np.zeros(6) if scalars is None else np.array(scalars, dtype=float)
-    plastic = C.c_uint32()
-    err = oracle.lib.oracle_plasticity_single(C.byref(cfg), dt, dptr(f), dptr(sc), dptr(out), C.byref(plastic))
-    return out.reshape(dim, dim, order="F"), sc, plastic.value, err
-
-
-def pq(F, cfg, K, e_mu):
-    s = np.linalg.svd(F, compute_uv=False)
-    h = np.log(s)
-    dev = h - h.mean()
-    return -K * h.sum(), e_mu * np.linalg.norm(dev), h, dev
-
-
-def consts(cfg):
-    E, nu, dim = cfg.E, cfg.nu, cfg.dim
-    lam, mu = nu * E / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
-    K = lam + 2 * mu / dim
-    qp = math.sqrt(1.5) if cfg.use_mises_q else 1 / math.sqrt(2)
-    dp = math.sqrt(2 / 3) if cfg.use_mises_q else math.sqrt(2)
-    return dict(lam=lam, mu=mu, K=K, e_mu=2 * qp * mu, f_mu=2 * qp / dp * mu, d=dp)
-
-
 def trial(dim, vol, dev_mag):
-    """Rotated trial deformation with Hencky trace `vol` and deviatoric magnitude `dev_mag`."""
-    d = np.array([1.0, -0.4, -0.6][:dim])
-    d = d - d.mean()
-    d = d / np.linalg.norm(d) * dev_mag
-    h = d + vol / dim
-    R = np.linalg.qr(rng.standard_normal((dim, dim)))[0]
-    Q = np.linalg.qr(rng.standard_normal((dim, dim)))[0]
-    return R @ np.diag(np.exp(h)) @ Q, h
+    return RM.trial(dim, vol, dev_mag, rng)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
