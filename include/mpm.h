@@ -46,7 +46,8 @@ enum mpm_object_kind {
     MPM_OBJECT_RAMP        = 4, /* ObjectRamp        src/objects/object_ramp.hpp:8-41   */
     MPM_OBJECT_GATE        = 5, /* ObjectGate        src/objects/object_gate.hpp:8-43   */
     MPM_OBJECT_SILO        = 6, /* ObjectSilo (3D)   src/objects/object_silo.hpp:8-56   */
-    MPM_OBJECT_CURVE       = 7  /* ObjectCurve       src/objects/object_curve.hpp:8-39  */
+    MPM_OBJECT_CURVE       = 7, /* ObjectCurve       src/objects/object_curve.hpp:8-39  */
+    MPM_OBJECT_SDF         = 8  /* ObjectVdb         src/objects/object_vdb.hpp:17-94 (mpm_add_sdf_object) */
 };
 
 /* Arithmetic form of the step. */
@@ -138,6 +139,21 @@ typedef struct mpm_object {
     double rotation[9];    /* BOX_ROTATED, dim x dim column-major (first dim*dim entries used) */
 } mpm_object;
 
+/* Level-set boundary object sampled on a regular lattice: the data of ObjectVdb (src/objects/object_vdb.hpp:17-94),
+ * whose OpenVDB grid a caller copies to a dense array (e.g. openvdb::tools::copyToDense over its active bounding
+ * box). Lattice point (i, j, k) sits at origin + spacing * (i, j, k) and holds phi[(i*n[1] + j)*n[2] + k], the
+ * signed distance (negative inside); n[2] = 1 in 2D. */
+typedef struct mpm_sdf {
+    int32_t bc;            /* enum mpm_bc */
+    int32_t reserved;
+    double friction;
+    double origin[3];
+    double spacing;
+    uint32_t n[3];         /* lattice points per axis, at least 2 (n[2] = 1 in 2D) */
+    uint32_t reserved1;
+    const float* phi;      /* n[0]*n[1]*n[2] values, copied by the call */
+} mpm_sdf;
+
 /* Host views of particle state (Particles, src/data_structures.hpp:8-47). Any pointer may be NULL
  * to skip that field. Sizes: x, v: n*dim; F, Bmat: n*dim*dim; scalars: n. */
 typedef struct mpm_particle_arrays {
@@ -195,6 +211,15 @@ int mpm_add_plate(mpm_context* ctx, const mpm_plate* plate);
 /* Appends an analytic object. Replaces `objects.push_back(std::make_unique<Object...>(...))`
  * (simulation.hpp:110, object_general.hpp:8-23). Objects are applied in insertion order, before plates. */
 int mpm_add_object(mpm_context* ctx, const mpm_object* object);
+
+/* Appends a level-set object. Replaces `objects.push_back(std::make_unique<ObjectVdb>(file, bc, friction))`
+ * (simulation.hpp:110, object_vdb.hpp:26-42); applied in insertion order with the other objects. As ObjectVdb:
+ * inside(X) is phi(X) <= 0 with phi interpolated trilinearly (bilinearly in 2D) between lattice points (the
+ * BoxSampler of object_vdb.hpp:44-56); normal(X) is the likewise interpolated gradient of phi, taken by central
+ * differences at the lattice points (one-sided at the border; openvdb::tools::Gradient), normalised, or zero
+ * where it vanishes (:58-77). Outside the lattice's box the object is absent (inside false). The lattice is
+ * copied to the device and held by the context. MPM_ERR_INVALID_ARGUMENT for a bad bc, spacing or size. */
+int mpm_add_sdf_object(mpm_context* ctx, const mpm_sdf* sdf);
 
 /* Uploads particle state (Np = n). Fields passed as NULL take the values of the reference's
  * Particles constructor (data_structures.hpp:10-27: zeros, F = I). Replaces writing

@@ -68,6 +68,8 @@ struct mpm_context {
     mpm::Params* dP = nullptr;
     uint32_t budget = 0;
     mpm_step_info last{};
+    // device lattices of the level-set objects (mpm_add_sdf_object)
+    std::vector<void*> sdf_bufs;
 };
 
 namespace mpm {

@@ -704,6 +704,7 @@ void mpm_destroy(mpm_context* c) {
     if (c->S) (void)hipFree(c->S);
     if (c->Sh) (void)hipHostFree(c->Sh);
     if (c->dP) (void)hipFree(c->dP);
+    for (void* q : c->sdf_bufs) (void)hipFree(q);
     for (hipEvent_t ev : c->ev_pool) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -741,6 +742,51 @@ int mpm_add_object(mpm_context* c, const mpm_object* o) {
     const int D = c->cfg.dim;
     // the rotation arrives as dim x dim column-major
     for (int i = 0; i < D * D; ++i) d.rot[i] = o->rotation[i];
+    c->P.n_objects++;
+    return 0;
+}
+
+int mpm_add_sdf_object(mpm_context* c, const mpm_sdf* sd) {
+    if (!c || !sd || !sd->phi || c->batch) return MPM_ERR_INVALID_ARGUMENT;
+    if (c->P.n_objects >= kMaxObjects) return MPM_ERR_UNSUPPORTED;
+    const int D = c->cfg.dim;
+    if (!valid_enum(sd->bc, 0, 2) || !(sd->spacing > 0) || !std::isfinite(sd->spacing)) return MPM_ERR_INVALID_ARGUMENT;
+    uint64_t npts = 1;
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t want_min = d < D ? 2u : 1u;
+        if (sd->n[d] < want_min || (d >= D && sd->n[d] != 1)) return MPM_ERR_INVALID_ARGUMENT;
+        npts *= sd->n[d];
+    }
+    if (npts > (1ull << 28)) return MPM_ERR_INVALID_ARGUMENT;
+    // phi and its gradient per lattice point: central differences, one-sided at the border
+    std::vector<float> field(4 * npts, 0.0f);
+    const uint32_t n0 = sd->n[0], n1 = sd->n[1], n2 = sd->n[2];
+    const double h = sd->spacing;
+    auto at = [&](uint32_t i, uint32_t j, uint32_t k) { return (double)sd->phi[((uint64_t)i * n1 + j) * n2 + k]; };
+    for (uint32_t i = 0; i < n0; ++i)
+        for (uint32_t j = 0; j < n1; ++j)
+            for (uint32_t k = 0; k < n2; ++k) {
+                const uint64_t q = ((uint64_t)i * n1 + j) * n2 + k;
+                const uint32_t idx[3] = {i, j, k};
+                field[4 * q] = sd->phi[q];
+                for (int d = 0; d < D; ++d) {
+                    uint32_t lo[3] = {i, j, k}, hi[3] = {i, j, k};
+                    if (idx[d] > 0) lo[d] = idx[d] - 1;
+                    if (idx[d] + 1 < sd->n[d]) hi[d] = idx[d] + 1;
+                    const double g = (at(hi[0], hi[1], hi[2]) - at(lo[0], lo[1], lo[2])) / ((double)(hi[d] - lo[d]) * h);
+                    field[4 * q + 1 + d] = (float)g;
+                }
+            }
+    void* dbuf = nullptr;
+    if (hipSetDevice(c->device) != hipSuccess) return MPM_ERR_HIP;
+    if (hipMalloc(&dbuf, field.size() * sizeof(float)) != hipSuccess) return MPM_ERR_OUT_OF_MEMORY;
+    if (hipMemcpy(dbuf, field.data(), field.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { (void)hipFree(dbuf); return MPM_ERR_HIP; }
+    c->sdf_bufs.push_back(dbuf);
+    DevObject& o = c->P.objects[c->P.n_objects];
+    std::memset(&o, 0, sizeof(o));
+    o.kind = MPM_OBJECT_SDF; o.bc = sd->bc; o.friction = sd->friction; o.param = h;
+    for (int d = 0; d < 3; ++d) { o.center[d] = d < D ? sd->origin[d] : 0.0; o.sdf.n[d] = sd->n[d]; }
+    o.sdf.field = (const float*)dbuf;
     c->P.n_objects++;
     return 0;
 }

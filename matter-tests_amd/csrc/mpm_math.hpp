@@ -751,7 +751,38 @@ template <int D> MPM_HD V<D> normalized_v(V<D> n) {
     return n;
 }
 
-// inside()/normal() of the analytic objects (src/objects/object_*.hpp).
+// Level set of an MPM_OBJECT_SDF object at X: phi and its gradient interpolated (trilinear in 3D, bilinear in
+// 2D, the BoxSampler of object_vdb.hpp:44-77) from the lattice's 2^dim surrounding points. False outside the
+// lattice's box.
+template <int D> MPM_HD bool sdf_sample(const DevObject& o, const V<D>& X, double& phi, double* g) {
+    const DevSdf& L = o.sdf;
+    int i0[3] = {0, 0, 0};
+    double f[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+        const double u = (X[d] - o.center[d]) / o.param;
+        if (!(u >= 0.0 && u <= (double)(L.n[d] - 1))) return false;
+        const int fi = (int)floor(u), last = (int)L.n[d] - 2;
+        i0[d] = fi < last ? fi : last;
+        f[d] = u - (double)i0[d];
+    }
+    phi = 0.0;
+    for (int d = 0; d < 3; ++d) g[d] = 0.0;
+    for (int c = 0; c < (1 << D); ++c) {
+        double w = 1.0;
+        uint32_t idx = 0;
+        for (int d = 0; d < D; ++d) {
+            const int b = (c >> (D - 1 - d)) & 1;
+            w = w * (b ? f[d] : 1.0 - f[d]);
+            idx = idx * L.n[d] + (uint32_t)(i0[d] + b);
+        }
+        const float* s = L.field + 4 * (size_t)idx;
+        phi += w * (double)s[0];
+        for (int d = 0; d < D; ++d) g[d] += w * (double)s[1 + d];
+    }
+    return true;
+}
+
+// inside()/normal() of the analytic objects (src/objects/object_*.hpp) and of sampled level sets.
 template <int D> MPM_HD bool object_inside(const DevObject& o, const V<D>& X) {
     const double x = X[0], y = X[1];
     switch (o.kind) {
@@ -774,6 +805,10 @@ template <int D> MPM_HD bool object_inside(const DevObject& o, const V<D>& X) {
         return !(x * x + z * z < rs * rs);
     }
     case MPM_OBJECT_CURVE: return !(y > x * x);                                 // object_curve.hpp:15-25
+    case MPM_OBJECT_SDF: {                                                      // object_vdb.hpp:44-56
+        double phi, g[3];
+        return sdf_sample<D>(o, X, phi, g) && phi <= 0.0;
+    }
     default: return false;
     }
 }
@@ -797,6 +832,11 @@ template <int D> MPM_HD V<D> object_normal(const DevObject& o, const V<D>& X) {
     case MPM_OBJECT_GATE: { double b = 2.0 * 100 * x; n[0] = -b; n[1] = 1; for (int i = 0; i < D; ++i) n[i] *= -1; return normalized_v(n); }
     case MPM_OBJECT_SILO: { double z = X[D - 1], y = X[1]; double th = atan2(z, x); double sy = 1.0 / cosh(y); n[0] = -cos(th); n[1] = sy * sy; n[D - 1] = -sin(th); return normalized_v(n); }
     case MPM_OBJECT_CURVE: n[0] = -2 * x; n[1] = 1; return normalized_v(n);
+    case MPM_OBJECT_SDF: {                                                      // object_vdb.hpp:58-77
+        double phi, g[3];
+        if (sdf_sample<D>(o, X, phi, g)) for (int i = 0; i < D; ++i) n[i] = g[i];
+        return normalized_v(n);
+    }
     default: return n;
     }
 }

@@ -16,12 +16,27 @@ struct DevPlate {
     int32_t plate_type, bc;
 };
 
-// Analytic object (ObjectGeneral subclasses, src/objects/object_*.hpp). Immutable.
+// Sampled level set of an MPM_OBJECT_SDF object: per lattice point (i, j, k), at index (i*n[1] + j)*n[2] + k,
+// four floats: phi, dphi/dx, dphi/dy, dphi/dz (gradient by central differences, one-sided at the border).
+struct DevSdf {
+    const float* field;
+    uint32_t n[3];  // points per axis (n[2] = 1 in 2D)
+    uint32_t pad;
+};
+
+// Boundary object (ObjectGeneral subclasses, src/objects/object_*.hpp, or a sampled level set). Immutable.
+// The kind-specific data share storage so that Params (passed to every kernel by value) stays small.
 struct DevObject {
     int32_t kind, bc;
-    double friction, param;
-    double center[3], half_extents[3];
-    double rot[9];  // dim x dim column-major
+    double friction, param;  // param: GROUND y, GATE height, SILO cut, SDF lattice spacing
+    double center[3];        // BOX_ROTATED centre; SDF position of lattice point (0, 0, 0)
+    union {
+        struct {
+            double half_extents[3];
+            double rot[9];   // dim x dim column-major
+        };
+        DevSdf sdf;
+    };
 };
 
 // Constants of a context: the simulate() set-up values (src/simulation/simulation.cpp:94-145) and the

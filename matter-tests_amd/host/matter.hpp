@@ -92,6 +92,8 @@ public:
     ObjectGeneral(BC bc_, T friction_, std::string name_) : bc(bc_), friction(friction_), name(std::move(name_)) {}
     virtual ~ObjectGeneral() {}
     virtual mpm_object descriptor() const = 0;
+    // Appends the object to a context (mpm_add_object; the level-set object uses mpm_add_sdf_object).
+    virtual int attach(mpm_context* ctx) const { const mpm_object d = descriptor(); return mpm_add_object(ctx, &d); }
 protected:
     mpm_object base(int kind) const {
         mpm_object o; std::memset(&o, 0, sizeof(o));
@@ -143,6 +145,24 @@ class ObjectCurve : public ObjectGeneral {  // object_curve.hpp:13
 public:
     ObjectCurve(BC b = BC::NoSlip, T f = 0.0, std::string n = "") : ObjectGeneral(b, f, n) {}
     mpm_object descriptor() const override { return base(MPM_OBJECT_CURVE); }
+};
+
+// ObjectVdb, src/objects/object_vdb.hpp:17-94: the level set as a dense lattice of signed distances (a caller
+// with OpenVDB copies its grid's active bounding box to dense, openvdb::tools::copyToDense), point (i, j, k) at
+// origin + spacing * (i, j, k), index (i*n[1] + j)*n[2] + k, n[2] = 1 in 2D.
+class ObjectSdf : public ObjectGeneral {
+public:
+    std::vector<float> phi; unsigned n[3]; TV origin; T spacing;
+    ObjectSdf(std::vector<float> phi_, const unsigned* n_, const TV& origin_, T spacing_, BC b = BC::NoSlip, T f = 0.0,
+              std::string nm = "")
+        : ObjectGeneral(b, f, nm), phi(std::move(phi_)), origin(origin_), spacing(spacing_) { for (int d = 0; d < 3; ++d) n[d] = n_[d]; }
+    mpm_object descriptor() const override { return base(MPM_OBJECT_SDF); }
+    int attach(mpm_context* ctx) const override {
+        mpm_sdf s; std::memset(&s, 0, sizeof(s));
+        s.bc = (int32_t)bc; s.friction = friction; s.spacing = spacing; s.phi = phi.data();
+        for (int d = 0; d < 3; ++d) { s.n[d] = n[d]; s.origin[d] = d < kDim ? origin(d) : 0.0; }
+        return mpm_add_sdf_object(ctx, &s);
+    }
 };
 
 // ObjectPlate, src/objects/object_plate.hpp:13-28 (3D) / :83-96 (2D): same constructor arguments.
@@ -341,7 +361,7 @@ private:
         if (ctx) { mpm_destroy(ctx); ctx = nullptr; }
         int rc = mpm_create(&c, &ctx);
         for (auto& p : plates) { if (rc) break; mpm_plate d = p->descriptor(); rc |= mpm_add_plate(ctx, &d); }
-        for (auto& o : objects) { if (rc) break; mpm_object d = o->descriptor(); rc |= mpm_add_object(ctx, &d); }
+        for (auto& o : objects) { if (rc) break; rc |= o->attach(ctx); }
         if (!rc) {
             mpm_particle_arrays a = arrays();
             rc = mpm_set_particles(ctx, Np, &a);

@@ -19,7 +19,7 @@ ELASTIC = dict(Hencky=0, NeoHookean=1)
 PLASTIC = dict(NoPlasticity=0, VM=1, DP=2, DPSoft=3, MCC=4, VMVisc=5, DPVisc=6, MCCVisc=7, DPMui=8, MCCMui=9, Snow=10)
 HARDENING = dict(NoHard=0, ExpoExpl=1, ExpoImpl=2, SinhExpl=3, SinhImpl=4)
 BC = dict(NoSlip=0, SlipStick=1, SlipFree=2)
-OBJECT = dict(BoxRotated=1, Bump=2, Ground=3, Ramp=4, Gate=5, Silo=6, Curve=7)
+OBJECT = dict(BoxRotated=1, Bump=2, Ground=3, Ramp=4, Gate=5, Silo=6, Curve=7, Sdf=8)
 PRECISION = dict(fp64=0, split=1)
 
 ERR = dict(INVALID_ARGUMENT=1 << 0, UNSUPPORTED=1 << 1, HIP=1 << 2, OUT_OF_MEMORY=1 << 3,
@@ -60,6 +60,11 @@ class Plate(C.Structure):
 class Object(C.Structure):
     _fields_ = [("kind", C.c_int32), ("bc", C.c_int32), ("friction", C.c_double), ("param", C.c_double),
                 ("center", C.c_double * 3), ("half_extents", C.c_double * 3), ("rotation", C.c_double * 9)]
+
+
+class Sdf(C.Structure):
+    _fields_ = [("bc", C.c_int32), ("reserved", C.c_int32), ("friction", C.c_double), ("origin", C.c_double * 3),
+                ("spacing", C.c_double), ("n", C.c_uint32 * 3), ("reserved1", C.c_uint32), ("phi", C.POINTER(C.c_float))]
 
 
 class ParticleArrays(C.Structure):
@@ -275,6 +280,7 @@ class Library:
             L.oracle_object_eval.argtypes = [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]
             L.oracle_object_eval.restype = C.c_int
         for name, args in (("add_plate", [vp, C.POINTER(Plate)]), ("add_object", [vp, C.POINTER(Object)]),
+                           ("add_sdf_object", [vp, C.POINTER(Sdf)]),
                            ("set_particles", [vp, C.c_uint32, C.POINTER(ParticleArrays)]),
                            ("step", [vp, C.c_uint32, C.c_uint32, C.POINTER(StepInfo)]),
                            ("get_particles", [vp, C.POINTER(ParticleArrays)]),
@@ -334,6 +340,19 @@ class Engine:
         for o in objects:
             self._check(lib.fn("add_object")(self.h, C.byref(o)), "add_object")
         self.n = 0
+
+    def add_sdf(self, phi, origin, spacing, bc="NoSlip", friction=0.0):
+        """Level-set object from a lattice of signed distances phi[i, j(, k)] (mpm_add_sdf_object)."""
+        phi = np.ascontiguousarray(phi, dtype=np.float32)
+        n = list(phi.shape) + [1] * (3 - phi.ndim)
+        sd = Sdf()
+        sd.bc = BC[bc] if isinstance(bc, str) else bc
+        sd.friction, sd.spacing = friction, spacing
+        for i in range(3):
+            sd.origin[i] = origin[i] if i < len(origin) else 0.0
+            sd.n[i] = n[i]
+        sd.phi = phi.ctypes.data_as(C.POINTER(C.c_float))
+        self._check(self.L.fn("add_sdf_object")(self.h, C.byref(sd)), "add_sdf_object")
 
     def _check(self, rc, what):
         if rc:

@@ -24,6 +24,7 @@ struct OracleBase {
     virtual double bulk_modulus() const = 0;
     virtual void phase_times(double*) const = 0;
     virtual int object_eval(uint32_t, const double*, int*, double*) const = 0;
+    virtual int add_sdf(const mpm_sdf*) = 0;
 };
 
 template <int D, int SPL> struct OracleImpl : OracleBase {
@@ -176,6 +177,11 @@ template <int D, int SPL> struct OracleImpl : OracleBase {
     void phase_times(double* t) const override {
         t[0] = sim.runtime_p2g; t[1] = sim.runtime_euler; t[2] = sim.runtime_g2p; t[3] = sim.runtime_defgrad;
     }
+    int add_sdf(const mpm_sdf* sd) override {
+        unsigned nn[3] = {sd->n[0], sd->n[1], sd->n[2]};
+        sim.objects.push_back(std::make_unique<ObjectSdf<D>>((BC)sd->bc, sd->friction, sd->origin, sd->spacing, nn, sd->phi));
+        return 0;
+    }
     int object_eval(uint32_t k, const double* X, int* inside, double* normal) const override {
         if (k >= sim.objects.size()) return MPM_ERR_INVALID_ARGUMENT;
         Vec<D> x; for (int d = 0; d < D; ++d) x(d) = X[d];
@@ -210,6 +216,7 @@ void* oracle_create(const mpm_config* cfg) { return cfg ? make(cfg) : nullptr; }
 void oracle_destroy(void* h) { delete (OracleBase*)h; }
 int oracle_add_plate(void* h, const mpm_plate* p) { return ((OracleBase*)h)->add_plate(p); }
 int oracle_add_object(void* h, const mpm_object* o) { return ((OracleBase*)h)->add_object(o); }
+int oracle_add_sdf_object(void* h, const mpm_sdf* s) { return ((OracleBase*)h)->add_sdf(s); }
 int oracle_set_particles(void* h, uint32_t n, const mpm_particle_arrays* in) { return ((OracleBase*)h)->set_particles(n, in); }
 int oracle_step(void* h, uint32_t nsteps, uint32_t flags, mpm_step_info* info) { return ((OracleBase*)h)->step(nsteps, flags, info); }
 int oracle_get_particles(void* h, mpm_particle_arrays* out) { return ((OracleBase*)h)->get_particles(out); }

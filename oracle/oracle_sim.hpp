@@ -149,6 +149,64 @@ template <int D> struct ObjectCurve : ObjectGeneral<D> {
     }
 };
 
+// ObjectVdb, src/objects/object_vdb.hpp:17-94, on a dense lattice instead of an OpenVDB grid (OpenVDB is not
+// available): point (i, j, k) at origin + h (i, j, k) holds phi; the gradient grid of openvdb::tools::Gradient is
+// formed by central differences (one-sided at the lattice border); both are sampled by trilinear (2D: bilinear)
+// interpolation, the BoxSampler of :44-77. Outside the lattice box the object is absent.
+template <int D> struct ObjectSdf : ObjectGeneral<D> {
+    Vec<D> origin; T h; unsigned n[3];
+    std::vector<float> field;  // per point: phi, dphi/dx, dphi/dy, dphi/dz
+    ObjectSdf(BC b, T f, const double* org, T spacing, const unsigned* nn, const float* phi) : ObjectGeneral<D>(b, f), h(spacing) {
+        for (int d = 0; d < D; ++d) origin(d) = org[d];
+        for (int d = 0; d < 3; ++d) n[d] = d < D ? nn[d] : 1;
+        const size_t npts = (size_t)n[0] * n[1] * n[2];
+        field.assign(4 * npts, 0.0f);
+        for (unsigned i = 0; i < n[0]; ++i)
+            for (unsigned j = 0; j < n[1]; ++j)
+                for (unsigned k = 0; k < n[2]; ++k) {
+                    const size_t q = ((size_t)i * n[1] + j) * n[2] + k;
+                    field[4 * q] = phi[q];
+                    const unsigned c[3] = {i, j, k};
+                    for (int d = 0; d < D; ++d) {
+                        unsigned a[3] = {i, j, k}, b[3] = {i, j, k};
+                        a[d] = c[d] > 0 ? c[d] - 1 : c[d];
+                        b[d] = c[d] + 1 < n[d] ? c[d] + 1 : c[d];
+                        const double pa = phi[((size_t)a[0] * n[1] + a[1]) * n[2] + a[2]];
+                        const double pb = phi[((size_t)b[0] * n[1] + b[1]) * n[2] + b[2]];
+                        field[4 * q + 1 + d] = (float)((pb - pa) / ((double)(b[d] - a[d]) * h));
+                    }
+                }
+    }
+    bool sample(const Vec<D>& X, T& phi, T* g) const {
+        int i0[3] = {0, 0, 0}; T fr[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) {
+            const T u = (X(d) - origin(d)) / h;
+            if (!(u >= 0.0 && u <= (T)(n[d] - 1))) return false;
+            const int fi = (int)std::floor(u), last = (int)n[d] - 2;
+            i0[d] = fi < last ? fi : last;
+            fr[d] = u - (T)i0[d];
+        }
+        phi = 0; for (int d = 0; d < 3; ++d) g[d] = 0;
+        for (int c = 0; c < (1 << D); ++c) {
+            T w = 1.0; size_t idx = 0;
+            for (int d = 0; d < D; ++d) {
+                const int b = (c >> (D - 1 - d)) & 1;
+                w = w * (b ? fr[d] : 1.0 - fr[d]);
+                idx = idx * n[d] + (size_t)(i0[d] + b);
+            }
+            phi += w * (T)field[4 * idx];
+            for (int d = 0; d < D; ++d) g[d] += w * (T)field[4 * idx + 1 + d];
+        }
+        return true;
+    }
+    bool inside(const Vec<D>& X) const override { T phi, g[3]; return sample(X, phi, g) && phi <= 0; }
+    Vec<D> normal(const Vec<D>& X) const override {
+        T phi, g[3]; Vec<D> r = Vec<D>::Zero();
+        if (sample(X, phi, g)) for (int d = 0; d < D; ++d) r(d) = g[d];
+        return normalized(r);
+    }
+};
+
 // ObjectPlate, src/objects/object_plate.hpp:8-190
 template <int D> struct ObjectPlate {
     T pos_object; PlateType plate_type; BC bc; T friction; T pos_upper; T pos_lower;

@@ -129,6 +129,16 @@ def object_scene(kind, bc, dim=3, friction=0.3, param=0.0, n=10):
     return cfg, [], [o], {"x": x, "v": v}
 
 
+def sdf_scene(bc, dim=3, friction=0.3, n=10):
+    """Block falling onto the level set of the curve y = x^2 sampled on a lattice (ObjectVdb,
+    src/objects/object_vdb.hpp), placed as object_scene places it on the analytic curve."""
+    import physics_cases as PC
+    cfg, plates, _, f = object_scene("Curve", bc, dim=dim, friction=friction, n=n)
+    phi, origin, h = PC.curve_lattice(dim, h=0.004, x=(-0.1, 0.2), y=(-0.1, 0.15), z=(-0.05, 0.1))
+    f["sdf"] = [(phi, origin, h, bc, friction)]
+    return cfg, plates, [], f
+
+
 def moving_plates(dim=3, n=10):
     """Plates moving with vmin_factor/load_factor (ElasticityTest.BulkModulus, tests/tests.cpp:476-541)."""
     dx = 0.01
@@ -223,6 +233,8 @@ SCENES = {
     "obj_bump_slipfree": lambda: object_scene("Bump", "SlipFree"),
     "obj_silo_slipstick": lambda: object_scene("Silo", "SlipStick", param=-1.0),
     "obj_box_rotated": lambda: object_scene("BoxRotated", "SlipFree"),
+    "obj_sdf_curve_slipfree": lambda: sdf_scene("SlipFree"),
+    "obj_sdf_curve_slipstick_2d": lambda: sdf_scene("SlipStick", dim=2),
     "moving_plates": lambda: moving_plates(),
     "gravity_ramp_2d": lambda: gravity_ramp(),
     "periodic2d_usl_apic": lambda: periodic(2),
@@ -236,6 +248,8 @@ def build(lib, name, n_threads=1):
     cfg, plates, objs, fields = SCENES[name]()
     cfg.n_threads = n_threads
     e = A.Engine(lib, cfg, plates, objs)
+    for phi, origin, h, bc, friction in fields.pop("sdf", []):
+        e.add_sdf(phi, origin, h, bc, friction)
     x = fields.pop("x")
     e.set_particles(x, **fields)
     return e

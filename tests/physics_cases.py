@@ -236,3 +236,62 @@ def bulk_modulus_on_sample(lib, sample, n_threads=1):
     kirch = (tau * J[:, None, None]).sum(axis=0) / J.sum()
     Javg = J.sum() / F.shape[0]
     return -np.trace(kirch) / dim / (-math.log(Javg)), K, Javg
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Level-set terrain (ObjectVdb, src/objects/object_vdb.hpp) on a lattice: the signed distance to the parabola
+# y = x^2 (the curve of the reference's tests/curve.vdb), negative below it.
+def parabola_sdf(xs, ys):
+    """Signed distance of the points (xs[i], ys[j]) to y = x^2: the nearest curve point t solves
+    2 t^3 + (1 - 2 y) t - x = 0 (a depressed cubic t^3 + p t + q = 0), taken over its real roots by Cardano's
+    formula (one real root) or the trigonometric form (three)."""
+    X, Y = np.meshgrid(np.asarray(xs, float), np.asarray(ys, float), indexing="ij")
+    p = (1.0 - 2.0 * Y) / 2.0
+    q = -X / 2.0
+    disc = (q / 2.0) ** 2 + (p / 3.0) ** 3
+    roots = []
+    sq = np.sqrt(np.maximum(disc, 0.0))
+    roots.append(np.where(disc > 0, np.cbrt(-q / 2.0 + sq) + np.cbrt(-q / 2.0 - sq), np.nan))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        m = 2.0 * np.sqrt(np.maximum(-p / 3.0, 0.0))
+        arg = np.clip(np.where(p < 0, (3.0 * q / (2.0 * p)) * np.sqrt(np.maximum(-3.0 / p, 0.0)), 0.0), -1.0, 1.0)
+        th = np.arccos(arg) / 3.0
+        for k in range(3):
+            roots.append(np.where(disc <= 0, m * np.cos(th - 2.0 * np.pi * k / 3.0), np.nan))
+    d = np.full(X.shape, np.inf)
+    for t in roots:
+        dd = np.sqrt((t - X) ** 2 + (t * t - Y) ** 2)
+        d = np.where(np.isfinite(dd) & (dd < d), dd, d)
+    return np.where(Y <= X * X, -d, d)
+
+
+def curve_lattice(dim, h=0.005, x=(-1.3, 0.4), y=(-0.3, 1.3), z=(-0.02, 0.02)):
+    """(phi, origin, spacing) of parabola_sdf on a lattice covering x, y (and z in 3D, where phi does not vary)."""
+    xs = np.arange(x[0], x[1] + 0.5 * h, h)
+    ys = np.arange(y[0], y[1] + 0.5 * h, h)
+    phi = parabola_sdf(xs, ys)
+    origin = [xs[0], ys[0]]
+    if dim == 3:
+        zs = np.arange(z[0], z[1] + 0.5 * h, h)
+        phi = np.repeat(phi[:, :, None], zs.size, axis=2)
+        origin.append(zs[0])
+    return phi.astype(np.float32), origin, h
+
+
+def sdf_slide(lib, bc="SlipFree", dim=3):
+    """BoundaryTest.VDB (tests/tests_vdb.cpp:40-82): one particle released at (-1, 1) on the level set of the curve
+    y = x^2 (frictionless) reaches v_x = sqrt(2 g) at the bottom, 1e-3 relative."""
+    dx = 0.001
+    vol = dx ** dim
+    cfg = A.default_config(dim=dim, dx=dx, particle_volume=vol, particle_mass=1000.0 * vol, rho=1000.0,
+                           end_frame=100, fps=100 / 0.5949238427, cfl=0.5, flip_ratio=-0.95, E=1e6, nu=0.3,
+                           gravity=[0.0, -G, 0.0])
+    e = A.Engine(lib, cfg)
+    phi, origin, h = curve_lattice(dim)
+    e.add_sdf(phi, origin, h, bc, 0.0)
+    x = np.zeros((1, dim))
+    x[0, 0], x[0, 1] = -1.0, 1.0
+    e.set_particles(x)
+    run_to_end(e)
+    v = e.get_particles()["v"][0, 0]
+    return abs(v - math.sqrt(2 * G)) / math.sqrt(2 * G)
