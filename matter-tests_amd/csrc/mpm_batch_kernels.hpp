@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void kb_bin_scatter(BatchTable T) {
 }
 
 template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
-__global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB)) void kb_p2g(BatchTable T) {
+__global__ __launch_bounds__((ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::TB)) void kb_p2g(BatchTable T) {
     BatchSlot<TF> q(T, BR_BLOCK);
     p2g_body<D, SPL, SEP, MIBF, TF, PBC>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.r->stg, q.local);
 }
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kG2PThreads) void kb_g2p(BatchTable T) {
 }
 
 template <int D, int SPL, class TF>
-__global__ __launch_bounds__((ScatterShape<D, SPL, D, TF>::TB)) void kb_musl_scatter(BatchTable T) {
+__global__ __launch_bounds__((ScatterLaunch<D, SPL, D, TF, false>::TB)) void kb_musl_scatter(BatchTable T) {
     BatchSlot<TF> q(T, BR_BLOCK);
     musl_scatter_body<D, SPL, TF>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.r->stg, q.local);
 }

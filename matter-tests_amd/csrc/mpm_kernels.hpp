@@ -721,8 +721,8 @@ __device__ __forceinline__ int tile_index(int t0, int t1, int t2) {
 // window leaves the grid) go last and are reported. L.cstart keeps the cell-major offsets (cell counts).
 // GHOST: the block's periodic copies (perm entries kGhostBit | p, at their copy positions); they are only
 // ordered in LDS for the scatter, and nothing is written back.
-template <int D, int SPL, int NV, class TF, bool GHOST = false>
-__device__ __forceinline__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, const Params& P, const StepState* S, const PartStateT<TF>& src,
+template <int D, int SPL, int NV, class TF, bool GHOST = false, class LT = ScatterLDS<D, SPL, NV, TF>>
+__device__ __forceinline__ uint32_t order_block(LT& L, const Params& P, const StepState* S, const PartStateT<TF>& src,
                                 const PartStateT<TF>& dst, const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b, uint32_t start,
                                 uint32_t cnt, const int* bc) {
     using G = BG<D, SPL>;
@@ -824,13 +824,13 @@ __device__ __forceinline__ uint32_t order_block(ScatterLDS<D, SPL, NV, TF>& L, c
     return err;
 }
 
-template <int D, int SPL, int NV, class TF>
-__device__ __forceinline__ void zero_tile(ScatterLDS<D, SPL, NV, TF>& L) {
+template <int D, int SPL, int NV, class TF, class LT = ScatterLDS<D, SPL, NV, TF>>
+__device__ __forceinline__ void zero_tile(LT& L) {
     for (int q = threadIdx.x; q < BG<D, SPL>::NT * NV; q += blockDim.x) L.u.tile[q] = 0.0;
 }
 
-template <int D, int SPL, int NV, class TF>
-__device__ __forceinline__ void write_tile(const ScatterLDS<D, SPL, NV, TF>& L, const StagingT<TF>& stg, uint32_t b) {
+template <int D, int SPL, int NV, class TF, class LT = ScatterLDS<D, SPL, NV, TF>>
+__device__ __forceinline__ void write_tile(const LT& L, const StagingT<TF>& stg, uint32_t b) {
     using G = BG<D, SPL>;
     for (int q = threadIdx.x; q < G::NT; q += blockDim.x)
         for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = (TF)L.u.tile[v * G::NT + q];
@@ -844,11 +844,9 @@ __device__ __forceinline__ void write_tile(const ScatterLDS<D, SPL, NV, TF>& L, 
 // in a fixed order. The tile is binary64 in the split form too: LDS binary32 atomic adds issue about ten
 // times slower than binary64 ones on gfx950 (measured: ~80 ns against ~7 ns per wavefront instruction
 // per CU for this access pattern).
-template <int D, int SPL, int NV, class TF>
-__device__ __forceinline__ void add_partials(ScatterLDS<D, SPL, NV, TF>& L, int cell, int a0, int half,
-                                             const TF (&acc)[ScatterShape<D, SPL, NV, TF>::SPP][ScatterShape<D, SPL, NV, TF>::NSH][NV]) {
+template <int D, int SPL, int NV, class TF, class SS = ScatterShape<D, SPL, NV, TF>, class LT = ScatterLDS<D, SPL, NV, TF>>
+__device__ __forceinline__ void add_partials(LT& L, int cell, int a0, int half, const TF (&acc)[SS::SPP][SS::NSH][NV]) {
     using G = BG<D, SPL>;
-    using SS = ScatterShape<D, SPL, NV, TF>;
     int c[3] = {0, 0, 0};
     if (D == 3) { c[2] = cell % G::E; c[1] = (cell / G::E) % G::E; c[0] = cell / (G::E * G::E); }
     else { c[1] = cell % G::E; c[0] = cell / G::E; }
@@ -905,18 +903,23 @@ __device__ __forceinline__ void accumulate_cells(const ScatterLDS<D, SPL, NV, TF
 }
 
 // The wavefronts' partial sums into the zeroed tile, one wavefront after another, and the tile to staging.
-template <int D, int SPL, int NV, class TF>
-__device__ __forceinline__ void finish_tile(ScatterLDS<D, SPL, NV, TF>& L, const StagingT<TF>& stg, uint32_t b,
-                                            const TF (&acc)[ScatterShape<D, SPL, NV, TF>::SPP][ScatterShape<D, SPL, NV, TF>::NSH][NV]) {
-    using SS = ScatterShape<D, SPL, NV, TF>;
+// The wavefronts' partial sums added into the zeroed tile one wavefront after another (no write to staging).
+template <int D, int SPL, int NV, class TF, class SS = ScatterShape<D, SPL, NV, TF>, class LT = ScatterLDS<D, SPL, NV, TF>>
+__device__ __forceinline__ void ordered_tile(LT& L, const TF (&acc)[SS::SPP][SS::NSH][NV]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int grp = wave % SS::NG, half = wave / SS::NG, a0 = grp * SS::SPP;
     zero_tile<D, SPL, NV, TF>(L);
     for (int w = 0; w < SS::WAVES; ++w) {
         __syncthreads();
-        if (wave == w) add_partials<D, SPL, NV, TF>(L, lane, a0, half, acc);
+        if (wave == w) add_partials<D, SPL, NV, TF, SS>(L, lane, a0, half, acc);
     }
     __syncthreads();
+}
+
+template <int D, int SPL, int NV, class TF>
+__device__ __forceinline__ void finish_tile(ScatterLDS<D, SPL, NV, TF>& L, const StagingT<TF>& stg, uint32_t b,
+                                            const TF (&acc)[ScatterShape<D, SPL, NV, TF>::SPP][ScatterShape<D, SPL, NV, TF>::NSH][NV]) {
+    ordered_tile<D, SPL, NV, TF>(L, acc);
     write_tile<D, SPL, NV, TF>(L, stg, b);
 }
 
@@ -943,11 +946,10 @@ __device__ __forceinline__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, con
 
 // Tile of a block with more than kMaxBP particles: every particle adds its window with LDS atomics.
 // Sorted slots start .. start+cnt-1, then gstart .. gstart+gcnt-1 (the block's periodic copies, if any).
-template <int D, int SPL, int NV, class TF, class Contrib>
-__device__ __forceinline__ uint32_t scatter_block_atomic(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, const SortBufs& sb,
+template <int D, int SPL, int NV, class TF, class Contrib, class SS = ScatterShape<D, SPL, NV, TF>, class LT = ScatterLDS<D, SPL, NV, TF>>
+__device__ __forceinline__ uint32_t scatter_block_atomic(LT& L, const Contrib& contrib, const StagingT<TF>& stg, const SortBufs& sb,
                                          uint32_t b, uint32_t start, uint32_t cnt, const int* bc, uint32_t gstart = 0, uint32_t gcnt = 0) {
     using G = BG<D, SPL>;
-    using SS = ScatterShape<D, SPL, NV, TF>;
     const int tid = threadIdx.x;
     __syncthreads();
     zero_tile<D, SPL, NV, TF>(L);
@@ -1060,6 +1062,67 @@ __device__ __forceinline__ bool slice_stencil(SliceStencil<D, SPL, SPP, OBN, TF>
         for (int o = 0; o < W; ++o) node_vals<SPL>(cs, 2, i0[2], o, xp[2], h1, ss.w2[o], ss.dw2[o], ss.pd2[o]);
     }
     return ok;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Staged scatter records (3D split form, cubic spline). A block's particles are taken in chunks of kStageCH
+// slots; for each (particle, axis) pair one thread loads what that axis needs and writes the particle's
+// per-axis factors to LDS, and the cell-owning wavefronts accumulate from LDS instead of each loading the
+// particle and evaluating its stencil again. The value groups factor by window row (a, b) and last-axis node c:
+//   mass      w0 w1 w2
+//   momentum  w0 w1 w2 (v + B~ (x_i - x_p)) = w2 X(a,b) + w0 w1 Z(c),  X = w0 w1 (U0(a) + P1(b)),
+//             U0(a) = v + B~[:,0] (x_0a - x_p0), P1(b) = B~[:,1] (x_1b - x_p1), Z(c) = w2 B~[:,2] (x_2c - x_p2)
+//   force     T grad w / h1 = w2 F(a,b) + dw2 H(a,b),  F = T[:,0] dw0 w1 + T[:,1] w0 dw1,  H = T[:,2] w0 w1
+// (B~ = D^-1 B, T the stress scaled as the force takes it). Record of chunk slot s, structure of arrays (value
+// v at rec[v * kStageCH + s]):
+//   per first-axis node a:  w0, dw0, U0[3]      per second-axis node b: w1, dw1, P1[3]
+//   per last-axis node c:   w2, dw2, Z[3]       T[r][d] (9), MINW[d] (3), muI (MIBF)
+// MINW: the smallest window weight per axis; a product of the three below 1e-24 (a node near the reference's
+// 1e-25 weight cut) sends the particle to the direct per-node form instead.
+// ----------------------------------------------------------------------------------------------
+// Slots per chunk: measured on the 7.95 M-particle inclined flow, the cubic MIBF P2G ran 2705, 2519, 2152, 1951,
+// 1976, 2088, 2160 and 2258 us at 64, 96, 128, 192, 224, 256, 320 and 384 (the workgroup runs alone on its CU
+// either way, so larger chunks cost no occupancy; fewer, longer chunks trade barriers against longer serial
+// staging).
+constexpr int kStageCH = 192;
+
+template <int D, int SPL, bool MU> struct StagedRec {
+    static constexpr int W = BG<D, SPL>::W;
+    static constexpr int OT = D * W * 5, OMIN = OT + D * D, OMU = OMIN + D, N = OMU + (MU ? 1 : 0);
+    // axis d, node k: weight, derivative, then the three components of U0 / P1 / Z
+    __device__ __forceinline__ static constexpr int w(int d, int k) { return (d * W + k) * 5; }
+    __device__ __forceinline__ static constexpr int dw(int d, int k) { return (d * W + k) * 5 + 1; }
+    __device__ __forceinline__ static constexpr int u(int d, int k, int r) { return (d * W + k) * 5 + 2 + r; }
+    __device__ __forceinline__ static constexpr int t(int r, int d) { return OT + r * D + d; }
+};
+
+// Values a thread loads for one (particle, axis) pair of a staged chunk (issued one chunk ahead).
+template <class TF> struct StagePre { double x, mu; TF b[3], t[3], v[3]; };
+
+// Wavefronts of a staged scatter: one first-axis slice each, and the second axis split in halves when a
+// slice's window nodes would need more than 64 partial sums per lane.
+template <int D, int SPL, int NV> struct StagedShape {
+    using G = BG<D, SPL>;
+    static constexpr int SPP = 1, NG = G::W;
+    static constexpr int SPLIT = (G::W % 2 == 0 && G::W * G::W * NV > 64) ? 2 : 1;
+    static constexpr int OBN = G::W / SPLIT, NSH = OBN * G::W, WAVES = NG * SPLIT, TB = 64 * WAVES;
+};
+
+// One axis of a particle's window as the clamped P2G / MUSL stencil takes it (stencil_base + the shift of
+// make_stencil): weights, derivatives and node-minus-particle offsets of the non-zero nodes; returns the first
+// node of the non-zero window.
+template <int SPL, class CS, class TF>
+__device__ __forceinline__ int axis_window(const CS& cs, int d, double x, double org, double h1, TF* w, TF* dw, TF* pd) {
+    constexpr int W = SPL == 2 ? 3 : 4;
+    const double f = floor((x - org) * h1);
+    int v = f < 2.0e9 ? (f > -2.0e9 ? (int)f - 1 : -0x3FFFFFFF) : 0x3FFFFFFF;
+    if (v < 0) v = 0;
+    const int bb = cs.clamp4(d, v);
+    int sh = 0;
+    if (SPL == 2) sh = MPM_SEL(fabs((x - cs(d, bb)) * h1) < 1.5) ? 0 : 1;
+#pragma unroll
+    for (int k = 0; k < W; ++k) node_vals<SPL>(cs, d, bb + sh, k, x, h1, w[k], dw[k], pd[k]);
+    return v + sh;
 }
 
 // P2G contributions (p2g.cpp:12-66 and the force of explicit_euler_update.cpp:15-168): value 0 sum of
@@ -1234,6 +1297,136 @@ struct P2GContrib {
             }
         }
     }
+
+    // ---- staged scatter (3D split form, cubic spline) ----
+    using SRec = StagedRec<D, SPL, MIBF>;
+    using Pre = StagePre<TF>;
+    // Loads of axis d of source particle p (position, column d of B and of tau, velocity and muI on axis 0).
+    template <int d>
+    __device__ __forceinline__ void load_axis(uint32_t p, uint32_t, Pre& q) const {
+        q.x = src.x[d][p];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            q.b[r] = apic ? src.B[r + d * D][p] : TF(0);
+            q.t[r] = src.tau[r == d ? r : (r + d == 1 ? 3 : (r + d == 2 ? 4 : 5))][p];  // tau[r][d], (xx, yy, zz, xy, xz, yz)
+            q.v[r] = d == 0 ? src.v[r][p] : TF(0);
+        }
+        q.mu = (MIBF && d == 0) ? src.muI[p] : 0.0;
+    }
+    // Phase 1: the axis-d part of the record of chunk slot s.
+    template <int d>
+    __device__ __forceinline__ void stage_axis(const Pre& q, TF* rec, int s) const {
+        constexpr int W = BG<D, SPL>::W;
+        TF w[W], dw[W], pd[W];
+        (void)axis_window<SPL>(cs, d, q.x, org[d], h1, w, dw, pd);
+        const TF aDt = (TF)aD;
+        const TF fs = SEP ? (TF)h1 : (TF)(-kf * h1);
+        TF mn = w[0];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            rec[SRec::w(d, k) * kStageCH + s] = w[k];
+            rec[SRec::dw(d, k) * kStageCH + s] = dw[k];
+            mn = fmin(mn, w[k]);
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                const TF bp = (q.b[r] * aDt) * pd[k];
+                rec[SRec::u(d, k, r) * kStageCH + s] = d == 0 ? q.v[r] + bp : (d == 1 ? bp : w[k] * bp);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < D; ++r) rec[SRec::t(r, d) * kStageCH + s] = q.t[r] * fs;
+        rec[(SRec::OMIN + d) * kStageCH + s] = mn;
+        if (MIBF && d == 0) rec[SRec::OMU * kStageCH + s] = (TF)q.mu;
+    }
+    // Phase 2: the particle of chunk slot s into the partial sums of first-axis node a and second-axis half
+    // `half`; false (nothing added) when a window weight is near the 1e-25 cut.
+    template <class SSt>
+    __device__ __forceinline__ bool accumulate_rec(const TF* rec, int s, int a, int half, TF (&acc)[1][SSt::NSH][NV]) const {
+        constexpr int W = BG<D, SPL>::W;
+        const TF mw = rec[SRec::OMIN * kStageCH + s] * rec[(SRec::OMIN + 1) * kStageCH + s] * rec[(SRec::OMIN + 2) * kStageCH + s];
+        if (!MPM_SEL(mw > TF(1e-24))) return false;
+        const TF w0 = rec[SRec::w(0, a) * kStageCH + s], dw0 = rec[SRec::dw(0, a) * kStageCH + s];
+        const TF muI = MIBF ? rec[SRec::OMU * kStageCH + s] : TF(0);
+        TF U0[3], T[3][3];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            U0[r] = rec[SRec::u(0, a, r) * kStageCH + s];
+#pragma unroll
+            for (int d = 0; d < D; ++d) T[r][d] = rec[SRec::t(r, d) * kStageCH + s];
+        }
+#pragma unroll
+        for (int o1 = 0; o1 < SSt::OBN; ++o1) {
+            // rows one after another: reading every row's record values ahead of the arithmetic would keep them
+            // all live beside the partial sums
+            __builtin_amdgcn_sched_barrier(0);
+            const int b = half * SSt::OBN + o1;
+            const TF w1 = rec[SRec::w(1, b) * kStageCH + s], dw1 = rec[SRec::dw(1, b) * kStageCH + s];
+            const TF w01 = w0 * w1, dw0w1 = dw0 * w1, w0dw1 = w0 * dw1;
+            TF X[3], F[3], H[3];
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                X[r] = w01 * (U0[r] + rec[SRec::u(1, b, r) * kStageCH + s]);
+                F[r] = madd(T[r][1], w0dw1, T[r][0] * dw0w1);
+                H[r] = T[r][2] * w01;
+                if (!SEP) X[r] = X[r] + F[r];  // momentum and force summed: w2 (X + F) + w01 Z + dw2 H
+            }
+            const TF w01mu = w01 * muI;
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const TF w2 = rec[SRec::w(2, c) * kStageCH + s], dw2 = rec[SRec::dw(2, c) * kStageCH + s];
+                TF* A = acc[0][o1 * W + c];
+                A[0] = madd(w01, w2, A[0]);
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    const TF Z = rec[SRec::u(2, c, r) * kStageCH + s];
+                    if (SEP) {
+                        A[1 + r] = madd(w2, X[r], madd(w01, Z, A[1 + r]));
+                        A[1 + D + r] = madd(w2, F[r], madd(dw2, H[r], A[1 + D + r]));
+                    } else {
+                        A[1 + r] = madd(w2, X[r], madd(w01, Z, madd(dw2, H[r], A[1 + r])));
+                    }
+                }
+                if (MIBF) A[NV - 1] = madd(w01mu, w2, A[NV - 1]);
+            }
+        }
+        return true;
+    }
+    // Direct form (add's per-node products and the 1e-25 weight cut) of one particle, for particles near the
+    // cut and blocks too large to order.
+    struct Win { TF w[3][4], dw[3][4], pd[3][4]; int wb[3]; TF v[3], Bc[3][3], T[3][3]; TF muI; };
+    __device__ __forceinline__ void window_of(uint32_t p, uint32_t, Win& z) const {
+#pragma unroll
+        for (int d = 0; d < D; ++d) z.wb[d] = axis_window<SPL>(cs, d, src.x[d][p], org[d], h1, z.w[d], z.dw[d], z.pd[d]);
+        const TF aDt = (TF)aD;
+        const TF fs = SEP ? (TF)h1 : (TF)(-kf * h1);
+        TF t6[6];
+        for (int k = 0; k < SymPack<D>::n; ++k) t6[k] = src.tau[k][p];
+        TF tau[3][3];
+        tau[0][0] = t6[0]; tau[1][1] = t6[1]; tau[2][2] = t6[2]; tau[0][1] = tau[1][0] = t6[3]; tau[0][2] = tau[2][0] = t6[4]; tau[1][2] = tau[2][1] = t6[5];
+        for (int r = 0; r < D; ++r) {
+            z.v[r] = src.v[r][p];
+            for (int k = 0; k < D; ++k) { z.Bc[r][k] = apic ? src.B[r + k * D][p] * aDt : TF(0); z.T[r][k] = tau[r][k] * fs; }
+        }
+        z.muI = MIBF ? (TF)src.muI[p] : TF(0);
+    }
+    __device__ __forceinline__ void node_direct(const Win& z, int a, int b, int c, TF* out) const {
+        TF u01[3];
+        for (int r = 0; r < D; ++r) u01[r] = madd(z.Bc[r][1], z.pd[1][b], madd(z.Bc[r][0], z.pd[0][a], z.v[r]));
+        const TF w01 = z.w[0][a] * z.w[1][b], dw0w1 = z.dw[0][a] * z.w[1][b], w0dw1 = z.w[0][a] * z.dw[1][b];
+        const TF wt = w01 * z.w[2][c];
+        const TF g[3] = {dw0w1 * z.w[2][c], w0dw1 * z.w[2][c], w01 * z.dw[2][c]};
+        const TF wm = wt > TF(1e-25) ? wt : TF(0);
+        out[0] = wm;
+        for (int r = 0; r < D; ++r) {
+            const TF u = madd(z.Bc[r][2], z.pd[2][c], u01[r]);
+            TF f = z.T[r][0] * g[0];
+            f = madd(z.T[r][1], g[1], f);
+            f = madd(z.T[r][2], g[2], f);
+            if (SEP) { out[1 + r] = wm * u; out[1 + D + r] = f; }
+            else out[1 + r] = madd(wm, u, f);
+        }
+        if (MIBF) out[NV - 1] = wm * z.muI;
+    }
 };
 
 // MUSL re-scatter contributions (musl.cpp:28-85): sum w (v_new + B_new (x_i - x_p) D^-1) over
@@ -1289,7 +1482,273 @@ struct MuslContrib {
         }
         return ok;
     }
+
+    // ---- staged scatter (3D split form, cubic spline): the momentum sum w2 X(a,b) + w0 w1 Z(c) ----
+    using SRec = StagedRec<D, SPL, false>;
+    using Pre = StagePre<TF>;
+    template <int d>
+    __device__ __forceinline__ void load_axis(uint32_t p, uint32_t i, Pre& q) const {
+        q.x = src.x[d][p];
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            q.b[r] = apic ? dst.B[r + d * D][i] : TF(0);
+            q.v[r] = d == 0 ? dst.v[r][i] : TF(0);
+            q.t[r] = TF(0);
+        }
+        q.mu = 0.0;
+    }
+    template <int d>
+    __device__ __forceinline__ void stage_axis(const Pre& q, TF* rec, int s) const {
+        constexpr int W = BG<D, SPL>::W;
+        TF w[W], dw[W], pd[W];
+        (void)axis_window<SPL>(cs, d, q.x, org[d], h1, w, dw, pd);
+        const TF aDt = (TF)aD;
+        TF mn = w[0];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            rec[SRec::w(d, k) * kStageCH + s] = w[k];
+            mn = fmin(mn, w[k]);
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                const TF bp = (q.b[r] * aDt) * pd[k];
+                rec[SRec::u(d, k, r) * kStageCH + s] = d == 0 ? q.v[r] + bp : (d == 1 ? bp : w[k] * bp);
+            }
+        }
+        rec[(SRec::OMIN + d) * kStageCH + s] = mn;
+    }
+    template <class SSt>
+    __device__ __forceinline__ bool accumulate_rec(const TF* rec, int s, int a, int half, TF (&acc)[1][SSt::NSH][NV]) const {
+        constexpr int W = BG<D, SPL>::W;
+        const TF mw = rec[SRec::OMIN * kStageCH + s] * rec[(SRec::OMIN + 1) * kStageCH + s] * rec[(SRec::OMIN + 2) * kStageCH + s];
+        if (!MPM_SEL(mw > TF(1e-24))) return false;
+        const TF w0 = rec[SRec::w(0, a) * kStageCH + s];
+        TF U0[3];
+#pragma unroll
+        for (int r = 0; r < D; ++r) U0[r] = rec[SRec::u(0, a, r) * kStageCH + s];
+#pragma unroll
+        for (int o1 = 0; o1 < SSt::OBN; ++o1) {
+            const int b = half * SSt::OBN + o1;
+            const TF w01 = w0 * rec[SRec::w(1, b) * kStageCH + s];
+            TF X[3];
+#pragma unroll
+            for (int r = 0; r < D; ++r) X[r] = w01 * (U0[r] + rec[SRec::u(1, b, r) * kStageCH + s]);
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const TF w2 = rec[SRec::w(2, c) * kStageCH + s];
+                TF* A = acc[0][o1 * W + c];
+#pragma unroll
+                for (int r = 0; r < D; ++r) A[r] = madd(w2, X[r], madd(w01, rec[SRec::u(2, c, r) * kStageCH + s], A[r]));
+            }
+        }
+        return true;
+    }
+    struct Win { TF w[3][4], dw[3][4], pd[3][4]; int wb[3]; TF v[3], Bc[3][3]; };
+    __device__ __forceinline__ void window_of(uint32_t p, uint32_t i, Win& z) const {
+#pragma unroll
+        for (int d = 0; d < D; ++d) z.wb[d] = axis_window<SPL>(cs, d, src.x[d][p], org[d], h1, z.w[d], z.dw[d], z.pd[d]);
+        const TF aDt = (TF)aD;
+        for (int r = 0; r < D; ++r) {
+            z.v[r] = dst.v[r][i];
+            for (int k = 0; k < D; ++k) z.Bc[r][k] = apic ? dst.B[r + k * D][i] * aDt : TF(0);
+        }
+    }
+    __device__ __forceinline__ void node_direct(const Win& z, int a, int b, int c, TF* out) const {
+        const TF w01 = z.w[0][a] * z.w[1][b];
+        const TF wt = w01 * z.w[2][c];
+        const TF wm = wt > TF(1e-25) ? wt : TF(0);
+        for (int r = 0; r < D; ++r) {
+            const TF u01 = madd(z.Bc[r][1], z.pd[1][b], madd(z.Bc[r][0], z.pd[0][a], z.v[r]));
+            out[r] = wm * madd(z.Bc[r][2], z.pd[2][c], u01);
+        }
+    }
 };
+
+// ----------------------------------------------------------------------------------------------
+// Staged scatter driver (see the record layout above). The 3D split form's cubic P2G and MUSL re-scatter use
+// it; other forms keep scatter_block. The cubic scatter's wavefronts each cover 8 of a particle's 64 window
+// nodes, so in scatter_block all 8 loaded every particle and evaluated its stencil again (C3: 2.48 -> 1.95 ms
+// staged). The quadratic scatter covers a particle's 27 nodes in one wavefront and has nothing to share: staged,
+// it ran 340 instead of 233 us on the 4,096,000-particle collapse.
+// ----------------------------------------------------------------------------------------------
+template <int D, int SPL, class TF, bool PBC> constexpr bool kStagedScatter = D == 3 && SPL == 3 && sizeof(TF) == 4 && !PBC;
+
+// Workgroup size of a scatter kernel: the staged shape or the register shape of scatter_block.
+template <int D, int SPL, int NV, class TF, bool PBC> struct ScatterLaunch {
+    static constexpr int TB = kStagedScatter<D, SPL, TF, PBC> ? StagedShape<D, SPL, NV>::TB : ScatterShape<D, SPL, NV, TF>::TB;
+};
+
+template <int D, int SPL, int NV, class TF, int NR> struct StagedLDS {
+    using G = BG<D, SPL>;
+    BlockCoordLDS<D, SPL> coords;
+    uint32_t p[kMaxBP];              // source slots, ordered as order_block orders them
+    uint32_t cstart[G::NC + 2];
+    uint32_t flag[kMaxBP / 32];      // slots whose window weights are near the 1e-25 cut (direct form)
+    union {
+        struct { uint32_t tmp[kMaxBP]; uint32_t ccount[G::NC + 1]; uint32_t ccur[G::NC + 1]; uint8_t cell[kMaxBP]; } sort;
+        double tile[G::NT * NV];
+        TF rec[NR * kStageCH];       // the records of one chunk
+    } u;
+};
+
+// Phases 1 and 2 over the block's ordered particles, chunk by chunk. A chunk is a run of whole ranks (the
+// rank-major layout of order_block: rank j holds the j-th particle of every cell with more than j) of at most
+// kStageCH slots; a rank has at most 64 slots, so every chunk holds at least one. All wavefronts see the same
+// 64 cells, hence the same chunks and barriers.
+// Chunk boundaries: the run of whole ranks from rank j (slot kb) holding at most kStageCH slots; a rank has at
+// most 64 slots (the rank-major layout of order_block), so a chunk holds at least one. Returns the end rank.
+__device__ __forceinline__ uint32_t stage_chunk_end(uint32_t j, uint32_t kb, uint32_t n_max, uint32_t n_c, uint32_t& kb1) {
+    uint32_t j1 = j;
+    kb1 = kb;
+    while (j1 < n_max) {
+        const uint32_t c = (uint32_t)__popcll(__ballot(j1 < n_c));
+        if (kb1 + c - kb > (uint32_t)kStageCH) break;
+        kb1 += c;
+        ++j1;
+    }
+    return j1;
+}
+
+// Loads of the (particle, axis) pairs of the chunk of slots kb .. kb + m - 1 that thread tid handles (pairs
+// t = tid, tid + blockDim.x, ... of m * D, axis t / m), at most NP of them.
+template <int D, class Contrib, int NP>
+__device__ __forceinline__ void stage_load(const Contrib& cf, const uint32_t* Lp, uint32_t start, uint32_t kb, uint32_t m,
+                                           typename Contrib::Pre (&pre)[NP]) {
+#pragma unroll
+    for (int it = 0; it < NP; ++it) {
+        const uint32_t t = threadIdx.x + (uint32_t)it * blockDim.x;
+        if (t >= m * D) break;
+        const uint32_t d = t / m, s = t - d * m, p = Lp[kb + s], i = start + kb + s;
+        // the axis is a template argument: indexing the particle arrays by a run-time axis would place the
+        // kernel's arrays of pointers in private memory
+        if (d == 0) cf.template load_axis<0>(p, i, pre[it]);
+        else if (d == 1) cf.template load_axis<1>(p, i, pre[it]);
+        else cf.template load_axis<2>(p, i, pre[it]);
+    }
+}
+template <int D, class Contrib, int NP, class TF>
+__device__ __forceinline__ void stage_write(const Contrib& cf, uint32_t m, const typename Contrib::Pre (&pre)[NP], TF* rec) {
+#pragma unroll
+    for (int it = 0; it < NP; ++it) {
+        const uint32_t t = threadIdx.x + (uint32_t)it * blockDim.x;
+        if (t >= m * D) break;
+        const uint32_t d = t / m, s = t - d * m;
+        if (d == 0) cf.template stage_axis<0>(pre[it], rec, (int)s);
+        else if (d == 1) cf.template stage_axis<1>(pre[it], rec, (int)s);
+        else cf.template stage_axis<2>(pre[it], rec, (int)s);
+    }
+}
+
+// Phases 1 and 2 over the block's ordered particles, chunk by chunk, with the next chunk's loads issued before
+// the current chunk is accumulated. All wavefronts see the same 64 cells, hence the same chunks and barriers.
+template <int D, int SPL, int NV, class TF, class Contrib, class LT>
+__device__ __forceinline__ void staged_accumulate(LT& L, const Contrib& cf, uint32_t start,
+                                                  TF (&acc)[1][StagedShape<D, SPL, NV>::NSH][NV]) {
+    using SS = StagedShape<D, SPL, NV>;
+    constexpr int NP = (kStageCH * D + SS::TB - 1) / SS::TB;  // pairs per thread and chunk
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int a = wave % SS::NG, half = wave / SS::NG;
+    const uint32_t n_c = L.cstart[lane + 1] - L.cstart[lane];
+    const uint32_t n_max = wave_max_u32(n_c);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    typename Contrib::Pre pre[NP];
+    uint32_t j = 0, kb = 0, kb1 = 0;
+    uint32_t j1 = stage_chunk_end(0, 0, n_max, n_c, kb1);
+    stage_load<D>(cf, L.p, start, kb, kb1 - kb, pre);
+    while (j < n_max) {
+        __syncthreads();  // every wavefront is done with the previous chunk's records
+        stage_write<D>(cf, kb1 - kb, pre, L.u.rec);
+        __syncthreads();
+        uint32_t kb2 = kb1;
+        const uint32_t j2 = stage_chunk_end(j1, kb1, n_max, n_c, kb2);
+        if (j1 < n_max) stage_load<D>(cf, L.p, start, kb1, kb2 - kb1, pre);  // next chunk, in flight meanwhile
+        uint32_t k = kb;
+        for (; j < j1; ++j) {
+            const bool has = j < n_c;
+            const unsigned long long mk = __ballot(has);
+            const uint32_t slot = k + (uint32_t)__popcll(mk & below);
+            k += (uint32_t)__popcll(mk);
+            if (has && !cf.template accumulate_rec<SS>(L.u.rec, (int)(slot - kb), a, half, acc))
+                atomicOr(&L.flag[slot >> 5], 1u << (slot & 31));
+        }
+        kb = kb1; kb1 = kb2; j1 = j2;
+    }
+}
+
+// Adds particle `slot`'s direct form to the tile: thread t takes node t of its window.
+template <int D, int SPL, int NV, class TF, class Contrib, class LT>
+__device__ __forceinline__ void staged_direct(LT& L, const Contrib& cf, uint32_t p, uint32_t i, const int* bc, bool atomic) {
+    using G = BG<D, SPL>;
+    typename Contrib::Win z;
+    cf.window_of(p, i, z);
+    const int t = threadIdx.x;
+    if (t >= G::W * G::W * G::W) return;
+    const int a = t / (G::W * G::W), b = (t / G::W) % G::W, c = t % G::W;
+    TF out[NV];
+    cf.node_direct(z, a, b, c, out);
+    const int q = tile_index<D, SPL>(z.wb[0] - G::E * bc[0] + a, z.wb[1] - G::E * bc[1] + b, z.wb[2] - G::E * bc[2] + c);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if (atomic) atomicAdd(&L.u.tile[v * G::NT + q], (double)out[v]);
+        else L.u.tile[v * G::NT + q] += (double)out[v];
+    }
+}
+
+// The block's tile by the staged scatter, to staging: ordered partial sums, then the direct form of the
+// flagged particles one after another in slot order.
+template <int D, int SPL, int NV, class TF, class Contrib, class LT>
+__device__ __forceinline__ void staged_scatter_block(LT& L, const Contrib& cf, const StagingT<TF>& stg, uint32_t b, uint32_t start,
+                                                     uint32_t cnt, const int* bc) {
+    using SS = StagedShape<D, SPL, NV>;
+    for (uint32_t w = threadIdx.x; w < (uint32_t)(kMaxBP / 32); w += blockDim.x) L.flag[w] = 0u;
+    TF acc[1][SS::NSH][NV];
+#pragma unroll
+    for (int o = 0; o < SS::NSH; ++o)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[0][o][v] = TF(0);
+    staged_accumulate<D, SPL, NV, TF>(L, cf, start, acc);
+    __syncthreads();  // the records are read; the tile takes their place
+    ordered_tile<D, SPL, NV, TF, SS>(L, acc);
+    for (uint32_t w = 0; w < (cnt + 31) / 32; ++w) {
+        uint32_t bits = L.flag[w];
+        while (bits) {
+            const uint32_t slot = w * 32 + (uint32_t)(__ffs(bits) - 1);
+            bits &= bits - 1u;
+            staged_direct<D, SPL, NV, TF>(L, cf, L.p[slot], start + slot, bc, false);
+            __syncthreads();
+        }
+    }
+    write_tile<D, SPL, NV, TF>(L, stg, b);
+}
+
+// A block with more than kMaxBP particles: every particle adds its direct form with LDS atomics (unordered).
+template <int D, int SPL, int NV, class TF, class Contrib, class LT>
+__device__ __forceinline__ uint32_t staged_block_atomic(LT& L, const Contrib& cf, const StagingT<TF>& stg, const SortBufs& sb, uint32_t b,
+                                                        uint32_t start, uint32_t cnt, const int* bc) {
+    using G = BG<D, SPL>;
+    constexpr int NN = G::W * G::W * G::W;
+    __syncthreads();
+    zero_tile<D, SPL, NV, TF>(L);
+    __syncthreads();
+    uint32_t err = 0;
+    for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+        typename Contrib::Win z;
+        cf.window_of(sb.perm[start + k], start + k, z);
+        int l[3];
+        bool in = true;
+        for (int d = 0; d < D; ++d) { l[d] = z.wb[d] - G::E * bc[d]; in = in && l[d] >= 0 && l[d] < G::E; }
+        if (!in) { err |= MPM_ERR_GRID_BOUNDS; continue; }
+        for (int t = 0; t < NN; ++t) {
+            const int a = t / (G::W * G::W), bb = (t / G::W) % G::W, c = t % G::W;
+            TF out[NV];
+            cf.node_direct(z, a, bb, c, out);
+            const int q = tile_index<D, SPL>(l[0] + a, l[1] + bb, l[2] + c);
+            for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], (double)out[v]);
+        }
+    }
+    __syncthreads();
+    write_tile<D, SPL, NV, TF>(L, stg, b);
+    return err;
+}
 
 // ----------------------------------------------------------------------------------------------
 // k_p2g: P2G (p2g.cpp:12-66) fused with the stress force of explicitEulerUpdate
@@ -1297,9 +1756,64 @@ struct MuslContrib {
 // m_i = 0 are never updated or read, so the force is scattered unconditionally.
 // ----------------------------------------------------------------------------------------------
 // PBC: the variant for periodic contexts, which also scatters the block's periodic copies.
+// The staged form of the block's P2G (3D split form, no periodic copies).
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+__device__ __forceinline__ void p2g_staged_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                                const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
+    using Cf = P2GContrib<D, SPL, SEP, MIBF, TF>;
+    constexpr int NV = Cf::NV;
+    __shared__ StagedLDS<D, SPL, NV, TF, Cf::SRec::N> L;
+    if (S->stop) return;
+    if (b >= S->nblocks) return;
+    const uint32_t cnt = sb.count[b];
+    if (cnt == 0) return;
+    const uint32_t start = sb.offset[b];
+    int bc[3];
+    block_coords<D>(b, S, bc);
+    Cf cf;
+    cf.src = src;
+    for (int d = 0; d < 3; ++d) { cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
+    L.coords.load(g.coord, bc, cf.nn);
+    cf.cs = L.coords.view(bc);
+    __syncthreads();
+    cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
+    cf.kf = S->dt * P.particle_volume / P.particle_mass;
+    cf.Lx = P.L[0]; cf.band = 4.0 * P.dx;
+    uint32_t err = 0;
+    if (cnt > kMaxBP) {
+        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) dst.id[start + k] = src.id[sb.perm[start + k]];
+        err |= staged_block_atomic<D, SPL, NV, TF>(L, cf, stg, sb, b, start, cnt, bc);
+    } else {
+        err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
+        staged_scatter_block<D, SPL, NV, TF>(L, cf, stg, b, start, cnt, bc);
+    }
+    if (S->check_conservation) {  // sum of m_p v_p (simulation.cpp:292-317)
+        double pm[3] = {0, 0, 0};
+        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+            const uint32_t p = sb.perm[start + k];
+            for (int d = 0; d < D; ++d) pm[d] += P.particle_mass * src.v[d][p];
+        }
+        for (int d = 0; d < D; ++d) pm[d] = wave_sum_f64(pm[d]);
+        if ((threadIdx.x & 63) == 0) for (int d = 0; d < D; ++d) atomic_add_f64(&S->cons_part_mom[d], pm[d]);
+    }
+    err = wave_or_u32(err);
+    if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
+}
+
+template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC = false>
+__device__ __forceinline__ void p2g_plain_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                               const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b);
+
 template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC = false>
 __device__ __forceinline__ void p2g_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                          const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
+    if constexpr (kStagedScatter<D, SPL, TF, PBC>) p2g_staged_body<D, SPL, SEP, MIBF, TF>(P, S, src, dst, sb, g, stg, b);
+    else p2g_plain_body<D, SPL, SEP, MIBF, TF, PBC>(P, S, src, dst, sb, g, stg, b);
+}
+
+template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
+__device__ __forceinline__ void p2g_plain_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                               const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
     constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
     using SS = ScatterShape<D, SPL, NV, TF>;
     __shared__ ScatterLDS<D, SPL, NV, TF> L;
@@ -1360,15 +1874,59 @@ __device__ __forceinline__ void p2g_body(const Params& P, StepState* S, const Pa
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
 template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
-__global__ __launch_bounds__((ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB))
+__global__ __launch_bounds__((ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::TB))
 void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
     p2g_body<D, SPL, SEP, MIBF, TF, PBC>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }
 
 // k_musl_scatter: the MUSL re-scatter, in the particle order that k_p2g fixed for the step.
 template <int D, int SPL, class TF>
+__device__ __forceinline__ void musl_staged_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                                 const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
+    using G = BG<D, SPL>;
+    using Cf = MuslContrib<D, SPL, TF>;
+    __shared__ StagedLDS<D, SPL, D, TF, Cf::SRec::N> L;
+    if (S->stop) return;
+    if (b >= S->nblocks) return;
+    const uint32_t cnt = sb.count[b];
+    if (cnt == 0) return;
+    const uint32_t start = sb.offset[b];
+    int bc[3];
+    block_coords<D>(b, S, bc);
+    Cf cf;
+    cf.src = src; cf.dst = dst;
+    for (int d = 0; d < 3; ++d) { cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
+    L.coords.load(g.coord, bc, cf.nn);
+    cf.cs = L.coords.view(bc);
+    cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
+    uint32_t err = 0;
+    if (cnt > kMaxBP) {
+        err |= staged_block_atomic<D, SPL, D, TF>(L, cf, stg, sb, b, start, cnt, bc);
+    } else {
+        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) L.p[k] = sb.perm[start + k];
+        for (int c = threadIdx.x; c <= G::NC; c += blockDim.x) L.cstart[c] = sb.cell_off[(size_t)b * (G::NC + 1) + c];
+        if (threadIdx.x == 0) L.cstart[G::NC + 1] = cnt;
+        __syncthreads();
+        staged_scatter_block<D, SPL, D, TF>(L, cf, stg, b, start, cnt, bc);
+    }
+    err = wave_or_u32(err);
+    if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
+}
+
+template <int D, int SPL, class TF>
+__device__ __forceinline__ void musl_plain_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                                const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b);
+
+template <int D, int SPL, class TF>
 __device__ __forceinline__ void musl_scatter_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                                   const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
+    if constexpr (kStagedScatter<D, SPL, TF, false>) musl_staged_body<D, SPL, TF>(P, S, src, dst, sb, g, stg, b);
+    else musl_plain_body<D, SPL, TF>(P, S, src, dst, sb, g, stg, b);
+}
+
+template <int D, int SPL, class TF>
+__device__ __forceinline__ void musl_plain_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                                const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
     using G = BG<D, SPL>;
     __shared__ ScatterLDS<D, SPL, D, TF> L;
     if (S->stop) return;
@@ -1398,7 +1956,7 @@ __device__ __forceinline__ void musl_scatter_body(const Params& P, StepState* S,
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
 template <int D, int SPL, class TF>
-__global__ __launch_bounds__((ScatterShape<D, SPL, D, TF>::TB))
+__global__ __launch_bounds__((ScatterLaunch<D, SPL, D, TF, false>::TB))
 void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
     musl_scatter_body<D, SPL, TF>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }

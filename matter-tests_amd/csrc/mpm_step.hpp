@@ -9,13 +9,14 @@ namespace mpm {
 // P2G and the grid update for one choice of separate force (flip needed or conservation checked) and MIBF.
 template <int D, int SPL, bool SEP, bool MIBF, class TF>
 void enqueue_p2g(mpm_context* c, const PartStateT<TF>& src, const PartStateT<TF>& dst, uint32_t gblocks) {
-    constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB;
+    constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
+    constexpr int TBP = ScatterLaunch<D, SPL, NV, TF, true>::TB, TB = ScatterLaunch<D, SPL, NV, TF, false>::TB;
     hipStream_t s = c->stream;
     const GridPtrsT<TF>& g = grid_of<TF>(c);
     const StagingT<TF>& stg = staging_of<TF>(c);
     {
         LaunchTimer t(c, KS_P2G);
-        if (c->sb.ghosts) k_p2g<D, SPL, SEP, MIBF, TF, true><<<c->act_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg);
+        if (c->sb.ghosts) k_p2g<D, SPL, SEP, MIBF, TF, true><<<c->act_blocks, TBP, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg);
         else k_p2g<D, SPL, SEP, MIBF, TF, false><<<c->act_blocks, TB, 0, s>>>(c->P, c->S, src, dst, c->sb, g, stg);
     }
     { LaunchTimer t(c, KS_GRID); k_grid_update<D, SPL, SEP, MIBF, TF><<<gblocks, kBlock, 0, s>>>(c->P, c->S, c->sb, g, stg); }
@@ -74,7 +75,7 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
         else k_g2p<D, SPL, false, TF><<<c->act_blocks, kG2PThreads, 0, s>>>(P, c->S, src, dst, c->sb, g);
     }
     if (P.use_musl) {
-        constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
+        constexpr int TBM = ScatterLaunch<D, SPL, D, TF, false>::TB;
         { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->act_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
         { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL, TF><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, g, stg); }
     }
@@ -87,8 +88,9 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
 
 template <int D, int SPL, bool SEP, bool MIBF, class TF>
 void enqueue_batch_p2g(const BatchClass& bc, hipStream_t s) {
-    constexpr int TB = ScatterShape<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF>::TB;
-    if (bc.key.pbc) kb_p2g<D, SPL, SEP, MIBF, TF, true><<<bc.wg[BR_BLOCK], TB, 0, s>>>(bc.T);
+    constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
+    constexpr int TBP = ScatterLaunch<D, SPL, NV, TF, true>::TB, TB = ScatterLaunch<D, SPL, NV, TF, false>::TB;
+    if (bc.key.pbc) kb_p2g<D, SPL, SEP, MIBF, TF, true><<<bc.wg[BR_BLOCK], TBP, 0, s>>>(bc.T);
     else kb_p2g<D, SPL, SEP, MIBF, TF, false><<<bc.wg[BR_BLOCK], TB, 0, s>>>(bc.T);
     kb_grid_update<D, SPL, SEP, MIBF, TF><<<bc.wg[BR_GRID], kBlock, 0, s>>>(bc.T);
 }
@@ -109,7 +111,7 @@ void enqueue_batch_step(const BatchClass& bc, hipStream_t s) {
     if (k.flip) kb_g2p<D, SPL, true, TF><<<bc.wg[BR_BLOCK], kG2PThreads, 0, s>>>(T);
     else kb_g2p<D, SPL, false, TF><<<bc.wg[BR_BLOCK], kG2PThreads, 0, s>>>(T);
     if (k.musl) {
-        constexpr int TBM = ScatterShape<D, SPL, D, TF>::TB;
+        constexpr int TBM = ScatterLaunch<D, SPL, D, TF, false>::TB;
         kb_musl_scatter<D, SPL, TF><<<bc.wg[BR_BLOCK], TBM, 0, s>>>(T);
         kb_musl_update<D, SPL, TF><<<bc.wg[BR_GRID], kBlock, 0, s>>>(T);
     }
