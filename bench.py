@@ -13,7 +13,9 @@ region, the split form's position error against the plain form, the parity of th
 against the CPU oracle at a small size, the oracle timed on the host cores on a bounded sample of the same
 workload (a reported baseline, not a target), and the extra workloads: the 2D column collapse run whole on the
 GPU and on the oracle (`c1`), the inclined flow (`c3`), 128 batched scenes (`batched`) and a served Poisson request
-stream (`served`, with p50/p99 step and request latency).
+stream (`served`, with p50/p99 step and request latency). Each of c3, batched and served carries the error of
+its own configuration against the FP64 oracle after 100 steps at a few thousand particles per scene
+(`error_vs_fp64_oracle`).
 
     python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--no-cpu-baseline] [--extra c1,c3,batched,served]
 """
@@ -112,6 +114,99 @@ def parity_check(prec, n_axis=20):
         mom_g, mom_o = gg["mass"][act, None] * gg["v"][act], og["mass"][act, None] * og["v"][act]
         out["grid_momentum"] = float(np.max(np.abs(mom_g - mom_o)) / max(np.max(np.abs(mom_o)), 1e-300))
     out.update(particles=int(x.shape[0]), steps=steps, reference="CPU oracle, 1 thread")
+    return out
+
+
+def field_errors(got, ref):
+    """Norm-wise relative errors of position, velocity and F, and the largest position difference in m."""
+    out = {f"{k}_norm_rel": float(np.linalg.norm(got[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300))
+           for k in ("x", "v", "F")}
+    out["x_max_abs_m"] = float(np.max(np.abs(got["x"] - ref["x"])))
+    return out
+
+
+def oracle_run(cfg, plates, objs, x, steps):
+    """The same scene on the FP64 oracle (min(16, host cores) threads) for `steps` steps: its particle state."""
+    ocfg = type(cfg).from_buffer_copy(cfg)
+    ocfg.precision = A.PRECISION["fp64"]
+    ocfg.n_threads = max(1, min(16, os.cpu_count() or 1))
+    oe = A.Engine(A.oracle_library(), ocfg, plates, objs)
+    oe.set_particles(x)
+    rc, info = oe.step(steps)
+    if rc or info.steps_done != steps:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], "oracle steps")
+    return oe.get_particles()
+
+
+def worst_of(per_scene):
+    """Summary over scenes: the largest of each error, and the scenes' own errors."""
+    keys = per_scene[0][1].keys()
+    return {"max_over_scenes": {k: max(e[k] for _, e in per_scene) for k in keys},
+            "scenes": {name: {k: e[k] for k in ("x_norm_rel", "x_max_abs_m")} for name, e in per_scene}}
+
+
+PARITY_STEPS = 100  # the split form's specification: positions within 1e-5 relative after 100 steps
+
+
+def incline_parity():
+    """The inclined-flow configuration (cubic, AFLIP, mu(I)-MCC, MIBF, bump) on a 40 x 6 x 12 lattice
+    (2,880 particles), split form on the GPU against the FP64 oracle after 100 steps."""
+    cfg, plates, objs, x = S.incline_flow(lattice=[40, 6, 12], ppc_axis=0.004 / 0.0015536, precision="split")
+    e = A.Engine(A.product_library(), cfg, plates, objs)
+    e.set_particles(x)
+    rc, info = e.step(PARITY_STEPS)
+    if rc or info.steps_done != PARITY_STEPS:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], "c3 parity steps")
+    out = field_errors(e.get_particles(), oracle_run(cfg, plates, objs, x, PARITY_STEPS))
+    out.update(particles=int(x.shape[0]), steps=PARITY_STEPS, reference="CPU oracle, FP64",
+               scene="same configuration, 40 x 6 x 12 lattice")
+    return out
+
+
+def batched_parity(count=8):
+    """The batched workload's scene generator at 2k-6k particles (8 scenes, both materials and dimensions),
+    one split-form batch on the GPU, each member against its FP64 oracle run after 100 steps."""
+    scs = S.batch_collapses(count=count, n_min=2_000, n_max=6_000)
+    L = A.product_library()
+    b = A.Batch(L, chunk_steps=16, use_graph=True)
+    engs = []
+    for name, cfg, plates, objs, x in scs:
+        e = A.Engine(L, cfg, plates, objs)
+        e.set_particles(x)
+        b.add(e, PARITY_STEPS)
+        engs.append(e)
+    rc, info = b.step(PARITY_STEPS)
+    if rc or info.n_active != 0:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], "batched parity steps")
+    per = [(name, field_errors(e.get_particles(), oracle_run(cfg, plates, objs, x, PARITY_STEPS)))
+           for (name, cfg, plates, objs, x), e in zip(scs, engs)]
+    b.close()
+    out = worst_of(per)
+    out.update(particles=int(sum(s[4].shape[0] for s in scs)), steps=PARITY_STEPS, reference="CPU oracle, FP64",
+               scene=f"batch of {count} scenes of the same generator at 2k-6k particles")
+    return out
+
+
+def served_parity(count=4):
+    """The served workload's request generator at 2k-6k particles and 100 steps (4 requests, collapses and
+    inclined flows), through the server, each result against its FP64 oracle run."""
+    reqs = S.served_requests(count=count, rate=200.0, n_min=2_000, n_max=6_000, steps_min=PARITY_STEPS,
+                             steps_max=PARITY_STEPS)
+    srv = A.Server(A.product_library(), chunk_steps=8, use_graph=True, keep_results=True)
+    ids = [srv.submit(cfg, plates, objs, x, steps, t) for _, cfg, plates, objs, x, steps, t in reqs]
+    rc, st = srv.run(120.0)
+    if rc or st.completed != count:
+        raise A.MPMError(rc or A.ERR["NOT_READY"], "served parity run")
+    per = []
+    for rid, (name, cfg, plates, objs, x, steps, _) in zip(ids, reqs):
+        got, info, _ = srv.result(rid)
+        if info.steps_done != steps:
+            raise A.MPMError(A.ERR["NOT_READY"], f"served parity request {name}")
+        per.append((name, field_errors(got, oracle_run(cfg, plates, objs, x, steps))))
+    srv.close()
+    out = worst_of(per)
+    out.update(particles=int(sum(q[4].shape[0] for q in reqs)), steps=PARITY_STEPS, reference="CPU oracle, FP64",
+               scene=f"{count} requests of the same generator at 2k-6k particles")
     return out
 
 
@@ -276,10 +371,16 @@ def main():
         result["c1"] = column2d_run(args)
     if "c3" in extra:
         result["c3"] = incline_run(args)
+        if not args.no_parity:
+            result["c3"]["error_vs_fp64_oracle"] = incline_parity()
     if "batched" in extra:
         result["batched"] = batched_run(args)
+        if not args.no_parity:
+            result["batched"]["error_vs_fp64_oracle"] = batched_parity()
     if "served" in extra:
         result["served"] = served_run(args)
+        if not args.no_parity:
+            result["served"]["error_vs_fp64_oracle"] = served_parity()
     if not args.no_parity:
         result["error_vs_fp64_oracle"] = {prec: parity_check(prec) for prec in forms}
     if not args.no_cpu_baseline:

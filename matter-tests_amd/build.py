@@ -57,7 +57,20 @@ def build_reference_tests():
     run(["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unused-function", REFTESTS_SRC, "-I", os.path.join(REPO, "include"),
          "-L", HERE, "-lmpm_hip", "-Wl,-rpath,$ORIGIN/../../matter-tests_amd", "-o", tmp])
     os.replace(tmp, REFTESTS_BIN)
+    build_sampler()
     return REFTESTS_BIN
+
+
+SAMPLER_SRC = os.path.join(REPO, "tests", "cpp", "sampler_check.cpp")
+SAMPLER_BIN = os.path.join(REPO, "tests", "cpp", "sampler_check_3d")
+
+
+def build_sampler():
+    """The host Poisson-disk sampler as a program (3D), so that a test can regenerate a recorded sample."""
+    tmp = SAMPLER_BIN + ".tmp"
+    run(["g++", "-O2", "-std=c++17", SAMPLER_SRC, "-o", tmp])
+    os.replace(tmp, SAMPLER_BIN)
+    return SAMPLER_BIN
 
 
 def build_oracle():

@@ -2086,25 +2086,54 @@ __device__ __forceinline__ double blend_velocity(double fr, double v, double fli
 // elasticity, or kMatNeoHookean (no return map).
 constexpr int kMatNeoHookean = 16;
 
+// The state the deformation update reads of source particle p: F and the plastic scalars.
+template <int D> struct DefgradIn { M<D> F; PlasticState ps; };
+template <int D, class TF>
+__device__ __forceinline__ void load_defgrad_in(const PartStateT<TF>& src, uint32_t p, DefgradIn<D>& in) {
+    for (int k = 0; k < D * D; ++k) in.F.m[k] = src.F[k][p];
+    in.ps.eps_dev = src.eps_dev[p]; in.ps.eps_vol = src.eps_vol[p]; in.ps.eps_prad = src.eps_prad[p];
+    in.ps.dgamma = 0.0; in.ps.visc = src.visc[p]; in.ps.muI = src.muI[p];
+}
+
 // F <- F + (dt*grad v) * F, return map, Kirchhoff stress for the next step (deformation_update.cpp:13-46,
-// plasticity.cpp:11-665). Reads particle p of the source layout, writes slot i of the destination layout.
+// plasticity.cpp:11-665), on the state `in` of a source particle; writes slot i of the destination layout.
 template <int D, class TF, int MAT>
-__device__ __forceinline__ void deform_and_project(const Params& P, double dt, const PartStateT<TF>& src, uint32_t p,
+__device__ __forceinline__ void deform_and_project(const Params& P, double dt, const DefgradIn<D>& in,
                                                    const PartStateT<TF>& dst, uint32_t i, const double (&gv)[3][3],
                                                    uint32_t& err, uint32_t& pcount) {
-    M<D> F, A;
-    for (int k = 0; k < D * D; ++k) F.m[k] = src.F[k][p];
+    M<D> F = in.F, A;
     for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) A(r, c) = dt * gv[r][c];
     M<D> AF = mul<D>(A, F);
     for (int k = 0; k < D * D; ++k) F.m[k] = F.m[k] + AF.m[k];
-    PlasticState ps;
-    ps.eps_dev = src.eps_dev[p]; ps.eps_vol = src.eps_vol[p]; ps.eps_prad = src.eps_prad[p];
-    ps.dgamma = 0.0; ps.visc = src.visc[p]; ps.muI = src.muI[p];
+    PlasticState ps = in.ps;
     double tau[6];
+    // The split form decomposes b = F F^T (left_spectral: half the rotations of the two-sided SVD, results
+    // within about 1e-12). The plain form keeps the two-sided SVD, whose rotations round like the reference's
+    // JacobiSVD: its yield decisions must match the reference's one for one, including those of particles
+    // whose strain is rounding noise (a uniform flow's grad v), which sit on a yield surface through the
+    // origin (MCC with beta = 0, cohesionless Drucker-Prager) and fall either side of it with the last bit.
+    constexpr bool kLeftSpectral = sizeof(TF) == 4;
+    M<D> U; V<D> lam;
     if (MAT == kMatNeoHookean) {
         neohookean_tau<D>(F, P.mu, P.lambda, tau);
+    } else if (kLeftSpectral && left_spectral<D>(F, U, lam, P.svd_tol)) {
+        // singular values sqrt(lam), all within a factor 100 of each other and above the 1e-4 clamp
+        V<D> ls;
+        for (int k = 0; k < D; ++k) ls[k] = 0.5 * log_d(lam[k]);
+        if (MAT != MPM_PLASTIC_NONE) {
+            V<D> h = ls;
+            RMResult r = return_map<D, MAT>(P, dt, h, ps);
+            err |= r.err;
+            pcount += r.plastic_incr;
+            if (r.recompose) {
+                V<D> e; for (int k = 0; k < D; ++k) e[k] = exp(h[k]) * rsqrt_d(lam[k]);
+                F = recompose_left<D>(U, e, F);
+                ls = h;
+            }
+        }
+        hencky_tau<D>(U, ls, P.mu, P.lambda, tau);
     } else {
-        M<D> U, W; V<D> s;
+        M<D> W; V<D> s;
         svd<D>(F, U, s, W, P.svd_tol);
         V<D> ls;
         if (MAT != MPM_PLASTIC_NONE) {
@@ -2389,18 +2418,6 @@ template <int D, int SPL, class TF, int SETS> struct SlotTiles {
     static constexpr int bytes = SETS * TileStride<D>::v * BG<D, SPL>::NT * (int)sizeof(TF);
     static constexpr int n = (32768 / bytes) < 1 ? 1 : ((32768 / bytes) > 4 ? 4 : (32768 / bytes));
 };
-// First block of the slots base .. min(n, base + blockDim.x) - 1 and the number of blocks, up to cap, from it.
-__device__ __forceinline__ void slot_block_span(const SortBufs& sb, uint32_t base, uint32_t n, uint32_t cap, uint32_t& blo, uint32_t& nt) {
-    __shared__ uint32_t s_blo, s_nt;
-    if (threadIdx.x == 0) {
-        const uint32_t lo = sb.key[sb.perm[base]], hi = sb.key[sb.perm[min(n, base + blockDim.x) - 1]];
-        s_blo = lo;
-        s_nt = min(cap, hi - lo + 1);
-    }
-    __syncthreads();
-    blo = s_blo; nt = s_nt;
-}
-
 // ----------------------------------------------------------------------------------------------
 // k_g2p: G2P (g2p.cpp:17-77), the particle velocity update (USL: position_update.cpp:14-23; MUSL: the
 // same formula in musl.cpp:13-25) and the position update (position_update.cpp:12). The deformation
@@ -2478,14 +2495,38 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
     constexpr int NTL = SlotTiles<D, SPL, TF, 1>::n;
     __shared__ __attribute__((aligned(16))) TF s_v[NTL][TileStride<D>::v * G::NT];
     __shared__ BlockCoordLDS<D, SPL> s_c[NTL];
+    __shared__ uint32_t s_lo, s_hi;
     if (S->stop) return;
     const uint32_t base = bid * blockDim.x;
     if (base >= n) return;
     const uint32_t i = base + threadIdx.x;
     const bool active = i < n;
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
-    uint32_t blo, nt;
-    slot_block_span(sb, base, n, (uint32_t)NTL, blo, nt);
+    // The blocks of the slots are those of the first and the last slot's particle (slots are in block order).
+    // The split form loads each particle's slot, block, position, F and plastic scalars first, so that these
+    // loads are in flight while the tiles load (C2: 357 -> 345 us). In the plain form that order takes the
+    // kernel from 118 to 131 VGPRs and 337 to 370 us, so there one thread reads the span and every particle
+    // is loaded after the tiles.
+    constexpr bool kEarlyState = sizeof(TF) == 4;
+    uint32_t p = 0, key = 0;
+    double xp[3] = {0, 0, 0};
+    DefgradIn<D> in;
+    const uint32_t last = min(n, base + blockDim.x) - 1;
+    if constexpr (kEarlyState) {
+        if (active) {
+            p = sb.perm[i];
+            key = sb.key[p];
+            for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
+            load_defgrad_in<D, TF>(src, p, in);
+            if (i == base) s_lo = key;
+            if (i == last) s_hi = key;
+        }
+    } else if (threadIdx.x == 0) {
+        s_lo = sb.key[sb.perm[base]];
+        s_hi = sb.key[sb.perm[last]];
+    }
+    __syncthreads();
+    const uint32_t blo = s_lo, nt = min((uint32_t)NTL, s_hi - s_lo + 1);
     for (uint32_t t = 0; t < nt; ++t) {
         if (sb.count[blo + t] == 0) continue;
         int bc[3];
@@ -2496,12 +2537,15 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
     __syncthreads();
     uint32_t err = 0, pcount = 0;
     if (active) {
-        const uint32_t p = sb.perm[i];
+        if constexpr (!kEarlyState) {
+            p = sb.perm[i];
+            for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
+            key = sb.key[p];
+            load_defgrad_in<D, TF>(src, p, in);
+        }
         const double h1 = P.one_over_dx, dt = S->dt;
         const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
-        double xp[3] = {0, 0, 0};
-        for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
-        const uint32_t t = sb.key[p] - blo;
+        const uint32_t t = key - blo;
         Stencil<D, SPL> st;
         TF gvt[3][3] = {{0}}, pic[3], fl[3], Bp[3][3];
         bool ok;
@@ -2519,7 +2563,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
         } else {
             double gv[3][3];
             for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) gv[r][c] = (double)gvt[r][c];
-            deform_and_project<D, TF, MAT>(P, dt, src, p, dst, i, gv, err, pcount);
+            deform_and_project<D, TF, MAT>(P, dt, in, dst, i, gv, err, pcount);
         }
     }
     pcount = wave_sum_u32(pcount);

@@ -62,7 +62,7 @@ MPM_HD double log_d(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ inline double rsqrt_d(double x) { return rsqrt(x); }
 #else
-inline double rsqrt_d(double x) { return 1.0 / sqrt(x); }
+MPM_HD double rsqrt_d(double x) { return 1.0 / sqrt(x); }
 #endif
 // BSpline<SPL>::node<K>(u, w, dw): weight and derivative of node K of a particle's non-zero window
 // (K = 0 .. SPL), u = (x_p - x_i)/dx. Within the window the node fixes the polynomial piece: u lies in
@@ -128,14 +128,14 @@ template <int D> struct V { double v[D]; MPM_HD double& operator[](int i) { retu
 template <int D> MPM_HD void set_identity(M<D>& a) { for (int i = 0; i < D * D; ++i) a.m[i] = 0.0; for (int i = 0; i < D; ++i) a(i, i) = 1.0; }
 // (A*B)(r,c) = sum_k A(r,k) B(k,c), summed in k order.
 template <int D> MPM_HD M<D> mul(const M<D>& A, const M<D>& B) {
-    #pragma clang fp contract(fast)  // products and sums fuse into FMAs: fewer VALU instructions, one rounding each
+    #pragma clang fp contract(on)  // a product and the sum it feeds in one statement fuse into an FMA, decided per statement, so every kernel that inlines this rounds alike
     M<D> R;
     for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { double s = A(r, 0) * B(0, c); for (int k = 1; k < D; ++k) s += A(r, k) * B(k, c); R(r, c) = s; }
     return R;
 }
 // U * diag(d) * V^T evaluated as (U*diag(d)) * V^T.
 template <int D> MPM_HD M<D> udvt(const M<D>& U, const V<D>& d, const M<D>& W) {
-    #pragma clang fp contract(fast)
+    #pragma clang fp contract(on)
     M<D> UD; for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) UD(r, c) = U(r, c) * d[c];
     M<D> R;
     for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) { double s = UD(r, 0) * W(c, 0); for (int k = 1; k < D; ++k) s += UD(r, k) * W(c, k); R(r, c) = s; }
@@ -173,7 +173,7 @@ MPM_HD PlaneRot sym_jacobi(double x, double y, double z) {  // annihilates y in 
 // One two-sided Jacobi step on the pair (p, q); compile-time indices keep X, U, W in registers.
 template <int D, int p, int q>
 MPM_HD void jacobi_pair(M<D>& X, M<D>& U, M<D>& W, double& dmax, bool& rotated, double tol) {
-    #pragma clang fp contract(fast)
+    #pragma clang fp contract(on)
     const double thr = fmax(2.2250738585072014e-308, tol * dmax);
     if (!(fabs(X(p, q)) > thr || fabs(X(q, p)) > thr)) return;
     rotated = true;
@@ -282,6 +282,120 @@ template <int D> MPM_HD void svd(const M<D>& A, M<D>& U, V<D>& s, M<D>& W, doubl
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Left spectral decomposition: b = F F^T = U diag(lam) U^T by cyclic Jacobi rotations of the symmetric b,
+// lam descending. U holds the left singular vectors of F and lam its squared singular values, so
+// F = U diag(sqrt lam) W^T with W = F^T U diag(1/sqrt lam). The Hencky stress and the return map need only U
+// and lam, and the plastic update F' = U diag(exp h) W^T = U diag(exp(h)/sqrt lam) U^T F, so this replaces
+// the two-sided SVD with one rotation per pair instead of two and no W. Squaring F squares its condition
+// number: the smallest lam carries an absolute error of about eps * lam_max, so the decomposition is
+// accepted (returns true) only when lam_min >= 1e-4 lam_max (singular values within a factor 100, their
+// logarithms then within about 1e-12 of the two-sided SVD's), lam_min >= 1e-8 (no singular value under the
+// return map's 1e-4 clamp), every lam is finite and the sweeps converged; otherwise the caller takes svd().
+// ------------------------------------------------------------------------------------------------
+template <int D, int p, int q>
+MPM_HD void sym_jacobi_rot(M<D>& A, M<D>& U, bool& rotated, double thr) {
+    // every fused operation is written out, so that all kernels that inline this round alike
+    const double apq = A(p, q);
+    if (!(fabs(apq) > thr)) return;
+    rotated = true;
+    // theta = (a_qq - a_pp) / (2 a_pq), t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)) (the smaller rotation),
+    // c = 1 / sqrt(1 + t^2), s = t c; with al = a_qq - a_pp, be = 2 a_pq and g = |al| + hypot(al, be):
+    // t = sgn(al) be / g, c = g / hypot(g, be), s = sgn(al) be / hypot(g, be)
+    const double al = A(q, q) - A(p, p), be = 2.0 * apq;
+    const double r2 = fma(al, al, be * be);
+    const double g = fabs(al) + r2 * rsqrt_d(r2);
+    const double ig = rsqrt_d(fma(g, g, be * be));
+    const double sbe = al < 0 ? -be : be;
+    const double c = g * ig, s = sbe * ig;
+    const double tapq = sbe * apq / g;
+    A(p, p) = A(p, p) - tapq;
+    A(q, q) = A(q, q) + tapq;
+    A(p, q) = 0.0; A(q, p) = 0.0;
+    if constexpr (D == 3) {
+        constexpr int r = 3 - p - q;
+        const double arp = A(r, p), arq = A(r, q);
+        const double np = fma(c, arp, -(s * arq)), nq = fma(s, arp, c * arq);
+        A(r, p) = np; A(p, r) = np; A(r, q) = nq; A(q, r) = nq;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double up = U(k, p), uq = U(k, q);
+        U(k, p) = fma(c, up, -(s * uq)); U(k, q) = fma(s, up, c * uq);
+    }
+}
+
+template <int D, int i, int j>
+MPM_HD void eig_order(V<D>& lam, M<D>& U) {
+    const bool sw = lam[j] > lam[i];
+    const double a = lam[i], b = lam[j];
+    lam[i] = sw ? b : a; lam[j] = sw ? a : b;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double ui = U(k, i), uj = U(k, j);
+        U(k, i) = sw ? uj : ui; U(k, j) = sw ? ui : uj;
+    }
+}
+
+template <int D> MPM_HD bool left_spectral(const M<D>& F, M<D>& U, V<D>& lam, double tol = kSvdTolEigen) {
+    M<D> A;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = r; c < D; ++c) {
+            double v = F(r, 0) * F(c, 0);
+            for (int k = 1; k < D; ++k) v = fma(F(r, k), F(c, k), v);
+            A(r, c) = v; A(c, r) = v;
+        }
+    set_identity(U);
+    bool converged = false;
+    for (int sweep = 0; sweep < 24; ++sweep) {
+        double dmax = fabs(A(0, 0));
+#pragma unroll
+        for (int k = 1; k < D; ++k) dmax = fmax(dmax, fabs(A(k, k)));
+        const double thr = fmax(2.2250738585072014e-308, tol * dmax);
+        bool rotated = false;
+        sym_jacobi_rot<D, 0, 1>(A, U, rotated, thr);
+        if constexpr (D == 3) {
+            sym_jacobi_rot<D, 0, 2>(A, U, rotated, thr);
+            sym_jacobi_rot<D, 1, 2>(A, U, rotated, thr);
+        }
+        if (!rotated) { converged = true; break; }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) lam[k] = A(k, k);
+    eig_order<D, 0, 1>(lam, U);
+    if constexpr (D == 3) {
+        eig_order<D, 1, 2>(lam, U);
+        eig_order<D, 0, 1>(lam, U);
+    }
+    const double lmin = lam[D - 1], lmax = lam[0];
+    return converged && lmax < 1.0e300 && lmin >= 1.0e-8 && lmin >= 1.0e-4 * lmax;
+}
+
+// F' = U diag(e) U^T F: the recomposition U diag(exp h) W^T of the plastic update with e = exp(h) / sqrt(lam).
+template <int D> MPM_HD M<D> recompose_left(const M<D>& U, const V<D>& e, const M<D>& F) {
+    M<D> Y;  // diag(e) U^T F
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double v = U(0, k) * F(0, c);
+            for (int r = 1; r < D; ++r) v = fma(U(r, k), F(r, c), v);
+            Y(k, c) = e[k] * v;
+        }
+    M<D> R;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double v = U(r, 0) * Y(0, c);
+            for (int k = 1; k < D; ++k) v = fma(U(r, k), Y(k, c), v);
+            R(r, c) = v;
+        }
+    return R;
+}
+
 // Symmetric Kirchhoff stress tau = P(F) F^T stored as packed upper triangle:
 // 3D (xx, yy, zz, xy, xz, yz); 2D (xx, yy, xy).
 template <int D> struct SymPack { static constexpr int n = D == 3 ? 6 : 3; };
@@ -289,7 +403,7 @@ template <int D> struct SymPack { static constexpr int n = D == 3 ? 6 : 3; };
 // Hencky: tau = U diag(2 mu ln s + lambda tr(ln s)) U^T, the product HenckyPiola(F) * F^T of
 // simulation.hpp:231-238 (P = U diag((2 mu ln s + lambda tr ln s)/s) W^T, F^T = W diag(s) U^T).
 template <int D> MPM_HD void hencky_tau(const M<D>& U, const V<D>& lnS, double mu, double lambda, double* tau) {
-    #pragma clang fp contract(fast)
+    #pragma clang fp contract(on)
     double tr = lnS[0]; for (int i = 1; i < D; ++i) tr += lnS[i];
     V<D> d; for (int i = 0; i < D; ++i) d[i] = 2.0 * mu * lnS[i] + lambda * tr;
     auto T = [&](int r, int c) { double s = U(r, 0) * d[0] * U(c, 0); for (int k = 1; k < D; ++k) s += U(r, k) * d[k] * U(c, k); return s; };

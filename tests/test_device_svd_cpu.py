@@ -15,7 +15,7 @@ SRC = os.path.join(HERE, "cpp", "device_svd_check.cpp")
 
 
 # The engine compiles with -ffp-contract=off and lets products and sums fuse only inside the functions marked
-# `#pragma clang fp contract(fast)` (the Jacobi rotations, 3x3 products, Hencky stress). The host build uses
+# `#pragma clang fp contract(on)` (the Jacobi rotations, 3x3 products, Hencky stress). The host build uses
 # the same compiler family and flags, with FMA instructions enabled, so that those functions round here as
 # they do on the device (g++ ignores the clang pragma and would check an unfused variant).
 CLANG = "/opt/rocm/llvm/bin/clang++"
