@@ -244,13 +244,20 @@ def host_cpu():
 
 
 COPY_BYTES = 256 * 1024 * 1024
-def traffic_file(prec):
-    return os.path.join(REPO, "profiles", f"c2_{prec}_traffic.txt")
+# HBM (TCC-to-fabric) requests by size: read bytes 32 RDREQ_32B + 64 RDREQ_64B + 128 RDREQ_128B, write bytes
+# 32 (WRREQ - WRREQ_64B) + 64 WRREQ_64B. (FETCH_SIZE counts 128-byte reads through TCC_BUBBLE, which stays zero
+# on gfx950, so it books them as 64 bytes.) No calibration: a 256 MiB device copy in the same run checks them.
+READ_COUNTERS = {"TCC_EA0_RDREQ_32B_sum": 32, "TCC_EA0_RDREQ_64B_sum": 64, "TCC_EA0_RDREQ_128B_sum": 128}
+WRITE_COUNTERS = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
+
+
+def traffic_file(workload, prec):
+    return os.path.join(REPO, "profiles", f"{workload}_{prec}_traffic.txt")
 
 
 def copy_calibration():
-    """Three device copies of a 256 MiB buffer, so that a counter run (rocprofv3 --pmc FETCH_SIZE or
-    WRITE_SIZE) records what the counter reads for a copy of known size next to the engine's kernels."""
+    """Three device copies of a 256 MiB buffer, so that a counter run records the request counts of a copy of
+    known size next to the engine's kernels (the check of the byte formulas)."""
     import torch
     a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
@@ -260,45 +267,62 @@ def copy_calibration():
     del a, b
 
 
-def summarize_pmc(fetch_csv, write_csv):
-    """Per-kernel HBM bytes per dispatch from two rocprofv3 counter-collection CSVs (FETCH_SIZE, WRITE_SIZE,
-    both KiB per dispatch), each scaled by the factor that makes the 256 MiB copy read/write 256 MiB."""
+def summarize_pmc(read_csv, write_csv, form):
+    """Per-kernel HBM bytes per dispatch from two rocprofv3 counter-collection CSVs (the read-request counters
+    READ_COUNTERS, then the write-request counters WRITE_COUNTERS), for the kernels of one precision form
+    ("split": float transfer type, "fp64": double; kernels without one count for both). Each kernel's value is
+    the median over its dispatches, which is a steady-state launch: the run's first step scatters from the
+    unsorted initial layout and reads several times as much."""
     import csv
     import collections
+    import statistics
 
-    def per_kernel(path, counter):
-        agg = collections.defaultdict(list)
+    def per_dispatch(path):
+        out = collections.defaultdict(lambda: collections.defaultdict(float))
+        names = {}
         for r in csv.DictReader(open(path)):
-            if r["Counter_Name"] == counter:
-                agg[r["Kernel_Name"]].append((int(r["Grid_Size"]), float(r["Counter_Value"]) * 1024.0))
-        return agg
+            d = int(r["Dispatch_Id"])
+            names[d] = r["Kernel_Name"]
+            out[d][r["Counter_Name"]] += float(r["Counter_Value"])
+        return names, out
 
-    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    rn, rv = per_dispatch(read_csv)
+    wn, wv = per_dispatch(write_csv)
+    rbytes = {d: sum(w * c.get(k, 0.0) for k, w in READ_COUNTERS.items()) for d, c in rv.items()}
+    wbytes = {d: 32.0 * (c.get(WRITE_COUNTERS[0], 0.0) - c.get(WRITE_COUNTERS[1], 0.0)) + 64.0 * c.get(WRITE_COUNTERS[1], 0.0)
+              for d, c in wv.items()}
+    rc = [rbytes[d] for d, k in rn.items() if "copyBuffer" in k and rbytes[d] > COPY_BYTES / 4]
+    wc = [wbytes[d] for d, k in wn.items() if "copyBuffer" in k and wbytes[d] > COPY_BYTES / 4]
+    if not rc or not wc:
+        raise SystemExit("no 256 MiB copy dispatch in the counter runs (run them with --calibrate-copy)")
+    tf = {"split": "float", "fp64": "double"}[form]
 
-    def copy_factor(agg, which):
-        big = [v for k, lst in agg.items() if "copyBuffer" in k for g, v in lst if v > COPY_BYTES / 4]
-        if not big:
-            raise SystemExit(f"no 256 MiB copy dispatch found in the {which} run (run it with --calibrate-copy)")
-        return COPY_BYTES / (sum(big) / len(big))
+    def grouped(names, values):
+        g = collections.defaultdict(list)
+        for d, k in names.items():
+            if not (k.startswith("void mpm::") or k.startswith("mpm::")):
+                continue
+            args = k.split("<", 1)[1] if "<" in k else ""
+            if ("float" in args or "double" in args) and tf not in args:
+                continue
+            g[k.split("mpm::", 1)[1].split("<")[0].split("(")[0]].append(values[d])
+        return g
 
-    ff, fw = copy_factor(fetch, "FETCH_SIZE"), copy_factor(write, "WRITE_SIZE")
-    lines = [f"# copy calibration: FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f} (256 MiB device copy)",
+    gr, gw = grouped(rn, rbytes), grouped(wn, wbytes)
+    lines = [f"# bytes per dispatch from request counts (32/64/128-byte reads, 32/64-byte writes); check: the 256 MiB "
+             f"copy reads {statistics.median(rc) / COPY_BYTES:.4f} x and writes {statistics.median(wc) / COPY_BYTES:.4f} x "
+             "its size",
+             "# per kernel: median over its dispatches (the first step scatters from the unsorted initial layout)",
              "# kernel  dispatches  hbm_read_bytes_per_dispatch  hbm_write_bytes_per_dispatch"]
-    for k in sorted(fetch):
-        if not k.startswith("void mpm::") and not k.startswith("mpm::"):
-            continue
-        name = k.split("mpm::", 1)[1].split("<")[0].split("(")[0]
-        fv = [v for _, v in fetch[k]]
-        wv = [v for _, v in write.get(k, [])]
-        if not wv:
-            continue
-        lines.append(f"{name} {len(fv)} {ff * sum(fv) / len(fv):.0f} {fw * sum(wv) / len(wv):.0f}")
+    for k in sorted(gr):
+        if k in gw:
+            lines.append(f"{k} {len(gr[k])} {statistics.median(gr[k]):.0f} {statistics.median(gw[k]):.0f}")
     return "\n".join(lines)
 
 
-def read_traffic(kernel, prec):
+def read_traffic(kernel, workload, prec):
     """HBM bytes per launch (read + write) of `kernel` from the committed counter summary, or None."""
-    path = traffic_file(prec)
+    path = traffic_file(workload, prec)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -319,12 +343,15 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="run a 256 MiB device copy first (counter runs)")
     ap.add_argument("--precision", choices=["both", "split", "fp64"], default="both",
                     help="forms to run; the first one run (split when both) gives the headline value")
+    ap.add_argument("--only", choices=["c3", "batched"], default=None,
+                    help="run only this extra workload and print its object (counter and trace runs)")
     ap.add_argument("--extra", default="c1,c3,batched,served",
                     help="comma-separated extra workloads reported beside the headline: c1 (the 2D column collapse, "
                          "GPU and oracle, 200 steps), c3 (inclined flow), batched "
                          "(128 scenes), served (Poisson request stream); empty for none")
-    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
-                    help="print the per-kernel HBM traffic summary of two rocprofv3 --pmc runs and exit")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("READ_CSV", "WRITE_CSV", "FORM"),
+                    help="print the per-kernel HBM traffic summary of two rocprofv3 --pmc runs (request counters) "
+                         "for one form (split or fp64) and exit")
     args = ap.parse_args()
     if args.summarize_pmc:
         print(summarize_pmc(*args.summarize_pmc))
@@ -336,6 +363,9 @@ def main():
 
     if args.calibrate_copy:
         copy_calibration()
+    if args.only:
+        print(json.dumps({"c3": incline_run, "batched": batched_run}[args.only](args)), flush=True)
+        return
     forms = ["split", "fp64"] if args.precision == "both" else [args.precision]
     runs = {}
     for prec in forms:
@@ -445,7 +475,7 @@ def incline_run(args):
             rc, _ = eng.step(steps)
             if rc:
                 raise A.MPMError(rc, "c3 warm-up")
-    kt = {k: {"avg_ms": ms / max(nl, 1), "launches": nl} for k, (ms, nl) in eng.kernel_times().items()}
+    warm_kt = eng.kernel_times() or {}
     eng.set_profiling(False)
     t0 = time.perf_counter()
     rc, info = eng.step(args.steps)
@@ -453,10 +483,12 @@ def incline_run(args):
     if rc or info.steps_done != args.steps:
         raise A.MPMError(rc or A.ERR["NOT_READY"], "c3 timed steps")
     wall = t1 - t0
+    kernels, roofline, (grid_nodes, active, blocks) = kernel_table(eng, cfg, n, warm_kt, {}, "c3", "split")
     return {"workload": "3D flow down a 35 deg incline with a rectangular bump, mu(I)-MCC, cubic, AFLIP, MIBF",
             "particles": n, "precision": "split", "value": n * args.steps / wall, "unit": "particle-updates/s",
             "ms_per_step": 1e3 * wall / args.steps, "steps": args.steps, "plastic_count_last_step": info.plastic_count,
-            "kernels": kt}
+            "grid_nodes_last_step": grid_nodes, "active_nodes_last_step": active, "blocks_last_step": blocks,
+            "roofline": roofline, "kernels": kernels}
 
 
 def batched_run(args, count=128):
@@ -526,6 +558,38 @@ def served_run(args, count=48):
     return d
 
 
+def kernel_table(eng, cfg, n, warm_kt, live_kt, workload, prec):
+    """Per-kernel average durations (the timed steps' where measured there, else the warm-up's), algorithmic
+    bytes and achieved GB/s on the engine's last-step grid, and the roofline of the kernel that took the most
+    time in the timed steps (with its counter-measured traffic from profiles/<workload>_<prec>_traffic.txt)."""
+    kt = dict(warm_kt)
+    kt.update(live_kt)
+    g = eng.get_grid()
+    active = int(np.count_nonzero(g["mass"] > 0))
+    grid_nodes = int(g["info"].grid_nodes)
+    _, keys = eng.get_binning()
+    blocks = int(np.unique(keys).size)  # non-empty blocks of the last step
+    del g, keys
+    w = 3 if cfg.spline_degree == 2 else 4
+    tile_nodes = (4 + w - 1) ** 3 if cfg.dim == 3 else (8 + w - 1) ** 2
+    kernels = {}
+    for name, (ms, launches) in kt.items():
+        avg_ms = ms / max(launches, 1)
+        b = algorithmic_bytes(name, cfg, n, active, grid_nodes, blocks, tile_nodes)
+        kernels[name] = {"avg_ms": avg_ms, "launches": launches, "total_ms": ms,
+                         "algorithmic_bytes": b, "GBps": (b / (avg_ms * 1e-3) / 1e9) if avg_ms > 0 and b else None,
+                         "timed_over": "timed steps" if name in live_kt else "warm-up steps"}
+    src = live_kt or warm_kt
+    dom = max(src, key=lambda k: src[k][0])
+    d = kernels[dom]
+    traffic = read_traffic(dom, workload, prec)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (d["GBps"] / HBM_PEAK_GBS) if d["GBps"] else None, "traffic": traffic,
+                "traffic_source": os.path.relpath(traffic_file(workload, prec), REPO) if traffic is not None else None,
+                "avg_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
+    return kernels, roofline, (grid_nodes, active, blocks)
+
+
 def timed_run(prec, args):
     """W warm-up steps (the last W-1 with every kernel timed by HIP events), then K steps timed by the host clock
     around one synchronised mpm_step call, with HIP events around the dominant kernel over the K steps."""
@@ -560,30 +624,8 @@ def timed_run(prec, args):
         raise A.MPMError(rc or A.ERR["NOT_READY"], f"timed steps ({info.steps_done}/{args.steps} done)")
     live_kt = eng.kernel_times()
     eng.set_profiling(False)
-    kt = dict(warm_kt)
-    kt.update(live_kt)
     wall = t1 - t0
-    g = eng.get_grid()
-    active = int(np.count_nonzero(g["mass"] > 0))
-    grid_nodes = int(g["info"].grid_nodes)
-    _, keys = eng.get_binning()
-    blocks = int(np.unique(keys).size)  # non-empty blocks of the last step
-    w = 3 if cfg.spline_degree == 2 else 4
-    tile_nodes = (4 + w - 1) ** 3 if cfg.dim == 3 else (8 + w - 1) ** 2
-    kernels = {}
-    for name, (ms, launches) in kt.items():
-        avg_ms = ms / max(launches, 1)
-        b = algorithmic_bytes(name, cfg, n, active, grid_nodes, blocks, tile_nodes)
-        kernels[name] = {"avg_ms": avg_ms, "launches": launches, "total_ms": ms,
-                         "algorithmic_bytes": b, "GBps": (b / (avg_ms * 1e-3) / 1e9) if avg_ms > 0 and b else None,
-                         "timed_over": "timed steps" if name in live_kt else "warm-up steps"}
-    dom = max(live_kt, key=lambda k: live_kt[k][0])
-    d = kernels[dom]
-    traffic = read_traffic(dom, prec)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (d["GBps"] / HBM_PEAK_GBS) if d["GBps"] else None, "traffic": traffic,
-                "traffic_source": os.path.relpath(traffic_file(prec), REPO) if traffic is not None else None,
-                "avg_ms": d["avg_ms"], "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
+    kernels, roofline, (grid_nodes, active, blocks) = kernel_table(eng, cfg, n, warm_kt, live_kt, "c2", prec)
     bpu = step_bytes_per_particle_update(cfg)
     step_gbs = bpu * n / (wall / args.steps) / 1e9
     step_roofline = {"bytes_per_particle_update": bpu, "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
