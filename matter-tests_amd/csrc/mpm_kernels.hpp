@@ -1080,11 +1080,10 @@ __device__ __forceinline__ bool slice_stencil(SliceStencil<D, SPL, SPP, OBN, TF>
 // MINW: the smallest window weight per axis; a product of the three below 1e-24 (a node near the reference's
 // 1e-25 weight cut) sends the particle to the direct per-node form instead.
 // ----------------------------------------------------------------------------------------------
-// Slots per chunk: measured on the 7.95 M-particle inclined flow, the cubic MIBF P2G ran 2705, 2519, 2152, 1951,
-// 1976, 2088, 2160 and 2258 us at 64, 96, 128, 192, 224, 256, 320 and 384 (the workgroup runs alone on its CU
-// either way, so larger chunks cost no occupancy; fewer, longer chunks trade barriers against longer serial
-// staging).
-constexpr int kStageCH = 192;
+// Slots per chunk: measured on the 7.95 M-particle inclined flow with two blocks per CU (the P2G bounded to 128
+// VGPRs), the cubic MIBF P2G ran 2005, 2286, 1631, 1886 and 1697 us at 64, 96, 128, 160 and 192; 256 and more
+// leave room for one block per CU (over 80 KB of LDS).
+constexpr int kStageCH = 128;
 
 template <int D, int SPL, bool MU> struct StagedRec {
     static constexpr int W = BG<D, SPL>::W;
@@ -1575,6 +1574,11 @@ template <int D, int SPL, class TF, bool PBC> constexpr bool kStagedScatter = D 
 // Workgroup size of a scatter kernel: the staged shape or the register shape of scatter_block.
 template <int D, int SPL, int NV, class TF, bool PBC> struct ScatterLaunch {
     static constexpr int TB = kStagedScatter<D, SPL, TF, PBC> ? StagedShape<D, SPL, NV>::TB : ScatterShape<D, SPL, NV, TF>::TB;
+    // Minimum wavefronts per SIMD that the compiler must allow (amdgpu_waves_per_eu). The staged P2G scatter
+    // is bounded to 4 (at most 128 VGPRs): two 8-wavefront blocks per CU instead of one at 158 VGPRs, at the
+    // price of 18 spilled registers (C3: 2.10 -> 1.70 ms). The 4-value payload (folded force, no MIBF) does not fit in
+    // 128 and other scatters are not bounded (1).
+    static constexpr int WPE = kStagedScatter<D, SPL, TF, PBC> && NV > D + 1 ? 4 : 1;
 };
 
 template <int D, int SPL, int NV, class TF, int NR> struct StagedLDS {
@@ -1638,8 +1642,8 @@ __device__ __forceinline__ void stage_write(const Contrib& cf, uint32_t m, const
     }
 }
 
-// Phases 1 and 2 over the block's ordered particles, chunk by chunk, with the next chunk's loads issued before
-// the current chunk is accumulated. All wavefronts see the same 64 cells, hence the same chunks and barriers.
+// Phases 1 and 2 over the block's ordered particles, chunk by chunk. All wavefronts see the same 64 cells, hence
+// the same chunks and barriers.
 template <int D, int SPL, int NV, class TF, class Contrib, class LT>
 __device__ __forceinline__ void staged_accumulate(LT& L, const Contrib& cf, uint32_t start,
                                                   TF (&acc)[1][StagedShape<D, SPL, NV>::NSH][NV]) {
@@ -1653,14 +1657,15 @@ __device__ __forceinline__ void staged_accumulate(LT& L, const Contrib& cf, uint
     typename Contrib::Pre pre[NP];
     uint32_t j = 0, kb = 0, kb1 = 0;
     uint32_t j1 = stage_chunk_end(0, 0, n_max, n_c, kb1);
-    stage_load<D>(cf, L.p, start, kb, kb1 - kb, pre);
     while (j < n_max) {
+        // The chunk's loads are issued before the barrier; loading the next chunk during this one's accumulation
+        // held its values in registers through it, which no longer fit once the kernel is bounded to 128 VGPRs.
+        stage_load<D>(cf, L.p, start, kb, kb1 - kb, pre);
         __syncthreads();  // every wavefront is done with the previous chunk's records
         stage_write<D>(cf, kb1 - kb, pre, L.u.rec);
         __syncthreads();
         uint32_t kb2 = kb1;
         const uint32_t j2 = stage_chunk_end(j1, kb1, n_max, n_c, kb2);
-        if (j1 < n_max) stage_load<D>(cf, L.p, start, kb1, kb2 - kb1, pre);  // next chunk, in flight meanwhile
         uint32_t k = kb;
         for (; j < j1; ++j) {
             const bool has = j < n_c;
@@ -1875,6 +1880,7 @@ __device__ __forceinline__ void p2g_plain_body(const Params& P, StepState* S, co
 }
 template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
 __global__ __launch_bounds__((ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::TB))
+__attribute__((amdgpu_waves_per_eu(ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::WPE)))
 void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
     p2g_body<D, SPL, SEP, MIBF, TF, PBC>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }
@@ -2573,8 +2579,12 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
         if (err) atomicOr(&S->err, err);
     }
 }
+// At least 3 wavefronts per SIMD (at most 168 VGPRs): the MCC-family return maps take up to 211 and would run 2
+// (C3, mu(I)-MCC: 1080 -> 935 us with 156 bytes of spill; a bound of 4 spilled more and ran 1139 us). Kernels
+// under 168 VGPRs are unchanged.
+constexpr int kDefgradWaves = 3;
 template <int D, int SPL, class TF, int MAT>
-__global__ __launch_bounds__(kBlock) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kDefgradWaves))) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
     defgrad_body<D, SPL, TF, MAT>(P, S, src, dst, sb, g, n, xcd_slab(blockIdx.x, gridDim.x));
 }
 
