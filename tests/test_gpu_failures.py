@@ -57,6 +57,25 @@ def test_nan_deformation_gradient_stops_the_run(engine):
     assert rc & A.ERR["NONFINITE"] and 1 <= info.steps_done <= 2 and info.step == info.steps_done and info.time > 0.0
 
 
+def test_supersonic_particle_warns_and_steps_on(engine, oracle):
+    """updateDt (update_dt.cpp:14-17): a particle at or above the elastic wave speed sqrt(E / rho) raises the
+    warning bit and the step is taken, with the oracle's dt and the same bits."""
+    print("supersonic warning", flush=True)
+    c_wave = (1e6 / 1000.0) ** 0.5
+    cfg = A.default_config(dim=3, dx=0.01, particle_volume=1e-6, particle_mass=1e-3, rho=1000.0, E=1e6,
+                           flip_ratio=-2.0, fps=1.0)
+    x = np.array([[0.0, 0.0, 0.0], [0.003, 0.002, 0.001], [0.006, 0.001, 0.004]])
+    v = np.tile([1.2 * c_wave, 0.0, 0.0], (3, 1))
+    out = []
+    for lib in (engine, oracle):
+        e = A.Engine(lib, cfg, [])
+        e.set_particles(x, v=v)
+        out.append(e.step(1))
+    (rg, ig), (ro, io) = out
+    assert rg == ro == 0 and ig.status == io.status and ig.status & A.ERR["WARN_SUPERSONIC"], A.status_names(ig.status)
+    assert ig.steps_done == io.steps_done == 1 and ig.dt == io.dt and ig.time == io.time
+
+
 def test_invalid_enumerations_are_refused_when_added(engine):
     cfg = A.default_config(dim=3, dx=0.01, particle_volume=1e-6, particle_mass=1e-3)
     for make in (lambda: A.make_object("Ground", "NoSlip", 0.0), lambda: A.make_plate(0.0, "bottom", "NoSlip")):
