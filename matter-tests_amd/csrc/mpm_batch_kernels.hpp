@@ -114,8 +114,7 @@ __global__ __launch_bounds__(kBlock) void kb_bin_scatter(BatchTable T) {
 }
 
 template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
-__global__ __launch_bounds__((ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::TB))
-__attribute__((amdgpu_waves_per_eu(ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::WPE))) void kb_p2g(BatchTable T) {
+__global__ __launch_bounds__((ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::TB)) void kb_p2g(BatchTable T) {
     BatchSlot<TF> q(T, BR_BLOCK);
     p2g_body<D, SPL, SEP, MIBF, TF, PBC>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.r->stg, q.local);
 }
@@ -145,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void kb_musl_update(BatchTable T) {
 }
 
 template <int D, int SPL, class TF, int MAT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kDefgradWaves))) void kb_defgrad(BatchTable T) {
+__global__ __launch_bounds__(kBlock) void kb_defgrad(BatchTable T) {
     BatchSlot<TF> q(T, BR_PART);
     defgrad_body<D, SPL, TF, MAT>(q.P(), q.r->S, q.src(), q.dst(), q.r->sb, q.r->g, q.r->n, q.local);
 }

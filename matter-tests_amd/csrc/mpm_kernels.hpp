@@ -876,7 +876,7 @@ __device__ __forceinline__ void add_partials(LT& L, int cell, int a0, int half, 
 // Ordered accumulation (see above) of the block's particles into its tile, written to staging.
 // contrib.load(p, i, part) fetches source particle p (sorted slot i); contrib.add(part, a0, half, acc, base)
 // adds it to acc for slices a0.. and second-axis half `half` and returns false if its window leaves the grid.
-template <int D, int SPL, int NV, class TF, class Contrib>
+template <int D, int SPL, int NV, class TF, class Contrib, bool NEXT = false>
 __device__ __forceinline__ void accumulate_cells(const ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, uint32_t start,
                                                  TF (&acc)[ScatterShape<D, SPL, NV, TF>::SPP][ScatterShape<D, SPL, NV, TF>::NSH][NV]) {
     using SS = ScatterShape<D, SPL, NV, TF>;
@@ -888,15 +888,37 @@ __device__ __forceinline__ void accumulate_cells(const ScatterLDS<D, SPL, NV, TF
         const unsigned long long below = (1ull << lane) - 1ull;
         uint32_t kb = 0;
         int base[3];
-        for (uint32_t j = 0; j < n_max; ++j) {
-            const bool has = j < n_c;
-            const unsigned long long m = __ballot(has);
-            const uint32_t k = kb + (uint32_t)__popcll(m & below);
-            kb += (uint32_t)__popcll(m);
-            if (has) {
-                typename Contrib::Part q;
-                contrib.load(L.p[k], start + k, q);
-                (void)contrib.add(q, a0, half, acc, base);
+        // NEXT: load the next rank's particle while this one is added (kP2GNextRank)
+        if constexpr (NEXT) {
+            typename Contrib::Part qn;
+            bool hasn = 0 < n_c;
+            {
+                const unsigned long long m = __ballot(hasn);
+                const uint32_t k = kb + (uint32_t)__popcll(m & below);
+                kb += (uint32_t)__popcll(m);
+                if (hasn) contrib.load(L.p[k], start + k, qn);
+            }
+            for (uint32_t j = 0; j < n_max; ++j) {
+                const typename Contrib::Part q = qn;
+                const bool has = hasn;
+                hasn = j + 1 < n_c;
+                const unsigned long long m = __ballot(hasn);
+                const uint32_t k = kb + (uint32_t)__popcll(m & below);
+                kb += (uint32_t)__popcll(m);
+                if (hasn) contrib.load(L.p[k], start + k, qn);
+                if (has) (void)contrib.add(q, a0, half, acc, base);
+            }
+        } else {
+            for (uint32_t j = 0; j < n_max; ++j) {
+                const bool has = j < n_c;
+                const unsigned long long m = __ballot(has);
+                const uint32_t k = kb + (uint32_t)__popcll(m & below);
+                kb += (uint32_t)__popcll(m);
+                if (has) {
+                    typename Contrib::Part q;
+                    contrib.load(L.p[k], start + k, q);
+                    (void)contrib.add(q, a0, half, acc, base);
+                }
             }
         }
     }
@@ -934,13 +956,13 @@ __device__ __forceinline__ void zero_acc(Acc& acc) {
             for (int v = 0; v < NV; ++v) acc[s][o][v] = TF(0);
 }
 
-template <int D, int SPL, int NV, class TF, class Contrib>
+template <int D, int SPL, int NV, class TF, class Contrib, bool NEXT = false>
 __device__ __forceinline__ void scatter_block(ScatterLDS<D, SPL, NV, TF>& L, const Contrib& contrib, const StagingT<TF>& stg, uint32_t b,
                               uint32_t start) {
     using SS = ScatterShape<D, SPL, NV, TF>;
     TF acc[SS::SPP][SS::NSH][NV];
     zero_acc<D, SPL, NV, TF>(acc);
-    accumulate_cells<D, SPL, NV, TF>(L, contrib, start, acc);
+    accumulate_cells<D, SPL, NV, TF, Contrib, NEXT>(L, contrib, start, acc);
     finish_tile<D, SPL, NV, TF>(L, stg, b, acc);
 }
 
@@ -1577,7 +1599,9 @@ template <int D, int SPL, int NV, class TF, bool PBC> struct ScatterLaunch {
     // Minimum wavefronts per SIMD that the compiler must allow (amdgpu_waves_per_eu). The staged P2G scatter
     // is bounded to 4 (at most 128 VGPRs): two 8-wavefront blocks per CU instead of one at 158 VGPRs, at the
     // price of 18 spilled registers (C3: 2.10 -> 1.70 ms). The 4-value payload (folded force, no MIBF) does not fit in
-    // 128 and other scatters are not bounded (1).
+    // 128 and other scatters are not bounded (1). Only the single-scene kernel takes the bound: the batched one spills
+    // more under it (420 bytes), and with it and the deformation bound the served stream of inclined flows and
+    // collapses ran 1.74e9 -> 1.49e9 particle-updates/s (1.70e9 with both bounds on the single-scene kernels only).
     static constexpr int WPE = kStagedScatter<D, SPL, TF, PBC> && NV > D + 1 ? 4 : 1;
 };
 
@@ -1805,18 +1829,25 @@ __device__ __forceinline__ void p2g_staged_body(const Params& P, StepState* S, c
     if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
 }
 
-template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC = false>
+template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC = false, bool NEXT = false>
 __device__ __forceinline__ void p2g_plain_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                                const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b);
 
-template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC = false>
+// The single-scene quadratic split P2G with folded force and no MIBF (one wavefront per block at 229 VGPRs, 2 per
+// SIMD) loads the next rank's particle while it adds this one: 255 VGPRs, still 2 per SIMD, C2 k_p2g 225 -> 215
+// us. Elsewhere it would cost occupancy: the batched P2G and the separate-force and plain forms reach 256 VGPRs
+// and 1 wavefront per SIMD, and the MUSL re-scatter dropped from 6 to 4 and ran 153 -> 171 us.
+template <int D, int SPL, bool SEP, bool MIBF, class TF>
+constexpr bool kP2GNextRank = D == 3 && SPL == 2 && sizeof(TF) == 4 && !SEP && !MIBF;
+
+template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC = false, bool NEXT = false>
 __device__ __forceinline__ void p2g_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                          const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
     if constexpr (kStagedScatter<D, SPL, TF, PBC>) p2g_staged_body<D, SPL, SEP, MIBF, TF>(P, S, src, dst, sb, g, stg, b);
-    else p2g_plain_body<D, SPL, SEP, MIBF, TF, PBC>(P, S, src, dst, sb, g, stg, b);
+    else p2g_plain_body<D, SPL, SEP, MIBF, TF, PBC, NEXT>(P, S, src, dst, sb, g, stg, b);
 }
 
-template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
+template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC, bool NEXT>
 __device__ __forceinline__ void p2g_plain_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                                const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
     constexpr int NV = P2GContrib<D, SPL, SEP, MIBF, TF>::NV;
@@ -1853,7 +1884,7 @@ __device__ __forceinline__ void p2g_plain_body(const Params& P, StepState* S, co
         err |= scatter_block_atomic<D, SPL, NV, TF>(L, cf, stg, sb, b, start, cnt, bc, gstart, gcnt);
     } else if (gcnt == 0) {
         err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
-        scatter_block<D, SPL, NV, TF>(L, cf, stg, b, start);
+        scatter_block<D, SPL, NV, TF, P2GContrib<D, SPL, SEP, MIBF, TF>, NEXT>(L, cf, stg, b, start);
     } else {
         TF acc[SS::SPP][SS::NSH][NV];
         zero_acc<D, SPL, NV, TF>(acc);
@@ -1882,7 +1913,7 @@ template <int D, int SPL, bool SEP, bool MIBF, class TF, bool PBC>
 __global__ __launch_bounds__((ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::TB))
 __attribute__((amdgpu_waves_per_eu(ScatterLaunch<D, SPL, P2GContrib<D, SPL, SEP, MIBF, TF>::NV, TF, PBC>::WPE)))
 void k_p2g(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, StagingT<TF> stg) {
-    p2g_body<D, SPL, SEP, MIBF, TF, PBC>(P, S, src, dst, sb, g, stg, blockIdx.x);
+    p2g_body<D, SPL, SEP, MIBF, TF, PBC, kP2GNextRank<D, SPL, SEP, MIBF, TF>>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }
 
 // k_musl_scatter: the MUSL re-scatter, in the particle order that k_p2g fixed for the step.
@@ -2581,7 +2612,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
 }
 // At least 3 wavefronts per SIMD (at most 168 VGPRs): the MCC-family return maps take up to 211 and would run 2
 // (C3, mu(I)-MCC: 1080 -> 935 us with 156 bytes of spill; a bound of 4 spilled more and ran 1139 us). Kernels
-// under 168 VGPRs are unchanged.
+// under 168 VGPRs are unchanged. The batched kb_defgrad is not bounded (see ScatterLaunch::WPE).
 constexpr int kDefgradWaves = 3;
 template <int D, int SPL, class TF, int MAT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kDefgradWaves))) void k_defgrad(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
