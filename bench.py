@@ -66,6 +66,10 @@ def algorithmic_bytes(kernel, cfg, n_particles, active_nodes, grid_nodes, blocks
         return (part_r + part_w) * n_particles + ft * (D + (D if flip else 0)) * active_nodes
     if kernel == "k_musl_scatter":
         return (f8 * D + ft * (D + B)) * n_particles + ft * D * staged
+    if kernel == "k_g2p_musl":  # k_g2p, and the re-scatter's tiles without its second pass over the particles
+        part_r = f8 * D + (ft * D if flip else 0)
+        part_w = f8 * D + ft * (B + D)
+        return (part_r + part_w) * n_particles + ft * (D + (D if flip else 0)) * active_nodes + ft * D * staged
     if kernel == "k_musl_update":
         return ft * (D * staged + (1 + D) * grid_nodes)
     if kernel == "k_defgrad":

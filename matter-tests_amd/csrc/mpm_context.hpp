@@ -87,9 +87,9 @@ template <> inline StagingT<float>& staging_of<float>(mpm_context* c) { return c
 
 inline const char* const kKernelNames[] = {"k_setup", "k_zero_grid", "k_p2g", "k_grid_update", "k_g2p",
                                     "k_musl_scatter", "k_musl_update", "k_defgrad", "k_finish", "k_bin_count", "k_bin_scan",
-                                    "k_bin_scatter"};
+                                    "k_bin_scatter", "k_g2p_musl"};
 enum KernelId { KS_SETUP, KS_ZERO, KS_P2G, KS_GRID, KS_G2P, KS_MUSL_SCATTER, KS_MUSL_UPDATE, KS_DEFGRAD,
-                KS_FINISH, KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_COUNT };
+                KS_FINISH, KS_BIN_COUNT, KS_BIN_SCAN, KS_BIN_SCATTER, KS_G2P_MUSL, KS_COUNT };
 
 // Records an event pair around one launch when profiling; returns the launch's slot.
 struct LaunchTimer {

@@ -654,13 +654,13 @@ __global__ __launch_bounds__(kBlock) void k_binning(Params P, const StepState* S
 // one after another, every lane to a different node at each instruction. The sums have a fixed order,
 // and the tile goes to the staging area.
 // ----------------------------------------------------------------------------------------------
-template <int D, int SPL, int NV, class TF> struct ScatterShape {
+template <int D, int SPL, int NV, class TF, int REGS_ = 0> struct ScatterShape {
     using G = BG<D, SPL>;
     // 32-bit registers for the partial sums. The quadratic P2G scatter (NV > D) takes enough for one
     // wavefront to cover all or most of a particle's window slices, which saves the repeated stencil and
     // particle loads of the other wavefronts (C2 split: k_p2g 289 -> 234 us at 229 VGPRs); the MUSL
     // re-scatter (NV == D) and the cubic scatters were measured as fast or faster with 40.
-    static constexpr int REGS = (NV > D && SPL == 2) ? 108 : 40;
+    static constexpr int REGS = REGS_ > 0 ? REGS_ : ((NV > D && SPL == 2) ? 108 : 40);
     static constexpr int WORDS = NV * (int)(sizeof(TF) / 4);
     static constexpr int SPP0 = REGS / (G::NS * WORDS);
     static constexpr int SPP = SPP0 < 1 ? 1 : (SPP0 > G::W ? G::W : SPP0);   // slices per wavefront
@@ -945,13 +945,13 @@ __device__ __forceinline__ void finish_tile(ScatterLDS<D, SPL, NV, TF>& L, const
     write_tile<D, SPL, NV, TF>(L, stg, b);
 }
 
-template <int D, int SPL, int NV, class TF, class Acc>
-__device__ __forceinline__ void zero_acc(Acc& acc) {
-    using SS = ScatterShape<D, SPL, NV, TF>;
+// Clears a partial-sum array of any register shape (its extents are the array's own).
+template <int D, int SPL, int NV, class TF, int SPP, int NSH>
+__device__ __forceinline__ void zero_acc(TF (&acc)[SPP][NSH][NV]) {
 #pragma unroll
-    for (int s = 0; s < SS::SPP; ++s)
+    for (int s = 0; s < SPP; ++s)
 #pragma unroll
-        for (int o = 0; o < SS::NSH; ++o)
+        for (int o = 0; o < NSH; ++o)
 #pragma unroll
             for (int v = 0; v < NV; ++v) acc[s][o][v] = TF(0);
 }
@@ -2464,6 +2464,38 @@ template <int D, int SPL, class TF, int SETS> struct SlotTiles {
 // leaves fewer threads idle in a block's last pass (a block holds about 440 particles at 8 per cell).
 constexpr int kG2PThreads = 64;
 
+// G2P, velocity and position update of source particle p into slot i of the destination layout, from the block's
+// LDS tiles: false if its window leaves the tile. st, xp, vs and Bp return the stencil, the position of the step's
+// start, the new velocity and the new affine matrix as stored (B zero without APIC).
+template <int D, int SPL, bool FLIP, class TF, class CS>
+__device__ __forceinline__ bool g2p_one(const Params& P, const PartStateT<TF>& src, const PartStateT<TF>& dst, uint32_t p, uint32_t i,
+                                        const double* org, const uint32_t* nn, const CS& cs, const int* bc, double h1, double dt,
+                                        bool apic, bool flip_rt, const TF* s_v, const TF* s_f, Stencil<D, SPL>& st,
+                                        double (&xp)[3], TF (&vs)[3], TF (&Bp)[3][3], BoundsAcc& ba) {
+    for (int d = 0; d < 3; ++d) xp[d] = d < D ? src.x[d][p] : 0.0;
+    int tl[3];
+    if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, cs, h1, bc)) return false;
+    TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, gv[3][3];
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Bp[r][c] = TF(0);
+    gather<D, SPL, true, false, FLIP, TF>(st, xp, h1, TileFetch<D, SPL, TF>{s_v, s_f, tl[0], tl[1], tl[2]}, apic, flip_rt, pic, fl, Bp, gv);
+    double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+        const double vo = FLIP ? (double)src.v[d][p] : 0.0;
+        vnew[d] = (double)(TF)blend_velocity(P.flip_ratio, vo, (double)fl[d], (double)pic[d]);
+        xnew[d] = xp[d] + dt * (double)pic[d];
+    }
+    if (P.pbc) {  // position_update.cpp:26-33
+        if (xnew[0] > P.L[0]) xnew[0] = xnew[0] - P.L[0];
+        else if (xnew[0] < 0) xnew[0] = P.L[0] + xnew[0];
+    }
+    if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
+    for (int d = 0; d < D; ++d) { vs[d] = (TF)vnew[d]; dst.v[d][i] = vs[d]; dst.x[d][i] = xnew[d]; }
+    if (!apic) for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Bp[r][c] = TF(0);
+    for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = Bp[r][c];
+    ba.add<D>(xnew, vnew);
+    return true;
+}
+
 template <int D, int SPL, bool FLIP, class TF>
 __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
                                               const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b) {
@@ -2486,28 +2518,13 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
     BoundsAcc ba;
 #pragma unroll 1
     for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) {
-        const uint32_t i = br.start + k, p = sb.perm[i];
-        double xp[3] = {0, 0, 0};
-        for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
+        const uint32_t i = br.start + k;
         Stencil<D, SPL> st;
-        int tl[3];
-        if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, s_c.view(br.bc), h1, br.bc)) { err |= MPM_ERR_GRID_BOUNDS; continue; }
-        TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[3][3] = {{0}}, gv[3][3];
-        gather<D, SPL, true, false, FLIP, TF>(st, xp, h1, TileFetch<D, SPL, TF>{s_v, s_f, tl[0], tl[1], tl[2]}, apic, flip_rt, pic, fl, Bp, gv);
-        double vnew[3] = {0, 0, 0}, xnew[3] = {0, 0, 0};
-        for (int d = 0; d < D; ++d) {
-            const double vo = useflip ? (double)src.v[d][p] : 0.0;
-            vnew[d] = (double)(TF)blend_velocity(P.flip_ratio, vo, (double)fl[d], (double)pic[d]);
-            xnew[d] = xp[d] + dt * (double)pic[d];
-        }
-        if (P.pbc) {  // position_update.cpp:26-33
-            if (xnew[0] > P.L[0]) xnew[0] = xnew[0] - P.L[0];
-            else if (xnew[0] < 0) xnew[0] = P.L[0] + xnew[0];
-        }
-        if (P.change_particle_positions && xnew[0] > 0.8) { xnew[0] = -0.1; xnew[1] += 0.27; }
-        for (int d = 0; d < D; ++d) { dst.v[d][i] = (TF)vnew[d]; dst.x[d][i] = xnew[d]; }
-        for (int c = 0; c < D; ++c) for (int r = 0; r < D; ++r) dst.B[r + c * D][i] = apic ? Bp[r][c] : TF(0);
-        ba.add<D>(xnew, vnew);
+        double xp[3];
+        TF vs[3], Bp[3][3];
+        if (!g2p_one<D, SPL, FLIP, TF>(P, src, dst, sb.perm[i], i, org, nn, s_c.view(br.bc), br.bc, h1, dt, apic, flip_rt, s_v, s_f,
+                                       st, xp, vs, Bp, ba))
+            err |= MPM_ERR_GRID_BOUNDS;
     }
     reduce_bounds<D>(S, ba);
     err = wave_or_u32(err);
@@ -2516,6 +2533,150 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
 template <int D, int SPL, bool FLIP, class TF>
 __global__ __launch_bounds__(kG2PThreads) void k_g2p(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g) {
     g2p_body<D, SPL, FLIP, TF>(P, S, src, dst, sb, g, blockIdx.x);
+}
+
+// ----------------------------------------------------------------------------------------------
+// k_g2p_musl: k_g2p and the MUSL re-scatter (musl.cpp:28-96) of a block in one pass, for the 3D quadratic split
+// form. One wavefront per block, lane = dual cell as in the P2G scatter: at rank j a lane updates its cell's j-th
+// particle (g2p_one, written to the next layout as k_g2p writes it) and adds that particle's re-scatter terms
+// w_i (v + B D^-1 (x_i - x_p)) over its 27 window nodes to register partial sums, with the gather's weights and
+// offsets rounded to binary32, which are the values the re-scatter's own stencil gives. The tile goes to staging
+// as k_musl_scatter's does. This saves the re-scatter's pass over the particles: its loads of x, v and B and the
+// three wavefronts that each evaluated every particle's stencil again.
+// ----------------------------------------------------------------------------------------------
+template <int D, int SPL, class TF> constexpr bool kFusedMusl = D == 3 && SPL == 2 && sizeof(TF) == 4;
+// All three first-axis slices in one wavefront: 27 nodes x 3 values of partial sums.
+template <int D, int SPL, class TF> using FusedMuslShape = ScatterShape<D, SPL, D, TF, 108>;
+
+// The re-scatter terms of one particle (MuslContrib::add on the stencil's weights): acc[a][b W + c][r] +=
+// w_abc (v_r + (B D^-1)_r0 (x_a - x_p) + (B D^-1)_r1 (y_b - y_p) + (B D^-1)_r2 (z_c - z_p)) in that order of
+// operations, with weights at or under the 1e-25 cut dropped (g2p.cpp:44, musl.cpp:48).
+template <int D, int SPL, class TF, class SS>
+__device__ __forceinline__ void musl_terms(const Stencil<D, SPL>& st, const double* xp, const TF (&v)[3], const TF (&B)[3][3], TF aD,
+                                           TF (&acc)[SS::SPP][SS::NSH][D]) {
+    constexpr int W = BG<D, SPL>::W;
+    static_assert(D == 3 && SS::SPP == W && SS::NSH == W * W, "one wavefront covers the whole window");
+    TF w[3][W], pd[3][W];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < W; ++k) { w[d][k] = (TF)st.w[d][k]; pd[d][k] = (TF)(st.xi[d][k] - xp[d]); }
+    TF Bc[3][3];
+    for (int r = 0; r < 3; ++r) for (int k = 0; k < 3; ++k) Bc[r][k] = B[r][k] * aD;
+#pragma unroll
+    for (int a = 0; a < W; ++a) {
+        TF u0[3];
+        for (int r = 0; r < 3; ++r) u0[r] = madd(Bc[r][0], pd[0][a], v[r]);
+#pragma unroll
+        for (int b = 0; b < W; ++b) {
+            const TF w01 = w[0][a] * w[1][b];
+            TF u01[3];
+            for (int r = 0; r < 3; ++r) u01[r] = madd(Bc[r][1], pd[1][b], u0[r]);
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const TF wt = w01 * w[2][c];
+                const TF wm = MPM_SEL(wt > TF(1e-25)) ? wt : TF(0);
+                TF* A = acc[a][b * W + c];
+                for (int r = 0; r < 3; ++r) A[r] = madd(wm, madd(Bc[r][2], pd[2][c], u01[r]), A[r]);
+            }
+        }
+    }
+}
+
+template <int D, int SPL, bool FLIP, class TF>
+__device__ __forceinline__ void g2p_musl_body(const Params& P, StepState* S, const PartStateT<TF>& src, const PartStateT<TF>& dst,
+                                              const SortBufs& sb, const GridPtrsT<TF>& g, const StagingT<TF>& stg, uint32_t b) {
+    using G = BG<D, SPL>;
+    using SS = FusedMuslShape<D, SPL, TF>;
+    __shared__ TF s_v[3 * G::NT], s_f[FLIP ? 3 * G::NT : 1];
+    __shared__ ScatterLDS<D, SPL, D, TF> L;
+    BlockRange br;
+    if (!block_range<D>(S, sb, b, br)) return;
+    const double h1 = P.one_over_dx, dt = S->dt;
+    const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
+    const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
+    const bool apic = P.flip_ratio < 0;
+    const bool flip_rt = P.flip_ratio >= -1;
+    load_tile<D, SPL, TF>(s_v, g.v, br.bc, nn);
+    if (FLIP) load_tile<D, SPL, TF>(s_f, g.flip, br.bc, nn);
+    L.coords.load(g.coord, br.bc, nn);
+    const bool ordered = br.cnt <= (uint32_t)kMaxBP;
+    if (ordered) {  // the block's cell order, as the P2G scatter left it (slot start + k holds source particle p[k])
+        for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) L.p[k] = sb.perm[br.start + k];
+        for (int c = threadIdx.x; c <= G::NC; c += blockDim.x) L.cstart[c] = sb.cell_off[(size_t)br.b * (G::NC + 1) + c];
+        if (threadIdx.x == 0) L.cstart[G::NC + 1] = br.cnt;
+    }
+    __syncthreads();
+    const auto cs = L.coords.view(br.bc);
+    uint32_t err = 0;
+    BoundsAcc ba;
+    if (ordered) {
+        TF acc[SS::SPP][SS::NSH][D];
+        zero_acc<D, SPL, D, TF>(acc);
+        const TF aD = (TF)P.apicDinverse;
+        const int lane = threadIdx.x & 63;
+        const uint32_t n_c = L.cstart[lane + 1] - L.cstart[lane];
+        const uint32_t n_max = wave_max_u32(n_c);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        uint32_t kb = 0;
+#pragma unroll 1
+        for (uint32_t j = 0; j < n_max; ++j) {
+            const bool has = j < n_c;
+            const unsigned long long m = __ballot(has);
+            const uint32_t k = kb + (uint32_t)__popcll(m & below);
+            kb += (uint32_t)__popcll(m);
+            if (has) {
+                Stencil<D, SPL> st;
+                double xp[3];
+                TF vs[3], Bp[3][3];
+                if (g2p_one<D, SPL, FLIP, TF>(P, src, dst, L.p[k], br.start + k, org, nn, cs, br.bc, h1, dt, apic, flip_rt, s_v, s_f,
+                                               st, xp, vs, Bp, ba))
+                    musl_terms<D, SPL, TF, SS>(st, xp, vs, Bp, aD, acc);
+                else
+                    err |= MPM_ERR_GRID_BOUNDS;
+            }
+        }
+        // particles whose window base lies outside the block's cells (the P2G scatter already raised the error
+        // bit for them): updated, not scattered
+        for (uint32_t k = L.cstart[G::NC] + threadIdx.x; k < br.cnt; k += blockDim.x) {
+            Stencil<D, SPL> st;
+            double xp[3];
+            TF vs[3], Bp[3][3];
+            (void)g2p_one<D, SPL, FLIP, TF>(P, src, dst, L.p[k], br.start + k, org, nn, cs, br.bc, h1, dt, apic, flip_rt, s_v, s_f,
+                                            st, xp, vs, Bp, ba);
+            err |= MPM_ERR_GRID_BOUNDS;
+        }
+        ordered_tile<D, SPL, D, TF, SS>(L, acc);
+        write_tile<D, SPL, D, TF>(L, stg, br.b);
+    } else {
+        // a block too dense to order: the update in slot order, then the re-scatter with LDS atomics as
+        // k_musl_scatter does for it
+#pragma unroll 1
+        for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) {
+            const uint32_t i = br.start + k;
+            Stencil<D, SPL> st;
+            double xp[3];
+            TF vs[3], Bp[3][3];
+            if (!g2p_one<D, SPL, FLIP, TF>(P, src, dst, sb.perm[i], i, org, nn, cs, br.bc, h1, dt, apic, flip_rt, s_v, s_f, st, xp, vs, Bp,
+                                           ba))
+                err |= MPM_ERR_GRID_BOUNDS;
+        }
+        __syncthreads();  // the block's new velocities and affine matrices are written
+        MuslContrib<D, SPL, TF> cf;
+        cf.src = src; cf.dst = dst;
+        for (int d = 0; d < 3; ++d) { cf.org[d] = org[d]; cf.nn[d] = nn[d]; }
+        cf.cs = cs;
+        cf.h1 = h1; cf.aD = P.apicDinverse; cf.apic = apic;
+        err |= scatter_block_atomic<D, SPL, D, TF>(L, cf, stg, sb, br.b, br.start, br.cnt, br.bc);
+    }
+    reduce_bounds<D>(S, ba);
+    err = wave_or_u32(err);
+    if ((threadIdx.x & 63) == 0 && err) atomicOr(&S->err, err);
+}
+template <int D, int SPL, bool FLIP, class TF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_g2p_musl(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> dst, SortBufs sb, GridPtrsT<TF> g,
+                                                 StagingT<TF> stg) {
+    g2p_musl_body<D, SPL, FLIP, TF>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }
 
 // ----------------------------------------------------------------------------------------------

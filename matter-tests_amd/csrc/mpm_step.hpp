@@ -69,14 +69,23 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     else if (sep) enqueue_p2g<D, SPL, true, false, TF>(c, src, dst, gblocks);
     else if (P.use_mibf) enqueue_p2g<D, SPL, false, true, TF>(c, src, dst, gblocks);
     else enqueue_p2g<D, SPL, false, false, TF>(c, src, dst, gblocks);
-    {
+    bool fused = false;  // G2P and the MUSL re-scatter in one pass (k_g2p_musl)
+    if constexpr (kFusedMusl<D, SPL, TF>) {
+        if (P.use_musl) {
+            LaunchTimer t(c, KS_G2P_MUSL);
+            if (P.flip_ratio >= -1) k_g2p_musl<D, SPL, true, TF><<<c->act_blocks, 64, 0, s>>>(P, c->S, src, dst, c->sb, g, stg);
+            else k_g2p_musl<D, SPL, false, TF><<<c->act_blocks, 64, 0, s>>>(P, c->S, src, dst, c->sb, g, stg);
+            fused = true;
+        }
+    }
+    if (!fused) {
         LaunchTimer t(c, KS_G2P);
         if (P.flip_ratio >= -1) k_g2p<D, SPL, true, TF><<<c->act_blocks, kG2PThreads, 0, s>>>(P, c->S, src, dst, c->sb, g);
         else k_g2p<D, SPL, false, TF><<<c->act_blocks, kG2PThreads, 0, s>>>(P, c->S, src, dst, c->sb, g);
     }
     if (P.use_musl) {
         constexpr int TBM = ScatterLaunch<D, SPL, D, TF, false>::TB;
-        { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->act_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
+        if (!fused) { LaunchTimer t(c, KS_MUSL_SCATTER); k_musl_scatter<D, SPL, TF><<<c->act_blocks, TBM, 0, s>>>(P, c->S, src, dst, c->sb, g, stg); }
         { LaunchTimer t(c, KS_MUSL_UPDATE); k_musl_update<D, SPL, TF><<<gblocks, kBlock, 0, s>>>(P, c->S, c->sb, g, stg); }
     }
     {
