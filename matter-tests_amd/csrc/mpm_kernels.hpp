@@ -721,6 +721,7 @@ __device__ __forceinline__ int tile_index(int t0, int t1, int t2) {
 // window leaves the grid) go last and are reported. L.cstart keeps the cell-major offsets (cell counts).
 // GHOST: the block's periodic copies (perm entries kGhostBit | p, at their copy positions); they are only
 // ordered in LDS for the scatter, and nothing is written back.
+constexpr int kOrderLoads = 8;
 template <int D, int SPL, int NV, class TF, bool GHOST = false, class LT = ScatterLDS<D, SPL, NV, TF>>
 __device__ __forceinline__ uint32_t order_block(LT& L, const Params& P, const StepState* S, const PartStateT<TF>& src,
                                 const PartStateT<TF>& dst, const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b, uint32_t start,
@@ -733,23 +734,43 @@ __device__ __forceinline__ uint32_t order_block(LT& L, const Params& P, const St
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
     uint32_t err = 0;
-    for (uint32_t k = tid; k < cnt; k += blockDim.x) {
-        const uint32_t p = sb.perm[start + k];
-        double xp[3] = {0, 0, 0};
-        for (int d = 0; d < D; ++d) xp[d] = src.x[d][p & ~kGhostBit];
-        if (GHOST) { double gx = xp[0]; (void)periodic_copy(P.L[0], 4.0 * P.dx, xp[0], gx); xp[0] = gx; }
-        int wb[3];
-        int cell = G::NC;
-        if (window_base<D, SPL>(wb, xp, org, nn, L.coords.view(bc), h1, true)) {
-            int l[3] = {0, 0, 0};
-            bool in = true;
-            for (int d = 0; d < D; ++d) { l[d] = wb[d] - G::E * bc[d]; in = in && l[d] >= 0 && l[d] < G::E; }
-            if (in) cell = D == 3 ? (l[0] * G::E + l[1]) * G::E + l[2] : l[0] * G::E + l[1];
+    // A thread takes kOrderLoads of the block's particles at a time and issues all their slot and position
+    // loads before it uses any: two memory latencies per group instead of two per particle (a one-wavefront
+    // workgroup orders about 8 particles per thread).
+    for (uint32_t k0 = tid; k0 < cnt; k0 += kOrderLoads * blockDim.x) {
+        uint32_t pk[kOrderLoads];
+        double xk[kOrderLoads][3];
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
+            pk[u] = k < cnt ? sb.perm[start + k] : 0u;
         }
-        if (cell == G::NC) err |= MPM_ERR_GRID_BOUNDS;
-        L.u.sort.tmp[k] = p;
-        L.u.sort.cell[k] = (uint8_t)cell;
-        atomicAdd(&L.u.sort.ccount[cell], 1u);
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xk[u][d] = (d < D && k < cnt) ? src.x[d][pk[u] & ~kGhostBit] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
+            if (k >= cnt) break;
+            const uint32_t p = pk[u];
+            double xp[3] = {xk[u][0], xk[u][1], xk[u][2]};
+            if (GHOST) { double gx = xp[0]; (void)periodic_copy(P.L[0], 4.0 * P.dx, xp[0], gx); xp[0] = gx; }
+            int wb[3];
+            int cell = G::NC;
+            if (window_base<D, SPL>(wb, xp, org, nn, L.coords.view(bc), h1, true)) {
+                int l[3] = {0, 0, 0};
+                bool in = true;
+                for (int d = 0; d < D; ++d) { l[d] = wb[d] - G::E * bc[d]; in = in && l[d] >= 0 && l[d] < G::E; }
+                if (in) cell = D == 3 ? (l[0] * G::E + l[1]) * G::E + l[2] : l[0] * G::E + l[1];
+            }
+            if (cell == G::NC) err |= MPM_ERR_GRID_BOUNDS;
+            L.u.sort.tmp[k] = p;
+            L.u.sort.cell[k] = (uint8_t)cell;
+            atomicAdd(&L.u.sort.ccount[cell], 1u);
+        }
     }
     __syncthreads();
     if (tid < 64) {  // exclusive scan of the 64 cell buckets by one wavefront; the outside bucket goes last
@@ -815,10 +836,22 @@ __device__ __forceinline__ uint32_t order_block(LT& L, const Params& P, const St
     for (uint32_t k = L.cstart[G::NC] + tid; k < cnt; k += blockDim.x) L.p[k] = L.u.sort.tmp[k];  // outside bucket last
     __syncthreads();
     if (GHOST) return err;
-    for (uint32_t k = tid; k < cnt; k += blockDim.x) {
-        const uint32_t p = L.p[k];
-        sb.perm[start + k] = p;
-        dst.id[start + k] = src.id[p];
+    // the ids are loaded kOrderLoads at a time before they are stored (a store to dst.id could alias src.id for
+    // the compiler, which would otherwise wait for each load in turn)
+    for (uint32_t k0 = tid; k0 < cnt; k0 += kOrderLoads * blockDim.x) {
+        uint32_t idk[kOrderLoads];
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
+            idk[u] = k < cnt ? src.id[L.p[k]] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
+            if (k >= cnt) break;
+            sb.perm[start + k] = L.p[k];
+            dst.id[start + k] = idk[u];
+        }
     }
     for (int c = tid; c <= G::NC; c += blockDim.x) sb.cell_off[(size_t)b * (G::NC + 1) + c] = (uint16_t)L.cstart[c];
     return err;
@@ -1193,8 +1226,9 @@ struct P2GContrib {
         // reference's 1e-25 weight cut (smallest per-axis weights multiplied, with a factor 10 of margin for
         // rounding): with w = w0 w1 w2 and grad w = h1 (dw0 w1 w2, w0 dw1 w2, w0 w1 dw2), every value is
         // w1 w2 A0[a] + w0 w2 C1[b] + w0 w1 C2[c], where A0 carries the velocity, the first-axis affine term
-        // and the first-axis force term, C1 and C2 the other axes' affine and force terms. About 10
-        // operations per node instead of about 24.
+        // and the first-axis force term, C1 and C2 the other axes' affine and force terms. In 3D a window row
+        // (a, b) is summed as w2 X + w0 w1 C2[c] with X = w1 A0 + w0 C1: 7 operations per node (mass and the
+        // folded momentum) instead of about 24.
         TF mw0 = TF(1), mw1 = ss.w1[0], mw2 = TF(1);
 #pragma unroll
         for (int s = 0; s < SS::SPP; ++s) if (a0 + s < W) mw0 = s == 0 ? ss.w0[0] : fmin(mw0, ss.w0[s]);
@@ -1294,17 +1328,27 @@ struct P2GContrib {
                 }
                 const TF w01 = ss.w0[s] * ss.w1[o1];
                 if (D == 3) {
+                    // row (a, b): w1 w2 A0 + w0 w2 C1 = w2 X with X = w1 A0 + w0 C1, so that a node adds
+                    // w2 X + w0 w1 C2[c] (two multiply-adds per value instead of three and two products).
+                    // Rows are scheduled one after another: hoisting every row's X ahead of the sums would
+                    // keep them all live and push the kernel past 256 registers (one wavefront per SIMD).
+                    __builtin_amdgcn_sched_barrier(0);
+                    TF X[NC][3];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+#pragma unroll
+                        for (int r = 0; r < D; ++r) X[c][r] = madd(ss.w1[o1], A0[c][r], ss.w0[s] * C1[c][r]);
+                    const TF w01m = MIBF ? w01 * muI : TF(0);
 #pragma unroll
                     for (int o2 = 0; o2 < W; ++o2) {
-                        const TF w12 = ss.w1[o1] * ss.w2[o2], w02 = ss.w0[s] * ss.w2[o2];
                         TF* a = acc[s][o1 * W + o2];
-                        a[0] += w01 * ss.w2[o2];
+                        a[0] = madd(w01, ss.w2[o2], a[0]);
 #pragma unroll
                         for (int c = 0; c < NC; ++c)
 #pragma unroll
                             for (int r = 0; r < D; ++r)
-                                a[1 + c * D + r] = madd(w12, A0[c][r], madd(w02, C1[c][r], madd(w01, C2[o2][c][r], a[1 + c * D + r])));
-                        if (MIBF) a[NV - 1] = madd(w01 * ss.w2[o2], muI, a[NV - 1]);
+                                a[1 + c * D + r] = madd(w01, C2[o2][c][r], madd(ss.w2[o2], X[c][r], a[1 + c * D + r]));
+                        if (MIBF) a[NV - 1] = madd(w01m, ss.w2[o2], a[NV - 1]);
                     }
                 } else {
                     TF* a = acc[s][o1];
@@ -1450,6 +1494,36 @@ struct P2GContrib {
     }
 };
 
+// The MUSL re-scatter terms of window slice a of one particle (musl.cpp:41-56), added to acc[b W + c][r]:
+// the weight w0 w1[b] w2[c], dropped at or under the reference's 1e-25 cut, times
+// v_r + Bc_r0 pd0 + Bc_r1 pd1[b] + Bc_r2 pd2[c] (Bc = B D^-1, pd the node offsets x_i - x_p). The fused G2P +
+// re-scatter kernel and k_musl_scatter both take this function, so their tiles are equal bit for bit. (A
+// row-factored form, w2[c] X[b] + w0 w1[b] Z[c], took 20 fewer instructions per node but more registers: the
+// fused kernel spilled more and ran 278 -> 298 us on C2.)
+template <int W, class TF>
+__device__ __forceinline__ void musl_slice(TF w0, TF pd0, const TF (&w1)[W], const TF (&pd1)[W], const TF (&w2)[W], const TF (&pd2)[W],
+                                           const TF (&v)[3], const TF (&Bc)[3][3], TF (&acc)[W * W][3]) {
+    TF u0[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) u0[r] = madd(Bc[r][0], pd0, v[r]);
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+        const TF w01 = w0 * w1[b];
+        TF u01[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) u01[r] = madd(Bc[r][1], pd1[b], u0[r]);
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            const TF wt = w01 * w2[c];
+            const TF wm = MPM_SEL(wt > TF(1e-25)) ? wt : TF(0);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) acc[b * W + c][r] = madd(wm, madd(Bc[r][2], pd2[c], u01[r]), acc[b * W + c][r]);
+        }
+    }
+}
+// Slices through musl_slice: the 3D split form, where a wavefront's partial sums cover whole slices.
+template <int D, class SS, class TF> constexpr bool kMuslSlices = D == 3 && sizeof(TF) == 4 && SS::SPLIT == 1 && SS::OBN == SS::G::W;
+
 // MUSL re-scatter contributions (musl.cpp:28-85): sum w (v_new + B_new (x_i - x_p) D^-1) over
 // weight > 1e-25, with the position of the step's start (source layout, slot p) and the new velocity
 // and affine matrix (destination layout, sorted slot i), arranged separably as in P2GContrib.
@@ -1476,6 +1550,14 @@ struct MuslContrib {
         const TF aDt = (TF)aD;
         TF Bc[3][3];
         for (int r = 0; r < D; ++r) for (int k = 0; k < D; ++k) Bc[r][k] = q.B[r + k * D] * aDt;
+        if constexpr (kMuslSlices<D, SS, TF>) {
+#pragma unroll
+            for (int s = 0; s < SS::SPP; ++s) {
+                if (a0 + s >= W) break;
+                musl_slice<W, TF>(ss.w0[s], ss.pd0[s], ss.w1, ss.pd1, ss.w2, ss.pd2, q.v, Bc, acc[s]);
+            }
+            return ok;
+        }
 #pragma unroll
         for (int s = 0; s < SS::SPP; ++s) {
             if (a0 + s >= W) break;
@@ -1602,7 +1684,10 @@ template <int D, int SPL, int NV, class TF, bool PBC> struct ScatterLaunch {
     // 128 and other scatters are not bounded (1). Only the single-scene kernel takes the bound: the batched one spills
     // more under it (420 bytes), and with it and the deformation bound the served stream of inclined flows and
     // collapses ran 1.74e9 -> 1.49e9 particle-updates/s (1.70e9 with both bounds on the single-scene kernels only).
-    static constexpr int WPE = kStagedScatter<D, SPL, TF, PBC> && NV > D + 1 ? 4 : 1;
+    // The one-wavefront quadratic 3D scatter keeps at least 2 (at most 256 VGPRs): its 108 registers of partial
+    // sums, the next rank's particle and the row factors otherwise reach 258 with three accumulation registers
+    // kept in AGPRs, and one wavefront per SIMD.
+    static constexpr int WPE = kStagedScatter<D, SPL, TF, PBC> && NV > D + 1 ? 4 : (D == 3 && SPL == 2 ? 2 : 1);
 };
 
 template <int D, int SPL, int NV, class TF, int NR> struct StagedLDS {
@@ -2467,12 +2552,13 @@ constexpr int kG2PThreads = 64;
 // G2P, velocity and position update of source particle p into slot i of the destination layout, from the block's
 // LDS tiles: false if its window leaves the tile. st, xp, vs and Bp return the stencil, the position of the step's
 // start, the new velocity and the new affine matrix as stored (B zero without APIC).
-template <int D, int SPL, bool FLIP, class TF, class CS>
+// XLOADED: xp already holds the particle's position (the caller loaded it ahead).
+template <int D, int SPL, bool FLIP, class TF, class CS, bool XLOADED = false>
 __device__ __forceinline__ bool g2p_one(const Params& P, const PartStateT<TF>& src, const PartStateT<TF>& dst, uint32_t p, uint32_t i,
                                         const double* org, const uint32_t* nn, const CS& cs, const int* bc, double h1, double dt,
                                         bool apic, bool flip_rt, const TF* s_v, const TF* s_f, Stencil<D, SPL>& st,
                                         double (&xp)[3], TF (&vs)[3], TF (&Bp)[3][3], BoundsAcc& ba) {
-    for (int d = 0; d < 3; ++d) xp[d] = d < D ? src.x[d][p] : 0.0;
+    if (!XLOADED) for (int d = 0; d < 3; ++d) xp[d] = d < D ? src.x[d][p] : 0.0;
     int tl[3];
     if (!gather_stencil<D, SPL>(st, tl, xp, org, nn, cs, h1, bc)) return false;
     TF pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, gv[3][3];
@@ -2549,8 +2635,9 @@ template <int D, int SPL, class TF> constexpr bool kFusedMusl = D == 3 && SPL ==
 template <int D, int SPL, class TF> using FusedMuslShape = ScatterShape<D, SPL, D, TF, 108>;
 
 // The re-scatter terms of one particle (MuslContrib::add on the stencil's weights): acc[a][b W + c][r] +=
-// w_abc (v_r + (B D^-1)_r0 (x_a - x_p) + (B D^-1)_r1 (y_b - y_p) + (B D^-1)_r2 (z_c - z_p)) in that order of
-// operations, with weights at or under the 1e-25 cut dropped (g2p.cpp:44, musl.cpp:48).
+// w_abc (v_r + (B D^-1)_r0 (x_a - x_p) + (B D^-1)_r1 (y_b - y_p) + (B D^-1)_r2 (z_c - z_p)), slice by slice with
+// musl_slice, as k_musl_scatter sums them, with weights at or under the 1e-25 cut dropped (g2p.cpp:44,
+// musl.cpp:48).
 template <int D, int SPL, class TF, class SS>
 __device__ __forceinline__ void musl_terms(const Stencil<D, SPL>& st, const double* xp, const TF (&v)[3], const TF (&B)[3][3], TF aD,
                                            TF (&acc)[SS::SPP][SS::NSH][D]) {
@@ -2564,23 +2651,7 @@ __device__ __forceinline__ void musl_terms(const Stencil<D, SPL>& st, const doub
     TF Bc[3][3];
     for (int r = 0; r < 3; ++r) for (int k = 0; k < 3; ++k) Bc[r][k] = B[r][k] * aD;
 #pragma unroll
-    for (int a = 0; a < W; ++a) {
-        TF u0[3];
-        for (int r = 0; r < 3; ++r) u0[r] = madd(Bc[r][0], pd[0][a], v[r]);
-#pragma unroll
-        for (int b = 0; b < W; ++b) {
-            const TF w01 = w[0][a] * w[1][b];
-            TF u01[3];
-            for (int r = 0; r < 3; ++r) u01[r] = madd(Bc[r][1], pd[1][b], u0[r]);
-#pragma unroll
-            for (int c = 0; c < W; ++c) {
-                const TF wt = w01 * w[2][c];
-                const TF wm = MPM_SEL(wt > TF(1e-25)) ? wt : TF(0);
-                TF* A = acc[a][b * W + c];
-                for (int r = 0; r < 3; ++r) A[r] = madd(wm, madd(Bc[r][2], pd[2][c], u01[r]), A[r]);
-            }
-        }
-    }
+    for (int a = 0; a < W; ++a) musl_slice<W, TF>(w[0][a], pd[0][a], w[1], pd[1], w[2], pd[2], v, Bc, acc[a]);
 }
 
 template <int D, int SPL, bool FLIP, class TF>
@@ -2618,19 +2689,31 @@ __device__ __forceinline__ void g2p_musl_body(const Params& P, StepState* S, con
         const uint32_t n_c = L.cstart[lane + 1] - L.cstart[lane];
         const uint32_t n_max = wave_max_u32(n_c);
         const unsigned long long below = (1ull << lane) - 1ull;
-        uint32_t kb = 0;
+        // the next rank's position is loaded while this rank's particle is updated and scattered
+        uint32_t kb = 0, kn = 0;
+        bool hasn = 0 < n_c;
+        double xn[3] = {0, 0, 0};
+        {
+            const unsigned long long m = __ballot(hasn);
+            kn = kb + (uint32_t)__popcll(m & below);
+            kb += (uint32_t)__popcll(m);
+            if (hasn) for (int d = 0; d < 3; ++d) xn[d] = src.x[d][L.p[kn]];
+        }
 #pragma unroll 1
         for (uint32_t j = 0; j < n_max; ++j) {
-            const bool has = j < n_c;
-            const unsigned long long m = __ballot(has);
-            const uint32_t k = kb + (uint32_t)__popcll(m & below);
+            const bool has = hasn;
+            const uint32_t k = kn;
+            double xp[3] = {xn[0], xn[1], xn[2]};
+            hasn = j + 1 < n_c;
+            const unsigned long long m = __ballot(hasn);
+            kn = kb + (uint32_t)__popcll(m & below);
             kb += (uint32_t)__popcll(m);
+            if (hasn) for (int d = 0; d < 3; ++d) xn[d] = src.x[d][L.p[kn]];
             if (has) {
                 Stencil<D, SPL> st;
-                double xp[3];
                 TF vs[3], Bp[3][3];
-                if (g2p_one<D, SPL, FLIP, TF>(P, src, dst, L.p[k], br.start + k, org, nn, cs, br.bc, h1, dt, apic, flip_rt, s_v, s_f,
-                                               st, xp, vs, Bp, ba))
+                if (g2p_one<D, SPL, FLIP, TF, decltype(cs), true>(P, src, dst, L.p[k], br.start + k, org, nn, cs, br.bc, h1, dt, apic,
+                                                                  flip_rt, s_v, s_f, st, xp, vs, Bp, ba))
                     musl_terms<D, SPL, TF, SS>(st, xp, vs, Bp, aD, acc);
                 else
                     err |= MPM_ERR_GRID_BOUNDS;
