@@ -677,7 +677,22 @@ template <int D, int SPL, int NV, class TF, int REGS_ = 0> struct ScatterShape {
 template <int D, int SPL> struct BlockCoordLDS {
     static constexpr int LC = BG<D, SPL>::E + 4;
     double c[3][LC];
+    // ONEPASS: one entry per thread over all axes (a workgroup has at least 64 threads), so that the D axes cost
+    // one memory latency instead of D (the fused G2P + re-scatter kernel: 258 -> 245 us on C2). The P2G scatter
+    // keeps the loop per axis: the one pass pushed its 3D quadratic split form past 256 registers (456 bytes of
+    // spill).
+    template <bool ONEPASS = false>
     __device__ __forceinline__ void load(double* const* coord, const int* bc, const uint32_t* n) {
+        if constexpr (ONEPASS) {
+            static_assert(D * LC <= 64, "one pass of a wavefront");
+            const int t = threadIdx.x;
+            if (t < D * LC) {
+                const int d = t / LC, i = t - d * LC;
+                const int gi = BG<D, SPL>::E * bc[d] - 1 + i;
+                c[d][i] = (gi >= 0 && gi < (int)n[d]) ? coord[d][gi] : 0.0;
+            }
+            return;
+        }
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const int e = BG<D, SPL>::E * bc[d] - 1, nd = (int)n[d];
@@ -722,6 +737,23 @@ __device__ __forceinline__ int tile_index(int t0, int t1, int t2) {
 // GHOST: the block's periodic copies (perm entries kGhostBit | p, at their copy positions); they are only
 // ordered in LDS for the scatter, and nothing is written back.
 constexpr int kOrderLoads = 8;
+// dst[k] = src[k] for k < n, global memory to LDS, with kOrderLoads loads in flight per thread.
+template <class T, class S>
+__device__ __forceinline__ void copy_to_lds(T* dst, const S* src, uint32_t n) {
+    for (uint32_t k0 = threadIdx.x; k0 < n; k0 += kOrderLoads * blockDim.x) {
+        S v[kOrderLoads];
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
+            v[u] = k < n ? src[k] : S(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
+            if (k < n) dst[k] = (T)v[u];
+        }
+    }
+}
 template <int D, int SPL, int NV, class TF, bool GHOST = false, class LT = ScatterLDS<D, SPL, NV, TF>>
 __device__ __forceinline__ uint32_t order_block(LT& L, const Params& P, const StepState* S, const PartStateT<TF>& src,
                                 const PartStateT<TF>& dst, const SortBufs& sb, const GridPtrsT<TF>& g, uint32_t b, uint32_t start,
@@ -2025,8 +2057,8 @@ __device__ __forceinline__ void musl_staged_body(const Params& P, StepState* S, 
     if (cnt > kMaxBP) {
         err |= staged_block_atomic<D, SPL, D, TF>(L, cf, stg, sb, b, start, cnt, bc);
     } else {
-        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) L.p[k] = sb.perm[start + k];
-        for (int c = threadIdx.x; c <= G::NC; c += blockDim.x) L.cstart[c] = sb.cell_off[(size_t)b * (G::NC + 1) + c];
+        copy_to_lds(L.p, sb.perm + start, cnt);
+        copy_to_lds(L.cstart, sb.cell_off + (size_t)b * (G::NC + 1), (uint32_t)G::NC + 1);
         if (threadIdx.x == 0) L.cstart[G::NC + 1] = cnt;
         __syncthreads();
         staged_scatter_block<D, SPL, D, TF>(L, cf, stg, b, start, cnt, bc);
@@ -2068,8 +2100,8 @@ __device__ __forceinline__ void musl_plain_body(const Params& P, StepState* S, c
     if (cnt > kMaxBP) {
         err |= scatter_block_atomic<D, SPL, D, TF>(L, cf, stg, sb, b, start, cnt, bc);
     } else {
-        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) L.p[k] = sb.perm[start + k];
-        for (int c = threadIdx.x; c <= G::NC; c += blockDim.x) L.cstart[c] = sb.cell_off[(size_t)b * (G::NC + 1) + c];
+        copy_to_lds(L.p, sb.perm + start, cnt);
+        copy_to_lds(L.cstart, sb.cell_off + (size_t)b * (G::NC + 1), (uint32_t)G::NC + 1);
         if (threadIdx.x == 0) L.cstart[G::NC + 1] = cnt;
         __syncthreads();
         scatter_block<D, SPL, D, TF>(L, cf, stg, b, start);
@@ -2302,23 +2334,38 @@ template <int D> struct TileStride { static constexpr int v = D == 3 ? 4 : 2; };
 
 // Copies the D node arrays src[0..D) over the block's tile into LDS; nodes outside the grid read as zero
 // (no particle window reaches them).
+// A thread issues the loads of all its nodes (at most kTileLoads) before it stores any, so that the tile costs
+// one memory latency rather than one per node.
+constexpr int kTileLoads = 4;
 template <int D, int SPL, class TF, bool NM = false>
 __device__ __forceinline__ void load_tile(TF* s, TF* const* src, const int* bc, const uint32_t* n) {
     using G = BG<D, SPL>;
     constexpr int ST = TileStride<D>::v;
-    for (int q = threadIdx.x; q < G::NT; q += blockDim.x) {
-        int t[3] = {0, 0, 0};
-        if (D == 3) { t[2] = q % G::T; const int r = q / G::T; t[1] = r % G::T; t[0] = r / G::T; }
-        else { t[1] = q % G::T; t[0] = q / G::T; }
-        bool in = true;
-        int gx[3] = {0, 0, 0};
-        for (int d = 0; d < D; ++d) { gx[d] = G::E * bc[d] + t[d]; in = in && gx[d] < (int)n[d]; }
-        const uint32_t node = in ? node_index<D>(gx[0], gx[1], gx[2], n) : 0u;
-        if constexpr (NM) {
-            for (int c = 0; c < D; ++c) s[q * ST + c] = in ? src[c][node] : TF(0);
-            for (int c = D; c < ST; ++c) s[q * ST + c] = TF(0);
-        } else {
-            for (int c = 0; c < D; ++c) s[c * G::NT + q] = in ? src[c][node] : TF(0);
+    for (int q0 = threadIdx.x; q0 < G::NT; q0 += kTileLoads * blockDim.x) {
+        TF val[kTileLoads][3];
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) {
+            const int q = q0 + u * (int)blockDim.x;
+            int t[3] = {0, 0, 0};
+            if (D == 3) { t[2] = q % G::T; const int r = q / G::T; t[1] = r % G::T; t[0] = r / G::T; }
+            else { t[1] = q % G::T; t[0] = q / G::T; }
+            bool in = q < G::NT;
+            int gx[3] = {0, 0, 0};
+            for (int d = 0; d < D; ++d) { gx[d] = G::E * bc[d] + t[d]; in = in && gx[d] < (int)n[d]; }
+            const uint32_t node = in ? node_index<D>(gx[0], gx[1], gx[2], n) : 0u;
+#pragma unroll
+            for (int c = 0; c < D; ++c) val[u][c] = in ? src[c][node] : TF(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) {
+            const int q = q0 + u * (int)blockDim.x;
+            if (q >= G::NT) break;
+            if constexpr (NM) {
+                for (int c = 0; c < D; ++c) s[q * ST + c] = val[u][c];
+                for (int c = D; c < ST; ++c) s[q * ST + c] = TF(0);
+            } else {
+                for (int c = 0; c < D; ++c) s[c * G::NT + q] = val[u][c];
+            }
         }
     }
 }
@@ -2598,7 +2645,7 @@ __device__ __forceinline__ void g2p_body(const Params& P, StepState* S, const Pa
     const bool flip_rt = P.flip_ratio >= -1;
     load_tile<D, SPL, TF>(s_v, g.v, br.bc, nn);
     if (useflip) load_tile<D, SPL, TF>(s_f, g.flip, br.bc, nn);
-    s_c.load(g.coord, br.bc, nn);
+    s_c.template load<true>(g.coord, br.bc, nn);
     __syncthreads();
     uint32_t err = 0;
     BoundsAcc ba;
@@ -2670,11 +2717,11 @@ __device__ __forceinline__ void g2p_musl_body(const Params& P, StepState* S, con
     const bool flip_rt = P.flip_ratio >= -1;
     load_tile<D, SPL, TF>(s_v, g.v, br.bc, nn);
     if (FLIP) load_tile<D, SPL, TF>(s_f, g.flip, br.bc, nn);
-    L.coords.load(g.coord, br.bc, nn);
+    L.coords.template load<true>(g.coord, br.bc, nn);
     const bool ordered = br.cnt <= (uint32_t)kMaxBP;
     if (ordered) {  // the block's cell order, as the P2G scatter left it (slot start + k holds source particle p[k])
-        for (uint32_t k = threadIdx.x; k < br.cnt; k += blockDim.x) L.p[k] = sb.perm[br.start + k];
-        for (int c = threadIdx.x; c <= G::NC; c += blockDim.x) L.cstart[c] = sb.cell_off[(size_t)br.b * (G::NC + 1) + c];
+        copy_to_lds(L.p, sb.perm + br.start, br.cnt);
+        copy_to_lds(L.cstart, sb.cell_off + (size_t)br.b * (G::NC + 1), (uint32_t)G::NC + 1);
         if (threadIdx.x == 0) L.cstart[G::NC + 1] = br.cnt;
     }
     __syncthreads();
@@ -2762,6 +2809,66 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     g2p_musl_body<D, SPL, FLIP, TF>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }
 
+// The node-major velocity tiles of blocks blo .. blo + nt - 1 (at most NTL; tile t at s_v + t ST NT) and their node
+// coordinates, for a workgroup of at least NTL * D * LC threads. Every thread issues its coordinate load and the
+// loads of all its tile nodes (kTileLoads at a time) before it stores any, so that the tiles cost one memory
+// latency instead of several per tile. Blocks without particles inside the span are loaded too; nodes outside
+// the grid are zero.
+template <int D, int SPL, class TF, int NTL>
+__device__ __forceinline__ void load_slot_tiles(TF* s_v, BlockCoordLDS<D, SPL>* s_c, uint32_t blo,
+                                                uint32_t nt, const StepState* S, TF* const* src, double* const* coord,
+                                                const uint32_t* n) {
+    using G = BG<D, SPL>;
+    constexpr int ST = TileStride<D>::v, LC = BlockCoordLDS<D, SPL>::LC;
+    static_assert(NTL * D * LC <= kBlock, "one coordinate entry per thread of a kBlock workgroup");
+    const uint32_t tid = threadIdx.x;
+    // coordinates: entry (t, d, i) per thread
+    const bool hc = tid < nt * (uint32_t)(D * LC);
+    int ct = 0, cd = 0, ci = 0;
+    double cv = 0.0;
+    if (hc) {
+        ct = (int)(tid / (D * LC));
+        const int r = (int)tid - ct * D * LC;
+        cd = r / LC; ci = r - cd * LC;
+        int bc[3];
+        block_coords<D>(blo + (uint32_t)ct, S, bc);
+        const int gi = G::E * bc[cd] - 1 + ci;
+        cv = (gi >= 0 && gi < (int)n[cd]) ? coord[cd][gi] : 0.0;
+    }
+    const uint32_t total = nt * (uint32_t)G::NT;
+    for (uint32_t k0 = tid; k0 < total; k0 += kTileLoads * blockDim.x) {
+        TF val[kTileLoads][3];
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) {
+            const uint32_t e = k0 + (uint32_t)u * blockDim.x;
+            const uint32_t t = e / (uint32_t)G::NT;
+            const int q = (int)(e - t * (uint32_t)G::NT);
+            int bc[3] = {0, 0, 0};
+            if (e < total) block_coords<D>(blo + t, S, bc);
+            int tq[3] = {0, 0, 0};
+            if (D == 3) { tq[2] = q % G::T; const int r = q / G::T; tq[1] = r % G::T; tq[0] = r / G::T; }
+            else { tq[1] = q % G::T; tq[0] = q / G::T; }
+            bool in = e < total;
+            int gx[3] = {0, 0, 0};
+            for (int d = 0; d < D; ++d) { gx[d] = G::E * bc[d] + tq[d]; in = in && gx[d] < (int)n[d]; }
+            const uint32_t node = in ? node_index<D>(gx[0], gx[1], gx[2], n) : 0u;
+#pragma unroll
+            for (int c = 0; c < D; ++c) val[u][c] = in ? src[c][node] : TF(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) {
+            const uint32_t e = k0 + (uint32_t)u * blockDim.x;
+            if (e >= total) break;
+            const uint32_t t = e / (uint32_t)G::NT;
+            const int q = (int)(e - t * (uint32_t)G::NT);
+            TF* row = s_v + (size_t)t * (ST * G::NT) + q * ST;
+            for (int c = 0; c < D; ++c) row[c] = val[u][c];
+            for (int c = D; c < ST; ++c) row[c] = TF(0);
+        }
+    }
+    if (hc) s_c[ct].c[cd][ci] = cv;
+}
+
 // ----------------------------------------------------------------------------------------------
 // k_defgrad: deformationUpdate (deformation_update.cpp:16-46) with the return map, on the velocity of the
 // step's grid update (USL) or of the MUSL re-scatter. One thread per sorted particle slot (the return map is
@@ -2808,13 +2915,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
     }
     __syncthreads();
     const uint32_t blo = s_lo, nt = min((uint32_t)NTL, s_hi - s_lo + 1);
-    for (uint32_t t = 0; t < nt; ++t) {
-        if (sb.count[blo + t] == 0) continue;
-        int bc[3];
-        block_coords<D>(blo + t, S, bc);
-        load_tile<D, SPL, TF, true>(s_v[t], g.v, bc, nn);
-        s_c[t].load(g.coord, bc, nn);
-    }
+    load_slot_tiles<D, SPL, TF, NTL>(&s_v[0][0], s_c, blo, nt, S, g.v, g.coord, nn);
     __syncthreads();
     uint32_t err = 0, pcount = 0;
     if (active) {
