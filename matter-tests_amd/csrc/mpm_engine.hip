@@ -89,8 +89,10 @@ size_t layout_bytes(int D, bool split) {
 int alloc_particles(mpm_context* c, uint32_t n) {
     const int D = c->cfg.dim;
     const size_t need = 2 * layout_bytes(D, c->split) * ((size_t)n + kArrayPad) + 64;
-    // id x2, key, perm; with periodic boundaries key and perm also hold one periodic copy per particle
-    const size_t ineed = (size_t)(c->cfg.pbc ? 6 : 4) * n;
+    // id x2, key, perm, then the byte arrays cellk and cells; with periodic boundaries key, perm, cellk and
+    // cells also hold one periodic copy per particle
+    const size_t ks = (size_t)(c->cfg.pbc ? 2 : 1) * n;
+    const size_t ineed = (size_t)2 * n + 2 * ks + (2 * ks + 3) / 4;
     if (!(c->pbuf && c->pbuf_bytes >= need)) {
         if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
         if (hipMalloc(&c->pbuf, need) != hipSuccess) { c->pbuf = nullptr; c->pbuf_bytes = 0; return MPM_ERR_OUT_OF_MEMORY; }
@@ -125,6 +127,8 @@ void layout_particles(mpm_context* c, uint32_t n) {
     const size_t ks = c->cfg.pbc ? (size_t)2 * n : (size_t)n;
     c->sb.key = c->ibuf + (size_t)2 * n;
     c->sb.perm = c->sb.key + ks;
+    c->sb.cellk = (uint8_t*)(c->sb.perm + ks);
+    c->sb.cells = c->sb.cellk + ks;
     c->sb.ghosts = c->cfg.pbc ? 1u : 0u;
 }
 

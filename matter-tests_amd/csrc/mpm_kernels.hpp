@@ -61,6 +61,8 @@ struct SortBufs {
     uint32_t* offset;    // exclusive prefix sum of count
     uint32_t* cursor;    // scatter cursors
     uint16_t* cell_off;  // per block: first sorted slot (relative to offset) of each dual cell, 65 entries
+    uint8_t* cellk;      // dual cell within its block of every entry of key (BG::NC: the window is outside)
+    uint8_t* cells;      // the same per sorted slot (k_bin_scatter), so that ordering a block needs no positions
     // Periodic boundaries (ghosts = 1): particle p near x = 0 or x = Lx also has a periodic copy, binned
     // under key[n + p] = nblocks + its block (kNoGhost: none). count/offset/cursor then have 2 nblocks
     // entries, and the copies' slots follow the particles' (perm entries kGhostBit | p).
@@ -503,19 +505,35 @@ __device__ __forceinline__ void block_coords(uint32_t b, const StepState* S, int
 }
 
 // Block of a particle from the first node of its non-zero window (P2G's clamped base). A particle whose
-// window leaves the grid goes to block 0, where k_p2g reports it.
+// window leaves the grid goes to block 0, where k_p2g reports it. `cell` receives the window base's dual
+// cell within the block (BG::NC when the window leaves the grid). A window that fits the grid starts at
+// most at node n - 3 < E nb, so its block needs no clamping and the cell is the base modulo the block edge;
+// with these coordinates order_block's own window base (k_p2g) is the same.
 template <int D, int SPL, class TF>
-__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrsT<TF>& g, double h1) {
+__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrsT<TF>& g, double h1, uint32_t& cell) {
     const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
+    constexpr uint32_t E = BlockEdge<D>::v;
     int wb[3];
+    cell = BG<D, SPL>::NC;
     if (!window_base<D, SPL>(wb, xp, org, nn, GlobalCoords{g.coord, nn}, h1, true)) return 0u;
     uint32_t c[3] = {0, 0, 0};
+    bool in = true;
     for (int d = 0; d < D; ++d) {
-        const uint32_t b = (uint32_t)wb[d] / BlockEdge<D>::v;
+        const uint32_t b = (uint32_t)wb[d] / E;
+        in = in && b < S->nb[d];
         c[d] = b < S->nb[d] ? b : S->nb[d] - 1;
     }
+    if (in) {
+        const uint32_t l0 = (uint32_t)wb[0] % E, l1 = (uint32_t)wb[1] % E, l2 = D == 3 ? (uint32_t)wb[2] % E : 0u;
+        cell = D == 3 ? (l0 * E + l1) * E + l2 : l0 * E + l1;
+    }
     return D == 3 ? (c[0] * S->nb[1] + c[1]) * S->nb[2] + c[2] : c[0] * S->nb[1] + c[1];
+}
+template <int D, int SPL, class TF>
+__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrsT<TF>& g, double h1) {
+    uint32_t cell;
+    return block_key<D, SPL, TF>(xp, S, g, h1, cell);
 }
 
 // Periodic copy of a particle at x0 (PBCAddParticles, pbc.cpp:37-83): within 4 dx of x = Lx (strictly
@@ -540,19 +558,21 @@ __device__ __forceinline__ void bin_count_body(const Params& P, const StepState*
     double xp[3] = {0, 0, 0};
     if (active) {
         for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
-        key = block_key<D, SPL, TF>(xp, S, g, P.one_over_dx);
+        uint32_t cell;
+        key = block_key<D, SPL, TF>(xp, S, g, P.one_over_dx, cell);
         sb.key[p] = key;
+        sb.cellk[p] = (uint8_t)cell;
     }
     (void)wave_claim(sb.count, key, active);
     if (sb.ghosts) {
         double gx = 0.0;
         const bool has = active && periodic_copy(P.L[0], 4.0 * P.dx, xp[0], gx);
-        uint32_t gkey = 0;
+        uint32_t gkey = 0, gcell = BG<D, SPL>::NC;
         if (has) {
             xp[0] = gx;
-            gkey = S->nblocks + block_key<D, SPL, TF>(xp, S, g, P.one_over_dx);
+            gkey = S->nblocks + block_key<D, SPL, TF>(xp, S, g, P.one_over_dx, gcell);
         }
-        if (active) sb.key[n + p] = has ? gkey : kNoGhost;
+        if (active) { sb.key[n + p] = has ? gkey : kNoGhost; sb.cellk[n + p] = (uint8_t)gcell; }
         (void)wave_claim(sb.count, gkey, has);
     }
 }
@@ -628,7 +648,7 @@ __device__ __forceinline__ void bin_scatter_body(const StepState* S, const SortB
     const uint32_t key = e < (n << sb.ghosts) ? sb.key[e] : kNoGhost;
     const bool active = key != kNoGhost;
     const uint32_t pos = wave_claim(sb.cursor, active ? key : 0u, active);
-    if (active) sb.perm[pos] = e < n ? e : (kGhostBit | (e - n));
+    if (active) { sb.perm[pos] = e < n ? e : (kGhostBit | (e - n)); sb.cells[pos] = sb.cellk[e]; }
 }
 template <int = 0>  // a template, so that every translation unit may instantiate it
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, SortBufs sb, uint32_t n) { bin_scatter_body(S, sb, n, blockIdx.x); }
@@ -710,6 +730,28 @@ template <int D, int SPL> struct BlockCoordLDS {
     }
 };
 
+// The coordinate entries of BlockCoordLDS, entry t of every axis in thread t (a workgroup has at least 64
+// threads), loaded into registers by issue() and written to LDS by put(): a block kernel issues the loads first
+// and puts them after other work that waits for memory anyway, so the coordinates cost no latency of their own.
+template <int D, int SPL> struct CoordPrefetch {
+    static constexpr int LC = BlockCoordLDS<D, SPL>::LC;
+    double v[3] = {0.0, 0.0, 0.0};
+    __device__ __forceinline__ void issue(double* const* coord, const int* bc, const uint32_t* n) {
+        static_assert(LC <= 64, "one pass of a wavefront per axis");
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int gi = BG<D, SPL>::E * bc[d] - 1 + t;
+            v[d] = (t < LC && gi >= 0 && gi < (int)n[d]) ? coord[d][gi] : 0.0;
+        }
+    }
+    __device__ __forceinline__ void put(BlockCoordLDS<D, SPL>& c) const {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int d = 0; d < D; ++d) if (t < LC) c.c[d][t] = v[d];
+    }
+};
+
 template <int D, int SPL, int NV, class TF> struct ScatterLDS {
     using G = BG<D, SPL>;
     BlockCoordLDS<D, SPL> coords;
@@ -737,6 +779,17 @@ __device__ __forceinline__ int tile_index(int t0, int t1, int t2) {
 // GHOST: the block's periodic copies (perm entries kGhostBit | p, at their copy positions); they are only
 // ordered in LDS for the scatter, and nothing is written back.
 constexpr int kOrderLoads = 8;
+// Ascending sort of 8 values in registers: Batcher's odd-even merge network, 19 compare-exchanges (checked on
+// every 0-1 input, which by the 0-1 principle covers all inputs).
+__device__ __forceinline__ void sort8_u32(uint32_t (&r)[8]) {
+    auto cx = [&](int i, int j) { const uint32_t a = r[i], b = r[j]; r[i] = min(a, b); r[j] = max(a, b); };
+    cx(0, 1); cx(2, 3); cx(4, 5); cx(6, 7);
+    cx(0, 2); cx(1, 3); cx(4, 6); cx(5, 7);
+    cx(1, 2); cx(5, 6);
+    cx(0, 4); cx(1, 5); cx(2, 6); cx(3, 7);
+    cx(2, 4); cx(3, 5);
+    cx(1, 2); cx(3, 4); cx(5, 6);
+}
 // dst[k] = src[k] for k < n, global memory to LDS, with kOrderLoads loads in flight per thread.
 template <class T, class S>
 __device__ __forceinline__ void copy_to_lds(T* dst, const S* src, uint32_t n) {
@@ -762,44 +815,26 @@ __device__ __forceinline__ uint32_t order_block(LT& L, const Params& P, const St
     const int tid = threadIdx.x;
     for (int c = tid; c <= G::NC; c += blockDim.x) L.u.sort.ccount[c] = 0;
     __syncthreads();
-    const double h1 = P.one_over_dx;
-    const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
-    const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
     uint32_t err = 0;
-    // A thread takes kOrderLoads of the block's particles at a time and issues all their slot and position
-    // loads before it uses any: two memory latencies per group instead of two per particle (a one-wavefront
-    // workgroup orders about 8 particles per thread).
+    // Each entry's dual cell comes from the binning (sb.cells, the window base that k_bin_count found with the
+    // same coordinates). A thread takes kOrderLoads of the block's entries at a time and issues their slot and
+    // cell loads together: one memory latency per group (a one-wavefront workgroup orders about 8 particles
+    // per thread).
     for (uint32_t k0 = tid; k0 < cnt; k0 += kOrderLoads * blockDim.x) {
-        uint32_t pk[kOrderLoads];
-        double xk[kOrderLoads][3];
+        uint32_t pk[kOrderLoads], ck[kOrderLoads];
 #pragma unroll
         for (int u = 0; u < kOrderLoads; ++u) {
             const uint32_t k = k0 + (uint32_t)u * blockDim.x;
             pk[u] = k < cnt ? sb.perm[start + k] : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < kOrderLoads; ++u) {
-            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) xk[u][d] = (d < D && k < cnt) ? src.x[d][pk[u] & ~kGhostBit] : 0.0;
+            ck[u] = k < cnt ? (uint32_t)sb.cells[start + k] : 0u;
         }
 #pragma unroll
         for (int u = 0; u < kOrderLoads; ++u) {
             const uint32_t k = k0 + (uint32_t)u * blockDim.x;
             if (k >= cnt) break;
-            const uint32_t p = pk[u];
-            double xp[3] = {xk[u][0], xk[u][1], xk[u][2]};
-            if (GHOST) { double gx = xp[0]; (void)periodic_copy(P.L[0], 4.0 * P.dx, xp[0], gx); xp[0] = gx; }
-            int wb[3];
-            int cell = G::NC;
-            if (window_base<D, SPL>(wb, xp, org, nn, L.coords.view(bc), h1, true)) {
-                int l[3] = {0, 0, 0};
-                bool in = true;
-                for (int d = 0; d < D; ++d) { l[d] = wb[d] - G::E * bc[d]; in = in && l[d] >= 0 && l[d] < G::E; }
-                if (in) cell = D == 3 ? (l[0] * G::E + l[1]) * G::E + l[2] : l[0] * G::E + l[1];
-            }
+            const uint32_t cell = min(ck[u], (uint32_t)G::NC);
             if (cell == G::NC) err |= MPM_ERR_GRID_BOUNDS;
-            L.u.sort.tmp[k] = p;
+            L.u.sort.tmp[k] = pk[u];
             L.u.sort.cell[k] = (uint8_t)cell;
             atomicAdd(&L.u.sort.ccount[cell], 1u);
         }
@@ -823,68 +858,80 @@ __device__ __forceinline__ uint32_t order_block(LT& L, const Params& P, const St
         L.p[slot] = L.u.sort.tmp[k];
     }
     __syncthreads();
-    // Each bucket ordered by source slot (the entries are distinct), into tmp. A one-wavefront workgroup
-    // sorts by insertion, one lane per bucket. A larger one ranks: one thread per slot counts the entries of
-    // its bucket below its own, so that every wavefront takes part instead of waiting for the first
+    // Each bucket ordered by source slot (the entries are distinct), then laid out rank-major: slot order is
+    // rank 0 of every non-empty cell in cell order, then rank 1, and so on. A one-wavefront workgroup whose
+    // buckets hold at most 8 entries (and none outside the block) sorts each bucket in its lane's registers
+    // with a 19-comparator network and writes the rank-major layout from them. Otherwise a one-wavefront
+    // workgroup sorts by insertion, one lane per bucket, and a larger one ranks: one thread per slot counts the
+    // entries of its bucket below its own, so that every wavefront takes part instead of waiting for the first
     // (measured: the 8-wavefront cubic scatter 3234 -> 2898 us on C3; the one-wavefront quadratic scatter of C2
     // was faster by insertion, 234 against 245 us).
+    bool laid_out = false;
     if (blockDim.x <= 64) {
-        for (int c = tid; c <= G::NC; c += blockDim.x) {
-            const uint32_t s0 = L.cstart[c], s1 = L.cstart[c + 1];
-            for (uint32_t i = s0 + 1; i < s1; ++i) {
-                const uint32_t key = L.p[i];
-                uint32_t j = i;
-                while (j > s0 && L.p[j - 1] > key) { L.p[j] = L.p[j - 1]; --j; }
-                L.p[j] = key;
+        const uint32_t s0 = L.cstart[tid], n_c = L.cstart[tid + 1] - s0;
+        if (wave_max_u32(n_c) <= 8u && L.cstart[G::NC] == cnt) {
+            uint32_t r[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) r[i] = (uint32_t)i < n_c ? L.p[s0 + i] : 0xFFFFFFFFu;
+            sort8_u32(r);
+            const unsigned long long below = (1ull << tid) - 1ull;
+            uint32_t base = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {  // every read of L.p above precedes these writes (one wavefront)
+                const bool has = (uint32_t)j < n_c;
+                const unsigned long long m = __ballot(has);
+                if (m == 0ull) break;
+                if (has) L.p[base + (uint32_t)__popcll(m & below)] = r[j];
+                base += (uint32_t)__popcll(m);
+            }
+            laid_out = true;
+        }
+    }
+    if (!laid_out) {
+        if (blockDim.x <= 64) {
+            for (int c = tid; c <= G::NC; c += blockDim.x) {
+                const uint32_t s0 = L.cstart[c], s1 = L.cstart[c + 1];
+                for (uint32_t i = s0 + 1; i < s1; ++i) {
+                    const uint32_t key = L.p[i];
+                    uint32_t j = i;
+                    while (j > s0 && L.p[j - 1] > key) { L.p[j] = L.p[j - 1]; --j; }
+                    L.p[j] = key;
+                }
+            }
+            __syncthreads();
+            for (uint32_t k = tid; k < cnt; k += blockDim.x) L.u.sort.tmp[k] = L.p[k];
+        } else {
+            for (uint32_t i = tid; i < cnt; i += blockDim.x) {
+                int c = 0;  // the last bucket starting at or before slot i
+#pragma unroll
+                for (int step = 64; step > 0; step >>= 1)
+                    if (c + step <= G::NC && L.cstart[c + step] <= i) c += step;
+                const uint32_t s0 = L.cstart[c], s1 = L.cstart[c + 1], key = L.p[i];
+                uint32_t r = 0;
+                for (uint32_t j = s0; j < s1; ++j) r += L.p[j] < key ? 1u : 0u;
+                L.u.sort.tmp[s0 + r] = key;
             }
         }
         __syncthreads();
-        for (uint32_t k = tid; k < cnt; k += blockDim.x) L.u.sort.tmp[k] = L.p[k];
-    } else {
-        for (uint32_t i = tid; i < cnt; i += blockDim.x) {
-            int c = 0;  // the last bucket starting at or before slot i
-#pragma unroll
-            for (int step = 64; step > 0; step >>= 1)
-                if (c + step <= G::NC && L.cstart[c + step] <= i) c += step;
-            const uint32_t s0 = L.cstart[c], s1 = L.cstart[c + 1], key = L.p[i];
-            uint32_t r = 0;
-            for (uint32_t j = s0; j < s1; ++j) r += L.p[j] < key ? 1u : 0u;
-            L.u.sort.tmp[s0 + r] = key;
+        if (tid < 64) {  // rank-major layout: one wavefront, lane = cell
+            const uint32_t c0 = L.cstart[tid], n_c = L.cstart[tid + 1] - c0;
+            const unsigned long long below = (1ull << tid) - 1ull;
+            uint32_t base = 0;
+            for (uint32_t j = 0;; ++j) {
+                const bool has = j < n_c;
+                const unsigned long long m = __ballot(has);
+                if (m == 0ull) break;
+                if (has) L.p[base + (uint32_t)__popcll(m & below)] = L.u.sort.tmp[c0 + j];
+                base += (uint32_t)__popcll(m);
+            }
         }
+        for (uint32_t k = L.cstart[G::NC] + tid; k < cnt; k += blockDim.x) L.p[k] = L.u.sort.tmp[k];  // outside bucket last
     }
-    __syncthreads();
-    if (tid < 64) {  // rank-major layout: one wavefront, lane = cell
-        const uint32_t c0 = L.cstart[tid], n_c = L.cstart[tid + 1] - c0;
-        const unsigned long long below = (1ull << tid) - 1ull;
-        uint32_t base = 0;
-        for (uint32_t j = 0;; ++j) {
-            const bool has = j < n_c;
-            const unsigned long long m = __ballot(has);
-            if (m == 0ull) break;
-            if (has) L.p[base + (uint32_t)__popcll(m & below)] = L.u.sort.tmp[c0 + j];
-            base += (uint32_t)__popcll(m);
-        }
-    }
-    for (uint32_t k = L.cstart[G::NC] + tid; k < cnt; k += blockDim.x) L.p[k] = L.u.sort.tmp[k];  // outside bucket last
     __syncthreads();
     if (GHOST) return err;
-    // the ids are loaded kOrderLoads at a time before they are stored (a store to dst.id could alias src.id for
-    // the compiler, which would otherwise wait for each load in turn)
-    for (uint32_t k0 = tid; k0 < cnt; k0 += kOrderLoads * blockDim.x) {
-        uint32_t idk[kOrderLoads];
-#pragma unroll
-        for (int u = 0; u < kOrderLoads; ++u) {
-            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
-            idk[u] = k < cnt ? src.id[L.p[k]] : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < kOrderLoads; ++u) {
-            const uint32_t k = k0 + (uint32_t)u * blockDim.x;
-            if (k >= cnt) break;
-            sb.perm[start + k] = L.p[k];
-            dst.id[start + k] = idk[u];
-        }
-    }
+    // The sorted permutation. The destination ids (dst.id[slot] = src.id[perm[slot]]) are written by the
+    // deformation kernel, which loads perm[slot] anyway; here their loads would add a memory latency per block.
+    for (uint32_t k = tid; k < cnt; k += blockDim.x) sb.perm[start + k] = L.p[k];
     for (int c = tid; c <= G::NC; c += blockDim.x) sb.cell_off[(size_t)b * (G::NC + 1) + c] = (uint16_t)L.cstart[c];
     return err;
 }
@@ -1927,7 +1974,6 @@ __device__ __forceinline__ void p2g_staged_body(const Params& P, StepState* S, c
     cf.Lx = P.L[0]; cf.band = 4.0 * P.dx;
     uint32_t err = 0;
     if (cnt > kMaxBP) {
-        for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) dst.id[start + k] = src.id[sb.perm[start + k]];
         err |= staged_block_atomic<D, SPL, NV, TF>(L, cf, stg, sb, b, start, cnt, bc);
     } else {
         err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
@@ -1985,24 +2031,26 @@ __device__ __forceinline__ void p2g_plain_body(const Params& P, StepState* S, co
     P2GContrib<D, SPL, SEP, MIBF, TF> cf;
     cf.src = src;
     for (int d = 0; d < 3; ++d) { cf.org[d] = S->origin[d]; cf.nn[d] = S->n[d]; }
-    L.coords.load(g.coord, bc, cf.nn);
+    // the block's coordinates are loaded with the ordering's loads and put into LDS after it
+    CoordPrefetch<D, SPL> cp;
+    cp.issue(g.coord, bc, cf.nn);
     cf.cs = L.coords.view(bc);
-    __syncthreads();
     cf.h1 = P.one_over_dx; cf.aD = P.apicDinverse; cf.apic = P.flip_ratio < 0;
     cf.kf = S->dt * P.particle_volume / P.particle_mass;
     cf.Lx = P.L[0]; cf.band = 4.0 * P.dx;
     uint32_t err = 0;
     if (cnt > kMaxBP || gcnt > kMaxBP) {
-        if (cnt > kMaxBP) {
-            for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) dst.id[start + k] = src.id[sb.perm[start + k]];
-        } else if (cnt) {
-            err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
-        }
+        if (cnt && cnt <= kMaxBP) err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
+        cp.put(L.coords);
         err |= scatter_block_atomic<D, SPL, NV, TF>(L, cf, stg, sb, b, start, cnt, bc, gstart, gcnt);
     } else if (gcnt == 0) {
         err |= order_block<D, SPL, NV, TF>(L, P, S, src, dst, sb, g, b, start, cnt, bc);
+        cp.put(L.coords);
+        __syncthreads();
         scatter_block<D, SPL, NV, TF, P2GContrib<D, SPL, SEP, MIBF, TF>, NEXT>(L, cf, stg, b, start);
     } else {
+        cp.put(L.coords);
+        __syncthreads();
         TF acc[SS::SPP][SS::NSH][NV];
         zero_acc<D, SPL, NV, TF>(acc);
         if (cnt) {
@@ -2715,13 +2763,39 @@ __device__ __forceinline__ void g2p_musl_body(const Params& P, StepState* S, con
     const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
     const bool apic = P.flip_ratio < 0;
     const bool flip_rt = P.flip_ratio >= -1;
+    // The block's coordinates, the first kOrderLoads wavefront-widths of its slots and its cell offsets are
+    // loaded into registers first, then the velocity tiles; the stores to LDS then wait for all of them at once
+    // (one memory latency for the prologue instead of one per array).
+    CoordPrefetch<D, SPL> cp;
+    cp.issue(g.coord, br.bc, nn);
+    const bool ordered = br.cnt <= (uint32_t)kMaxBP;  // the block's cell order, as the P2G scatter left it
+    uint32_t pk[kOrderLoads], co[2];
+#pragma unroll
+    for (int u = 0; u < kOrderLoads; ++u) {
+        const uint32_t k = threadIdx.x + (uint32_t)u * blockDim.x;
+        pk[u] = ordered && k < br.cnt ? sb.perm[br.start + k] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint32_t c = threadIdx.x + (uint32_t)u * blockDim.x;
+        co[u] = ordered && c <= (uint32_t)G::NC ? (uint32_t)sb.cell_off[(size_t)br.b * (G::NC + 1) + c] : 0u;
+    }
     load_tile<D, SPL, TF>(s_v, g.v, br.bc, nn);
     if (FLIP) load_tile<D, SPL, TF>(s_f, g.flip, br.bc, nn);
-    L.coords.template load<true>(g.coord, br.bc, nn);
-    const bool ordered = br.cnt <= (uint32_t)kMaxBP;
-    if (ordered) {  // the block's cell order, as the P2G scatter left it (slot start + k holds source particle p[k])
-        copy_to_lds(L.p, sb.perm + br.start, br.cnt);
-        copy_to_lds(L.cstart, sb.cell_off + (size_t)br.b * (G::NC + 1), (uint32_t)G::NC + 1);
+    cp.put(L.coords);
+    if (ordered) {  // slot start + k holds source particle p[k]
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const uint32_t k = threadIdx.x + (uint32_t)u * blockDim.x;
+            if (k < br.cnt) L.p[k] = pk[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const uint32_t c = threadIdx.x + (uint32_t)u * blockDim.x;
+            if (c <= (uint32_t)G::NC) L.cstart[c] = co[u];
+        }
+        const uint32_t head = kOrderLoads * blockDim.x;
+        if (br.cnt > head) copy_to_lds(L.p + head, sb.perm + br.start + head, br.cnt - head);
         if (threadIdx.x == 0) L.cstart[G::NC + 1] = br.cnt;
     }
     __syncthreads();
@@ -2896,7 +2970,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
     // kernel from 118 to 131 VGPRs and 337 to 370 us, so there one thread reads the span and every particle
     // is loaded after the tiles.
     constexpr bool kEarlyState = sizeof(TF) == 4;
-    uint32_t p = 0, key = 0;
+    uint32_t p = 0, key = 0, idp = 0;
     double xp[3] = {0, 0, 0};
     DefgradIn<D> in;
     const uint32_t last = min(n, base + blockDim.x) - 1;
@@ -2904,6 +2978,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
         if (active) {
             p = sb.perm[i];
             key = sb.key[p];
+            idp = src.id[p];
             for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
             load_defgrad_in<D, TF>(src, p, in);
             if (i == base) s_lo = key;
@@ -2923,6 +2998,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
             p = sb.perm[i];
             for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
             key = sb.key[p];
+            idp = src.id[p];
             load_defgrad_in<D, TF>(src, p, in);
         }
         const double h1 = P.one_over_dx, dt = S->dt;
@@ -2940,6 +3016,7 @@ __device__ __forceinline__ void defgrad_body(const Params& P, StepState* S, cons
             ok = make_stencil<D, SPL>(st, xp, org, nn, GlobalCoords{g.coord, nn}, h1, false);
             if (ok) gather<D, SPL, false, true, false, TF>(st, xp, h1, GridFetch<D, TF>{g, nn, st.base[0], st.base[1], st.base[2]}, false, false, pic, fl, Bp, gvt);
         }
+        dst.id[i] = idp;  // the particle's id moves with it to its sorted slot (for every slot, in bounds or not)
         if (!ok) {
             err |= MPM_ERR_GRID_BOUNDS;
         } else {

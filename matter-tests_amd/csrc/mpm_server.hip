@@ -50,8 +50,8 @@ uint64_t estimate_bytes(const mpm_config& c, uint32_t n, const uint32_t* span, u
     const bool split = c.precision == MPM_PRECISION_SPLIT;
     const size_t tf = split ? sizeof(float) : sizeof(double);
     const size_t nd = D + D * D + 6, nt = D + D * D + (D == 3 ? 6 : 3);
-    const uint64_t pbc = c.pbc ? 1 : 0;  // periodic: key and perm and the block arrays also hold the copies
-    uint64_t bytes = 2 * (nd * sizeof(double) + nt * tf) * ((uint64_t)n + 1024) + (uint64_t)(16 + 8 * pbc) * n;
+    const uint64_t pbc = c.pbc ? 1 : 0;  // periodic: key, perm, the cell bytes and the block arrays also hold the copies
+    uint64_t bytes = 2 * (nd * sizeof(double) + nt * tf) * ((uint64_t)n + 1024) + (uint64_t)(16 + 8 * pbc + 2 + 2 * pbc) * n + 4;
     uint64_t nodes = 1, blocks = 1;
     const uint32_t E = D == 3 ? 4 : 8;
     for (int d = 0; d < D; ++d) {
