@@ -428,12 +428,18 @@ def main():
 def column2d_run(args, steps=200):
     """The 2D plane-strain column collapse that the reference runs on the CPU: 0.5 x 0.5 m, dx = 0.01 m, 4 particles
     per cell (10,000), quadratic, USL, APIC, Drucker-Prager 30 deg, NoSlip floor, 200 steps from rest, plain FP64.
-    The whole run on the GPU (one mpm_step call) and on the oracle (min(16, host cores) threads), and the
-    difference of the final states."""
+    The whole run on the GPU (one mpm_step call, after a 32-step run of the same scene on another context has
+    loaded the 2D kernels' code objects) and on the oracle (min(16, host cores) threads), and the difference of
+    the final states."""
     cfg, plates, objs, x = S.column_2d()
     n = x.shape[0]
     out = {"workload": "2D plane-strain column collapse, DP 30 deg, APIC, USL, NoSlip floor, 200 steps",
            "particles": n, "precision": "fp64", "steps": steps, "unit": "particle-updates/s"}
+    # the process's first launches of the 2D kernels load their code objects: done once, on a context of its own
+    warm = A.Engine(A.product_library(), cfg, plates, objs)
+    warm.set_particles(x)
+    warm.step(32)
+    del warm
     eng = A.Engine(A.product_library(), cfg, plates, objs)
     eng.set_particles(x)
     t0 = time.perf_counter()

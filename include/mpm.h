@@ -313,6 +313,13 @@ int mpm_set_profiling(mpm_context* ctx, int enable);
  * MPM_ERR_INVALID_ARGUMENT for an unknown name. */
 int mpm_set_profiling_kernel(mpm_context* ctx, const char* name);
 
+/* Enables (1, the default) or disables (0) HIP-graph replay of mpm_step batches. With it, a scene of at most
+ * 262,144 particles stepped at least 16 steps per host synchronisation has each batch's launches captured once
+ * and replayed (the results are those of ordinary launches, bit for bit); other calls launch as usual. No
+ * counterpart in the reference (a CPU loop); for callers that compare the two launch paths or step while
+ * another thread captures on the same device. */
+int mpm_set_step_graphs(mpm_context* ctx, int enable);
+
 /* Copies the accumulated timings (one entry per kernel that ran) into out[0..capacity). */
 int mpm_get_kernel_times(mpm_context* ctx, mpm_kernel_timing* out, uint32_t capacity, uint32_t* count);
 

@@ -70,6 +70,18 @@ struct mpm_context {
     mpm_step_info last{};
     // device lattices of the level-set objects (mpm_add_sdf_object)
     std::vector<void*> sdf_bufs;
+    // HIP graphs of step batches (mpm_step on small scenes): the launches of `batch` steps captured once and
+    // replayed while everything they bake in (launch shapes, buffers, parameters, layout parity) is unchanged
+    struct StepGraph {
+        uint32_t batch, xpar, flags, act_blocks, n;
+        uint64_t grid_gen;
+        const void* pbuf;
+        const void* ibuf;
+        mpm::Params P;
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> step_graphs;
+    bool step_graphs_on = true;  // mpm_set_step_graphs
 };
 
 namespace mpm {

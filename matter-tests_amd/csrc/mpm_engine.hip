@@ -710,6 +710,7 @@ void mpm_destroy(mpm_context* c) {
     if (c->dP) (void)hipFree(c->dP);
     for (void* q : c->sdf_bufs) (void)hipFree(q);
     for (hipEvent_t ev : c->ev_pool) (void)hipEventDestroy(ev);
+    for (auto& g : c->step_graphs) (void)hipGraphExecDestroy(g.exec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -849,6 +850,49 @@ int mpm_set_particles(mpm_context* c, uint32_t n, const mpm_particle_arrays* in)
 }  // extern "C"
 
 namespace {
+// Single-context step graphs. A batch of a scene with at most kGraphMaxParticles particles and at least
+// kGraphMinSteps steps is captured once as a HIP graph and replayed: such a step is bound by the host's launch
+// rate (about a dozen launches per step), not by the device. Larger scenes keep ordinary launches, whose
+// enqueue cost hides behind the device work, so that a graph's capture would only add time.
+constexpr uint32_t kGraphMaxParticles = 262144, kGraphMinSteps = 16;
+constexpr size_t kMaxStepGraphs = 4;
+
+int launch_step_graph(mpm_context* c, uint32_t batch, uint32_t xsel, uint32_t flags) {
+    const bool check_cons = (flags & MPM_STEP_CHECK_CONSERVATION) != 0;
+    auto match = [&](const mpm_context::StepGraph& g) {
+        return g.batch == batch && g.xpar == (xsel & 1u) && g.flags == flags && g.act_blocks == c->act_blocks && g.n == c->n &&
+               g.grid_gen == c->grid_gen && g.pbuf == (const void*)c->pbuf && g.ibuf == (const void*)c->ibuf &&
+               std::memcmp(&g.P, &c->P, sizeof(Params)) == 0;
+    };
+    auto it = std::find_if(c->step_graphs.begin(), c->step_graphs.end(), match);
+    if (it == c->step_graphs.end()) {
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u, check_cons ? 1u : 0u, c->cap_axis,
+                                           (uint32_t)c->cap_nodes, c->act_blocks);
+        for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, check_cons);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (e != hipSuccess || !graph) { if (graph) (void)hipGraphDestroy(graph); return MPM_ERR_HIP; }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return MPM_ERR_HIP;
+        if (c->step_graphs.size() >= kMaxStepGraphs) {
+            (void)hipGraphExecDestroy(c->step_graphs.front().exec);
+            c->step_graphs.erase(c->step_graphs.begin());
+        }
+        mpm_context::StepGraph g;
+        g.batch = batch; g.xpar = xsel & 1u; g.flags = flags; g.act_blocks = c->act_blocks; g.n = c->n; g.grid_gen = c->grid_gen;
+        g.pbuf = c->pbuf; g.ibuf = c->ibuf; std::memcpy(&g.P, &c->P, sizeof(Params)); g.exec = exec;
+        c->step_graphs.push_back(g);
+        it = c->step_graphs.end() - 1;
+    } else {
+        c->flip_formed = c->P.flip_ratio >= -1 || check_cons;  // as enqueue_step records it
+    }
+    HIP_TRY(hipGraphLaunch(it->exec, c->stream));
+    return 0;
+}
+
 // The loop of mpm_step: batches of steps enqueued without host synchronisation, the grid grown between
 // batches. scratch == nullptr: the context's own store (allocated as needed). Otherwise the grid lives in
 // the device scratch at `scratch`, batches are kNoAllocBatch steps (a smaller growth margin), and nothing
@@ -862,10 +906,15 @@ int step_loop(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* in
         int rc = scratch ? bind_scratch(c, batch, scratch) : ensure_grid(c, batch);
         if (rc) { status |= rc; break; }
         uint32_t xsel = c->Sh->xsel;
-        k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u,
-                                           (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u, c->cap_axis, (uint32_t)c->cap_nodes,
-                                           c->act_blocks);
-        for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, (flags & MPM_STEP_CHECK_CONSERVATION) != 0);
+        if (!scratch && !c->profiling && c->step_graphs_on && c->n <= kGraphMaxParticles && batch >= kGraphMinSteps) {
+            rc = launch_step_graph(c, batch, xsel, flags);
+            if (rc) { status |= rc; break; }
+        } else {
+            k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u,
+                                               (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u, c->cap_axis, (uint32_t)c->cap_nodes,
+                                               c->act_blocks);
+            for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, (flags & MPM_STEP_CHECK_CONSERVATION) != 0);
+        }
         if (hipGetLastError() != hipSuccess) { status |= MPM_ERR_HIP; break; }
         rc = sync_state(c);
         if (rc) { status |= rc; break; }
@@ -1093,6 +1142,13 @@ int mpm_set_profiling(mpm_context* c, int enable) {
     for (int k = 0; k < KS_COUNT; ++k) { c->kt_ms[k] = 0; c->kt_n[k] = 0; }
     c->profiling = enable != 0;
     c->prof_mask = ~0u;
+    return 0;
+}
+
+int mpm_set_step_graphs(mpm_context* c, int enable) {
+    if (!c) return MPM_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->step_graphs_on = enable != 0;
     return 0;
 }
 

@@ -216,6 +216,7 @@ class Library:
             L.mpm_config_defaults.argtypes = [C.POINTER(Config)]
             L.mpm_set_profiling.argtypes = [vp, C.c_int]
             L.mpm_set_profiling_kernel.argtypes = [vp, C.c_char_p]
+            L.mpm_set_step_graphs.argtypes = [vp, C.c_int]
             L.mpm_get_binning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]
             L.mpm_get_kernel_times.argtypes = [vp, C.POINTER(KernelTiming), C.c_uint32, C.POINTER(C.c_uint32)]
             L.mpm_abi_version.restype = C.c_int
@@ -435,6 +436,10 @@ class Engine:
 
     def set_profiling(self, enable=True):
         self._check(self.L.lib.mpm_set_profiling(self.h, 1 if enable else 0), "set_profiling")
+
+    def set_step_graphs(self, enable=True):
+        """HIP-graph replay of step batches for small scenes (on by default)."""
+        self._check(self.L.lib.mpm_set_step_graphs(self.h, 1 if enable else 0), "set_step_graphs")
 
     def set_profiling_kernel(self, name):
         """Times only the kernel `name` (two HIP events per step instead of two per launch)."""
