@@ -2321,14 +2321,14 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
     } else if (kLeftSpectral && left_spectral<D>(F, U, lam, P.svd_tol)) {
         // singular values sqrt(lam), all within a factor 100 of each other and above the 1e-4 clamp
         V<D> ls;
-        for (int k = 0; k < D; ++k) ls[k] = 0.5 * log_d(lam[k]);
+        for (int k = 0; k < D; ++k) ls[k] = 0.5 * log_d_split(lam[k]);
         if (MAT != MPM_PLASTIC_NONE) {
             V<D> h = ls;
-            RMResult r = return_map<D, MAT>(P, dt, h, ps);
+            RMResult r = return_map<D, MAT, false>(P, dt, h, ps);
             err |= r.err;
             pcount += r.plastic_incr;
-            if (r.recompose) {
-                V<D> e; for (int k = 0; k < D; ++k) e[k] = exp(h[k]) * rsqrt_d(lam[k]);
+            if (r.recompose) {  // exp(h) / sqrt(lam) = exp(h - ln sqrt(lam)): one exponential, no square root
+                V<D> e; for (int k = 0; k < D; ++k) e[k] = exp(h[k] - ls[k]);
                 F = recompose_left<D>(U, e, F);
                 ls = h;
             }
@@ -2341,7 +2341,7 @@ __device__ __forceinline__ void deform_and_project(const Params& P, double dt, c
         if (MAT != MPM_PLASTIC_NONE) {
             V<D> h; for (int k = 0; k < D; ++k) h[k] = log_d(fmax(fabs(s[k]), 1e-4));
             const V<D> h0 = h;
-            RMResult r = return_map<D, MAT>(P, dt, h, ps);
+            RMResult r = return_map<D, MAT, !kLeftSpectral>(P, dt, h, ps);
             err |= r.err;
             pcount += r.plastic_incr;
             if (r.recompose) {  // the new F = U exp(h) W^T has singular values exp(h): its Hencky strain is h

@@ -58,6 +58,42 @@ MPM_HD double log_d(double x) {
     return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lm));
 }
 
+// 1/x for the split form's arithmetic: the hardware reciprocal estimate refined by two Newton steps (about 1 ulp,
+// against the correctly rounded IEEE division's 10 instructions; each step at least doubles the correct bits of
+// an estimate that starts near single precision). For positive normal x. The plain form keeps IEEE division, so
+// that its results round as the reference's.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline double rcp_d(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+#else
+MPM_HD double rcp_d(double x) { return 1.0 / x; }
+#endif
+
+// log_d with the reciprocal of rcp_d in place of the division (m - 1)/(m + 1): for the split form.
+MPM_HD double log_d_split(double x) {
+    if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308)) return log(x);
+    int e;
+    double m = frexp(x, &e);
+    const bool lo = m < 0.70710678118654752440;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const double f = (m - 1.0) * rcp_d(m + 1.0);
+    const double s = f * f;
+    double q = 1.0 / 21.0;
+    q = fma(q, s, 1.0 / 19.0); q = fma(q, s, 1.0 / 17.0); q = fma(q, s, 1.0 / 15.0); q = fma(q, s, 1.0 / 13.0);
+    q = fma(q, s, 1.0 / 11.0); q = fma(q, s, 1.0 / 9.0); q = fma(q, s, 1.0 / 7.0); q = fma(q, s, 1.0 / 5.0);
+    q = fma(q, s, 1.0 / 3.0);
+    const double f2 = f + f;
+    const double lm = fma(f2 * s, q, f2);
+    const double de = (double)e;
+    return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lm));
+}
+
 // 1/sqrt(x): the device library's reciprocal square root (a hardware estimate refined to about 1 ulp).
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ inline double rsqrt_d(double x) { return rsqrt(x); }
@@ -308,7 +344,7 @@ MPM_HD void sym_jacobi_rot(M<D>& A, M<D>& U, bool& rotated, double thr) {
     const double ig = rsqrt_d(fma(g, g, be * be));
     const double sbe = al < 0 ? -be : be;
     const double c = g * ig, s = sbe * ig;
-    const double tapq = sbe * apq / g;
+    const double tapq = sbe * apq * rcp_d(g);  // g > 0 (split form only: the plain form takes svd())
     A(p, p) = A(p, p) - tapq;
     A(q, q) = A(q, q) + tapq;
     A(p, q) = 0.0; A(q, p) = 0.0;
@@ -580,27 +616,33 @@ constexpr int kMatDPViscLinear = 18;
 // ln(max(|s|,1e-4)) of F_trial on entry and the projected strains on exit when recompose is set.
 // For VMVisc's capped branch the result is F = U W^T, i.e. h = 0. PM >= 0 fixes the plastic model at
 // compile time (the kernel then carries only that model's code and registers); PM < 0 reads it from P.
-template <int D, int PM = -1>
+// EXACT = false (the split form): the divisions of the shared head and of the von Mises and Drucker-Prager branches
+// by dim, |dev|, d_prefac, dt and K become products with rcp_d reciprocals (about 1 ulp each).
+template <int D, int PM = -1, bool EXACT = true>
 MPM_HD RMResult return_map(const Params& P, double dt, V<D>& h, PlasticState& st) {
     RMResult r{false, false, 0, 0u};
     const double dim = D;
     double tr = h[0]; for (int i = 1; i < D; ++i) tr += h[i];
-    V<D> dev; for (int i = 0; i < D; ++i) dev[i] = h[i] - (tr / dim) * 1.0;
+    V<D> dev; for (int i = 0; i < D; ++i) dev[i] = h[i] - (EXACT ? tr / dim : tr * (1.0 / dim)) * 1.0;
     double nrm2 = dev[0] * dev[0]; for (int i = 1; i < D; ++i) nrm2 += dev[i] * dev[i];
     const double dn = sqrt(nrm2);
-    if (dn > 0) for (int i = 0; i < D; ++i) dev[i] /= dn;
-    const double K = P.K, inv_d = 1.0 / P.d_prefac;
-    auto set_iso = [&](double pp) { double e = -pp / (K * dim); for (int i = 0; i < D; ++i) h[i] = e * 1.0; r.recompose = true; };
+    if (dn > 0) {
+        if (EXACT) { for (int i = 0; i < D; ++i) dev[i] /= dn; }
+        else { const double idn = rcp_d(dn); for (int i = 0; i < D; ++i) dev[i] *= idn; }
+    }
+    const double K = P.K, inv_d = EXACT ? 1.0 / P.d_prefac : rcp_d(P.d_prefac);
+    auto div = [&](double a, double b) { return EXACT ? a / b : a * rcp_d(b); };
+    auto set_iso = [&](double pp) { double e = EXACT ? -pp / (K * dim) : -pp * rcp_d(K * dim); for (int i = 0; i < D; ++i) h[i] = e * 1.0; r.recompose = true; };
     auto shift_dev = [&](double amount) { for (int i = 0; i < D; ++i) h[i] -= amount * dev[i]; r.recompose = true; };
 
     constexpr bool kLin = PM == kMatVMViscLinear || PM == kMatDPViscLinear;
     constexpr int PMB = PM == kMatVMViscLinear ? MPM_PLASTIC_VMVISC : (PM == kMatDPViscLinear ? MPM_PLASTIC_DPVISC : PM);
     switch (PMB >= 0 ? PMB : P.plastic_model) {
     case MPM_PLASTIC_VM: {  // :30-49
-        double dg = dn - P.q_max / P.e_mu_prefac;
+        double dg = dn - div(P.q_max, P.e_mu_prefac);
         if (dg > 0) {
             r.plastic_incr++;
-            st.dgamma = P.d_prefac * dg / dt;
+            st.dgamma = div(P.d_prefac * dg, dt);
             shift_dev(dg);
             st.eps_dev += dg;
         }
@@ -610,17 +652,17 @@ MPM_HD RMResult return_map(const Params& P, double dt, V<D>& h, PlasticState& st
         double q_yield = P.M * p_trial + P.q_cohesion;
         if (q_yield < 1e-10) {
             r.plastic_incr++;
-            double p_proj = -P.q_cohesion / P.M;
+            double p_proj = div(-P.q_cohesion, P.M);
             set_iso(p_proj);
             double dg = P.d_prefac * dn;
-            st.dgamma = dg / dt;
+            st.dgamma = div(dg, dt);
             st.eps_dev += inv_d * dg;
-            st.eps_vol += (p_proj - p_trial) / K;
+            st.eps_vol += div(p_proj - p_trial, K);
         } else {
-            double dg = P.d_prefac * (dn - q_yield / P.e_mu_prefac);
+            double dg = P.d_prefac * (dn - div(q_yield, P.e_mu_prefac));
             if (dg > 0) {
                 r.plastic_incr++;
-                st.dgamma = dg / dt;
+                st.dgamma = div(dg, dt);
                 shift_dev(inv_d * dg);
                 st.eps_dev += inv_d * dg;
             }
