@@ -2163,32 +2163,52 @@ void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> d
     musl_scatter_body<D, SPL, TF>(P, S, src, dst, sb, g, stg, blockIdx.x);
 }
 
-// Sum over the blocks that cover node gx (at most two per axis), in increasing block order, of the
-// staged tile values.
+// Sum over the blocks that cover node gx (at most two per axis: a tile spans E + W - 1 < 2 E nodes), in increasing
+// block order, of the staged tile values. The 2^D candidate blocks' counts and tile values are loaded together,
+// one memory latency per node instead of two per covering block; a candidate outside the grid's blocks or
+// without particles adds nothing (its staging holds stale values, read but not used).
 // with_copies: the tiles of the P2G scatter, which also exist for blocks that hold only periodic copies
 // (the MUSL re-scatter has none).
 template <int D, int SPL, int NV, class TF>
 __device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& sb, const StagingT<TF>& stg, const int* gx, double (&s)[NV],
                                             bool with_copies = false) {
     using G = BG<D, SPL>;
+    static_assert(G::T <= 2 * G::E, "a node lies in the tiles of at most two blocks per axis");
+    constexpr int NCB = 1 << D;
+    const uint32_t nb[3] = {S->nb[0], S->nb[1], D == 3 ? S->nb[2] : 1u};
+    const uint32_t nblocks = S->nblocks;
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) {
         const int l = gx[d] - (G::T - 1);
         lo[d] = l <= 0 ? 0 : (l + G::E - 1) / G::E;
-        hi[d] = min(gx[d] / G::E, (int)S->nb[d] - 1);
+        hi[d] = min(gx[d] / G::E, (int)nb[d] - 1);
+    }
+    bool ok[NCB];
+    uint32_t cnt[NCB];
+    TF val[NCB][NV];
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {  // candidate c: block (lo0 + c_0, lo1 + c_1, lo2 + c_2), last axis fastest
+        int b[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < D; ++d) b[d] = lo[d] + ((c >> (D - 1 - d)) & 1);
+        bool in = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) in = in && b[d] <= hi[d];
+        ok[c] = in;
+        const uint32_t bl = D == 3 ? ((uint32_t)b[0] * nb[1] + (uint32_t)b[1]) * nb[2] + (uint32_t)b[2] : (uint32_t)b[0] * nb[1] + (uint32_t)b[1];
+        cnt[c] = in ? sb.count[bl] + (with_copies ? sb.count[nblocks + bl] : 0u) : 0u;
+        const int q = in ? tile_index<D, SPL>(gx[0] - G::E * b[0], gx[1] - G::E * b[1], D == 3 ? gx[2] - G::E * b[2] : 0) : 0;
+        const size_t at = (size_t)(in ? bl : 0u) * G::NT + q;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) val[c][v] = stg.base[(size_t)v * stg.stride + at];
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) s[v] = 0.0;
-    for (int b0 = lo[0]; b0 <= hi[0]; ++b0)
-        for (int b1 = lo[1]; b1 <= hi[1]; ++b1)
-            for (int b2 = lo[2]; b2 <= hi[2]; ++b2) {
-                const uint32_t bl = D == 3 ? ((uint32_t)b0 * S->nb[1] + (uint32_t)b1) * S->nb[2] + (uint32_t)b2 : (uint32_t)b0 * S->nb[1] + (uint32_t)b1;
-                if (sb.count[bl] == 0 && !(with_copies && sb.count[S->nblocks + bl] != 0)) continue;
-                const int q = tile_index<D, SPL>(gx[0] - G::E * b0, gx[1] - G::E * b1, D == 3 ? gx[2] - G::E * b2 : 0);
-                const size_t at = (size_t)bl * G::NT + q;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) s[v] += (double)stg.base[(size_t)v * stg.stride + at];
-            }
+    for (int c = 0; c < NCB; ++c)
+        if (ok[c] && cnt[c] != 0)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s[v] += (double)val[c][v];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -2206,11 +2226,17 @@ __device__ __forceinline__ void grid_update_body(const Params& P, StepState* S, 
     const double dtV = dt * P.particle_volume;
     double dtg[3]; for (int d = 0; d < 3; ++d) dtg[d] = dt * S->gravity[d];
     const bool cons = S->check_conservation;
+    // the plates' state of this step, read once per workgroup (plate_move updates it in k_finish)
+    __shared__ DevPlate s_pl[kMaxPlates];
+    for (int k = threadIdx.x; k < P.n_plates; k += blockDim.x) s_pl[k] = S->plates[k];
+    __syncthreads();
     double cm = 0.0, cmv[3] = {0, 0, 0};
     for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nn; i += nbid * blockDim.x) {
         int gx[3] = {0, 0, 0};
         if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
         else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
+        V<D> X;  // loaded with the sums
+        for (int d = 0; d < D; ++d) X[d] = g.coord[d][gx[d]];
         double s[NV];
         sum_staging<D, SPL, NV, TF>(S, sb, stg, gx, s, sb.ghosts != 0);
         const double sw = s[0];
@@ -2223,9 +2249,7 @@ __device__ __forceinline__ void grid_update_body(const Params& P, StepState* S, 
         V<D> vn, flip;
         if (mi > 0) {
             for (int d = 0; d < D; ++d) vn[d] = SEP ? v[d] + (((-dtV) * s[1 + D + d]) / mi + dtg[d]) : v[d] + dtg[d];
-            V<D> X;
-            for (int d = 0; d < D; ++d) X[d] = g.coord[d][gx[d]];
-            boundary_collision<D>(P, S->plates, dt, fr, X, vn);
+            boundary_collision<D>(P, s_pl, dt, fr, X, vn);
             for (int d = 0; d < D; ++d) flip[d] = vn[d] - v[d];
         } else {
             vn = v;
@@ -2262,9 +2286,9 @@ __device__ __forceinline__ void musl_update_body(const Params& P, const StepStat
         if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
         else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
         const double mi = g.mass[i];
+        double s[D];
+        sum_staging<D, SPL, D, TF>(S, sb, stg, gx, s);  // loaded with the mass
         if (mi > 0) {
-            double s[D];
-            sum_staging<D, SPL, D, TF>(S, sb, stg, gx, s);
             const double q = mi / P.particle_mass;
             for (int d = 0; d < D; ++d) g.v[d][i] = (TF)(s[d] / q);
         } else {
