@@ -206,10 +206,11 @@ __device__ __forceinline__ void reduce_bounds(StepState* S, const BoundsAcc& ba)
     }
 }
 
-// Folds the bound slots into StepState and resets them (one wavefront of 64 lanes calls). Min/max folds
-// are idempotent, so a step that is retried after a grid reallocation sees the same bounds.
+// Folds the bound slots into H (the caller's LDS copy of the head of S) and resets them (one wavefront of 64
+// lanes calls). Min/max folds are idempotent, so a step that is retried after a grid reallocation sees the same
+// bounds.
 template <int D>
-__device__ __forceinline__ void fold_bound_slots(StepState* S) {
+__device__ __forceinline__ void fold_bound_slots(StepState* S, StepState& H) {
     unsigned long long v2 = 0ull;
     long long mn[3], mx[3];
     for (int d = 0; d < 3; ++d) { mn[d] = 0x7FFFFFFFFFFFFFFFLL; mx[d] = (-0x7FFFFFFFFFFFFFFFLL - 1); }
@@ -222,9 +223,9 @@ __device__ __forceinline__ void fold_bound_slots(StepState* S) {
     }
     v2 = wave_max_u64(v2);
     for (int d = 0; d < D; ++d) { mn[d] = wave_min_i64(mn[d]); mx[d] = wave_max_i64(mx[d]); }
-    if (threadIdx.x == 0) {
-        S->vmax2_bits = max(S->vmax2_bits, v2);
-        for (int d = 0; d < D; ++d) { S->xmin_key[d] = min(S->xmin_key[d], mn[d]); S->xmax_key[d] = max(S->xmax_key[d], mx[d]); }
+    if (threadIdx.x == 0) {  // into H, the workgroup's copy of the head of S
+        H.vmax2_bits = max(H.vmax2_bits, v2);
+        for (int d = 0; d < D; ++d) { H.xmin_key[d] = min(H.xmin_key[d], mn[d]); H.xmax_key[d] = max(H.xmax_key[d], mx[d]); }
     }
 }
 
@@ -372,112 +373,127 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, Part
 
 // ----------------------------------------------------------------------------------------------
 // k_setup: updateDt (update_dt.cpp:5-66) and remeshFixedInit/Cont (remesh.cpp:38-311).
-// One workgroup of 64 lanes; lane d builds the node coordinates of axis d (arange, tools.hpp:291-296).
+// One workgroup of 64 lanes. The head of StepState (every field before the plates) is copied to LDS by all
+// lanes and written back at the end, so that the serial decisions read LDS instead of waiting on a chain of
+// global loads. Lane d < dim remeshes axis d (the snap and the node count of arange, tools.hpp:291-296, whose
+// accumulating sum is sequential per axis) and later builds that axis's node coordinates; lane 0 then takes
+// dt, gravity and the capacity and finiteness decisions.
 // ----------------------------------------------------------------------------------------------
+constexpr int kHeadWords = (int)(offsetof(StepState, plates) / 4);
 template <int D, class TF>
 __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const GridPtrsT<TF>& g) {
-    __shared__ double lo_s[3], stop_s[3];
+    __shared__ __attribute__((aligned(16))) uint32_t head_w[kHeadWords];
+    __shared__ double lo_s[3], hi_s[3], stop_s[3], mn_s[3], mx_s[3], mni_s[3], mxi_s[3], loi_s[3], hii_s[3];
+    __shared__ uint32_t cnt_s[3];
     __shared__ int go;
-    fold_bound_slots<D>(S);
+    StepState& H = *reinterpret_cast<StepState*>(head_w);  // only head fields are used through H
+    const uint32_t* Sw = reinterpret_cast<const uint32_t*>(S);
+    for (int w = threadIdx.x; w < kHeadWords; w += 64) head_w[w] = Sw[w];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    fold_bound_slots<D>(S, H);
+    __syncthreads();
+    const double dx = P.dx;
+    const int t = threadIdx.x;
+    if (t < D) {  // remesh of axis t
+        const int d = t;
+        double mn = key_val(H.xmin_key[d]), mx = key_val(H.xmax_key[d]);
+        mn_s[d] = mn; mx_s[d] = mx;  // the particle bounds as reduced (the finiteness check reads these)
+        double mni, mxi, loi, hii, lo, hi;
+        if (H.step == 0) {
+            if (P.grid_reference_point[0] < 1e10) {
+                if (P.grid_reference_point[d] < mn) mn = P.grid_reference_point[d];
+                else if (P.grid_reference_point[d] > mx) mx = P.grid_reference_point[d];
+            }
+            mxi = mx; mni = mn;
+            loi = mn - dx * 2.0; hii = mx + dx * 2.0;
+            lo = loi; hi = hii;
+        } else {
+            mni = H.min_init[d]; mxi = H.max_init[d]; loi = H.low_init[d]; hii = H.high_init[d];
+            if (mx < mxi) { unsigned rf = (unsigned)floor(fmax(0.0, (mxi - mx) / dx - 1e-8 * dx)); hi = hii - rf * dx; }
+            else { unsigned ef = (unsigned)floor(fmax(0.0, (mx - mxi) / dx - 1e-8 * dx)); hi = hii + ef * dx; }
+            if (mn > mni) { unsigned rf = (unsigned)floor(fmax(0.0, (mn - mni) / dx - 1e-8 * dx)); lo = loi + rf * dx; }
+            else { unsigned ef = (unsigned)floor(fmax(0.0, (mni - mn) / dx - 1e-8 * dx)); lo = loi - ef * dx; }
+        }
+        // node count of arange(lo, hi + dx, dx); periodic runs keep the fixed grid of remeshFixed(4)
+        // (remesh.cpp:14-36): x and z from -5 dx to L + 6 dx, y from -dx to Ly + 6 dx
+        double stop = hi + dx;
+        if (P.pbc) { lo = d == 1 ? -dx : -dx * 5.0; stop = P.L[d] + 6.0 * dx; hi = stop - dx; }
+        uint32_t c = 0;
+        for (double val = lo; val < stop; val += dx) { if (++c > H.cap_axis) break; }
+        cnt_s[d] = c;
+        lo_s[d] = lo; hi_s[d] = hi; stop_s[d] = stop;
+        mni_s[d] = mni; mxi_s[d] = mxi; loi_s[d] = loi; hii_s[d] = hii;
+    }
+    __syncthreads();
+    if (t == 0) {
         go = 0;
-        if (!S->stop && S->frame >= P.end_frame) { S->stop = 1; S->finished = 1; }
-        if (!S->stop) {
-            double max_speed = sqrt(__longlong_as_double((long long)S->vmax2_bits));
+        if (!H.stop && H.frame >= P.end_frame) { H.stop = 1; H.finished = 1; }
+        if (!H.stop) {
+            double max_speed = sqrt(__longlong_as_double((long long)H.vmax2_bits));
             uint32_t warn = 0;
             if (max_speed >= P.wave_speed) warn |= MPM_WARN_SUPERSONIC;
             double dt;
             if (fabs(max_speed) > 1e-10) { double dt_cfl = P.cfl * P.dx / max_speed; dt = dt_cfl < P.dt_max ? dt_cfl : P.dt_max; }
             else dt = P.dt_max;
-            double c1 = P.frame_dt * (double)(S->frame + 1) - S->time; dt = c1 < dt ? c1 : dt;
-            double c2 = P.final_time - S->time; dt = c2 < dt ? c2 : dt;
-            double grav[3] = {S->gravity[0], S->gravity[1], S->gravity[2]};
+            double c1 = P.frame_dt * (double)(H.frame + 1) - H.time; dt = c1 < dt ? c1 : dt;
+            double c2 = P.final_time - H.time; dt = c2 < dt ? c2 : dt;
+            double grav[3] = {H.gravity[0], H.gravity[1], H.gravity[2]};
             if (P.gravity_special) {
-                for (int d = 0; d < D; ++d) grav[d] = (S->time < P.gravity_time) ? P.gravity_final[d] * S->time / P.gravity_time : P.gravity_final[d];
+                for (int d = 0; d < D; ++d) grav[d] = (H.time < P.gravity_time) ? P.gravity_final[d] * H.time / P.gravity_time : P.gravity_final[d];
             }
-            double mn[3], mx[3], lo[3], hi[3];
-            for (int d = 0; d < D; ++d) { mn[d] = key_val(S->xmin_key[d]); mx[d] = key_val(S->xmax_key[d]); }
             // A NaN or infinite velocity or position anywhere shows up in these reductions (a NaN's order key
             // sorts above +inf or below -inf). The reference steps on with it (dt falls back to dt_max and the
             // remesh snap's fmax drops the NaN); here the run stops before the step.
             bool state_finite = isfinite(max_speed);
-            for (int d = 0; d < D; ++d) state_finite = state_finite && isfinite(mn[d]) && isfinite(mx[d]);
-            double mni[3], mxi[3], loi[3], hii[3];
-            const double dx = P.dx;
-            if (S->step == 0) {
-                if (P.grid_reference_point[0] < 1e10) {
-                    for (int d = 0; d < D; ++d) {
-                        if (P.grid_reference_point[d] < mn[d]) mn[d] = P.grid_reference_point[d];
-                        else if (P.grid_reference_point[d] > mx[d]) mx[d] = P.grid_reference_point[d];
-                    }
-                }
-                for (int d = 0; d < D; ++d) {
-                    mxi[d] = mx[d]; mni[d] = mn[d];
-                    loi[d] = mn[d] - dx * 2.0; hii[d] = mx[d] + dx * 2.0;
-                    lo[d] = loi[d]; hi[d] = hii[d];
-                }
-            } else {
-                for (int d = 0; d < D; ++d) {
-                    mni[d] = S->min_init[d]; mxi[d] = S->max_init[d]; loi[d] = S->low_init[d]; hii[d] = S->high_init[d];
-                    if (mx[d] < mxi[d]) { unsigned rf = (unsigned)floor(fmax(0.0, (mxi[d] - mx[d]) / dx - 1e-8 * dx)); hi[d] = hii[d] - rf * dx; }
-                    else { unsigned ef = (unsigned)floor(fmax(0.0, (mx[d] - mxi[d]) / dx - 1e-8 * dx)); hi[d] = hii[d] + ef * dx; }
-                    if (mn[d] > mni[d]) { unsigned rf = (unsigned)floor(fmax(0.0, (mn[d] - mni[d]) / dx - 1e-8 * dx)); lo[d] = loi[d] + rf * dx; }
-                    else { unsigned ef = (unsigned)floor(fmax(0.0, (mni[d] - mn[d]) / dx - 1e-8 * dx)); lo[d] = loi[d] - ef * dx; }
-                }
-            }
-            // node counts of arange(lo, hi + dx, dx); periodic runs keep the fixed grid of remeshFixed(4)
-            // (remesh.cpp:14-36): x and z from -5 dx to L + 6 dx, y from -dx to Ly + 6 dx
+            for (int d = 0; d < D; ++d) state_finite = state_finite && isfinite(mn_s[d]) && isfinite(mx_s[d]);
             uint32_t cnt[3] = {1, 1, 1};
             bool fits = true;
             for (int d = 0; d < D; ++d) {
-                double stop = hi[d] + dx;
-                if (P.pbc) { lo[d] = d == 1 ? -dx : -dx * 5.0; stop = P.L[d] + 6.0 * dx; hi[d] = stop - dx; }
-                uint32_t c = 0;
-                for (double val = lo[d]; val < stop; val += dx) { if (++c > S->cap_axis) break; }
-                cnt[d] = c;
-                if (c > S->cap_axis || c < 1) fits = false;
-                lo_s[d] = lo[d]; stop_s[d] = stop;
+                cnt[d] = cnt_s[d];
+                if (cnt[d] > H.cap_axis || cnt[d] < 1) fits = false;
             }
             unsigned long long nodes = (unsigned long long)cnt[0] * cnt[1] * cnt[2];
-            if (nodes > S->cap_nodes) fits = false;
+            if (nodes > H.cap_nodes) fits = false;
             uint32_t nb[3] = {1, 1, 1};
             for (int d = 0; d < D; ++d) nb[d] = (cnt[d] + BlockEdge<D>::v - 1) / BlockEdge<D>::v;
             const unsigned long long nblocks = (unsigned long long)nb[0] * nb[1] * nb[2];
-            if (nblocks > S->cap_blocks) fits = false;
+            if (nblocks > H.cap_blocks) fits = false;
             bool finite = state_finite;
-            for (int d = 0; d < D; ++d) if (!(lo[d] == lo[d]) || !(hi[d] == hi[d]) || !isfinite(lo[d]) || !isfinite(hi[d]) || cnt[d] < 1) finite = false;
+            for (int d = 0; d < D; ++d)
+                if (!(lo_s[d] == lo_s[d]) || !(hi_s[d] == hi_s[d]) || !isfinite(lo_s[d]) || !isfinite(hi_s[d]) || cnt[d] < 1) finite = false;
             if (!(dt == dt)) finite = false;
             if (!finite) {
-                S->err |= MPM_ERR_NONFINITE; S->stop = 1;
+                H.err |= MPM_ERR_NONFINITE; H.stop = 1;
             } else if (!fits) {
-                S->need_realloc = 1; S->stop = 1;
-                for (int d = 0; d < D; ++d) S->n[d] = cnt[d];
-                S->nblocks = (uint32_t)nblocks;
+                H.need_realloc = 1; H.stop = 1;
+                for (int d = 0; d < D; ++d) H.n[d] = cnt[d];
+                H.nblocks = (uint32_t)nblocks;
             } else {
-                S->dt = dt; S->warn |= warn;
-                for (int d = 0; d < 3; ++d) S->gravity[d] = grav[d];
+                H.dt = dt; H.warn |= warn;
+                for (int d = 0; d < 3; ++d) H.gravity[d] = grav[d];
                 for (int d = 0; d < D; ++d) {
-                    S->min_init[d] = mni[d]; S->max_init[d] = mxi[d]; S->low_init[d] = loi[d]; S->high_init[d] = hii[d];
-                    S->origin[d] = lo[d]; S->n[d] = cnt[d];
+                    H.min_init[d] = mni_s[d]; H.max_init[d] = mxi_s[d]; H.low_init[d] = loi_s[d]; H.high_init[d] = hii_s[d];
+                    H.origin[d] = lo_s[d]; H.n[d] = cnt[d];
                 }
-                if (D == 2) { S->n[2] = 1; S->origin[2] = 0.0; }
-                S->grid_nodes = (uint32_t)nodes;
-                for (int d = 0; d < 3; ++d) S->nb[d] = nb[d];
-                S->nblocks = (uint32_t)nblocks;
+                if (D == 2) { H.n[2] = 1; H.origin[2] = 0.0; }
+                H.grid_nodes = (uint32_t)nodes;
+                for (int d = 0; d < 3; ++d) H.nb[d] = nb[d];
+                H.nblocks = (uint32_t)nblocks;
                 // reset the reductions that this step's particle kernels fill
-                S->vmax2_bits = 0ull;
-                for (int d = 0; d < 3; ++d) { S->xmin_key[d] = 0x7FFFFFFFFFFFFFFFLL; S->xmax_key[d] = (-0x7FFFFFFFFFFFFFFFLL - 1); }
-                S->plastic_count = 0;
-                S->cons_grid_mass = 0; for (int d = 0; d < 3; ++d) { S->cons_grid_mom[d] = 0; S->cons_part_mom[d] = 0; }
+                H.vmax2_bits = 0ull;
+                for (int d = 0; d < 3; ++d) { H.xmin_key[d] = 0x7FFFFFFFFFFFFFFFLL; H.xmax_key[d] = (-0x7FFFFFFFFFFFFFFFLL - 1); }
+                H.plastic_count = 0;
+                H.cons_grid_mass = 0; for (int d = 0; d < 3; ++d) { H.cons_grid_mom[d] = 0; H.cons_part_mom[d] = 0; }
                 go = 1;
             }
         }
     }
     __syncthreads();
-    if (go && threadIdx.x < (unsigned)D) {
-        int d = threadIdx.x;
-        double dx = P.dx, stop = stop_s[d];
+    uint32_t* Swr = reinterpret_cast<uint32_t*>(S);
+    for (int w = threadIdx.x; w < kHeadWords; w += 64) Swr[w] = head_w[w];
+    if (go && t < D) {
+        const int d = t;
+        double stop = stop_s[d];
         uint32_t i = 0;
         for (double val = lo_s[d]; val < stop; val += dx) g.coord[d][i++] = val;
     }
