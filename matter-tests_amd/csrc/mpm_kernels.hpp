@@ -616,7 +616,8 @@ __device__ __forceinline__ uint32_t run_sum(const uint32_t* count, uint32_t b0, 
 __device__ __forceinline__ void bin_scan_body(StepState* S, const SortBufs& sb) {
     if (S->stop) return;
     __shared__ uint32_t wsum[16];
-    const uint32_t nb = S->nblocks << sb.ghosts;
+    const uint32_t nblocks = S->nblocks;  // read once: the stores below could alias S for the compiler
+    const uint32_t nb = nblocks << sb.ghosts;
     const uint32_t per = (nb + 1023) / 1024;
     const uint32_t b0 = min(nb, threadIdx.x * per), b1 = min(nb, b0 + per);
     const uint32_t s = run_sum(sb.count, b0, b1);
@@ -647,7 +648,7 @@ __device__ __forceinline__ void bin_scan_body(StepState* S, const SortBufs& sb) 
         for (int k = 0; k < kScanLoads; ++k) {
             const uint32_t b = c0 + k;
             if (b < b1) {
-                if (sb.ghosts && b == S->nblocks) S->n_ghosts = total - run;  // entries from here on are copies
+                if (sb.ghosts && b == nblocks) S->n_ghosts = total - run;  // entries from here on are copies
                 sb.offset[b] = run; sb.cursor[b] = run; run += v[k];
             }
         }
@@ -2185,14 +2186,20 @@ void k_musl_scatter(Params P, StepState* S, PartStateT<TF> src, PartStateT<TF> d
 // without particles adds nothing (its staging holds stale values, read but not used).
 // with_copies: the tiles of the P2G scatter, which also exist for blocks that hold only periodic copies
 // (the MUSL re-scatter has none).
+// BlockGrid: the step's blocks per axis and block count, read from StepState once per workgroup (inside a loop
+// that stores to global memory the compiler would reload them for every node).
+struct BlockGrid {
+    uint32_t nb[3], nblocks;
+    __device__ __forceinline__ explicit BlockGrid(const StepState* S) : nb{S->nb[0], S->nb[1], S->nb[2]}, nblocks(S->nblocks) {}
+};
 template <int D, int SPL, int NV, class TF>
-__device__ __forceinline__ void sum_staging(const StepState* S, const SortBufs& sb, const StagingT<TF>& stg, const int* gx, double (&s)[NV],
+__device__ __forceinline__ void sum_staging(const BlockGrid& bgrid, const SortBufs& sb, const StagingT<TF>& stg, const int* gx, double (&s)[NV],
                                             bool with_copies = false) {
     using G = BG<D, SPL>;
     static_assert(G::T <= 2 * G::E, "a node lies in the tiles of at most two blocks per axis");
     constexpr int NCB = 1 << D;
-    const uint32_t nb[3] = {S->nb[0], S->nb[1], D == 3 ? S->nb[2] : 1u};
-    const uint32_t nblocks = S->nblocks;
+    const uint32_t nb[3] = {bgrid.nb[0], bgrid.nb[1], D == 3 ? bgrid.nb[2] : 1u};
+    const uint32_t nblocks = bgrid.nblocks;
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int d = 0; d < D; ++d) {
         const int l = gx[d] - (G::T - 1);
@@ -2247,6 +2254,7 @@ __device__ __forceinline__ void grid_update_body(const Params& P, StepState* S, 
     for (int k = threadIdx.x; k < P.n_plates; k += blockDim.x) s_pl[k] = S->plates[k];
     __syncthreads();
     double cm = 0.0, cmv[3] = {0, 0, 0};
+    const BlockGrid bgrid(S);
     for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nn; i += nbid * blockDim.x) {
         int gx[3] = {0, 0, 0};
         if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
@@ -2254,7 +2262,7 @@ __device__ __forceinline__ void grid_update_body(const Params& P, StepState* S, 
         V<D> X;  // loaded with the sums
         for (int d = 0; d < D; ++d) X[d] = g.coord[d][gx[d]];
         double s[NV];
-        sum_staging<D, SPL, NV, TF>(S, sb, stg, gx, s, sb.ghosts != 0);
+        sum_staging<D, SPL, NV, TF>(bgrid, sb, stg, gx, s, sb.ghosts != 0);
         const double sw = s[0];
         V<D> v;
         if (sw > 0) { for (int d = 0; d < D; ++d) v[d] = s[1 + d] / sw; }
@@ -2297,13 +2305,14 @@ __device__ __forceinline__ void musl_update_body(const Params& P, const StepStat
     if (S->stop) return;
     const uint32_t nn = S->grid_nodes;
     const uint32_t n[3] = {S->n[0], S->n[1], S->n[2]};
+    const BlockGrid bgrid(S);
     for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nn; i += nbid * blockDim.x) {
         int gx[3] = {0, 0, 0};
         if (D == 3) { gx[2] = (int)(i % n[2]); const uint32_t r = i / n[2]; gx[1] = (int)(r % n[1]); gx[0] = (int)(r / n[1]); }
         else { gx[1] = (int)(i % n[1]); gx[0] = (int)(i / n[1]); }
         const double mi = g.mass[i];
         double s[D];
-        sum_staging<D, SPL, D, TF>(S, sb, stg, gx, s);  // loaded with the mass
+        sum_staging<D, SPL, D, TF>(bgrid, sb, stg, gx, s);  // loaded with the mass
         if (mi > 0) {
             const double q = mi / P.particle_mass;
             for (int d = 0; d < D; ++d) g.v[d][i] = (TF)(s[d] / q);
