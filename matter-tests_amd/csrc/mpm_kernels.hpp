@@ -342,6 +342,14 @@ template <int D>
 __device__ __forceinline__ uint32_t node_index(int i, int j, int k, const uint32_t* n) {
     return D == 3 ? node_index3(i, j, k, n) : (uint32_t)i * n[1] + (uint32_t)j;
 }
+// The same index computed in 64 bits, for address arithmetic in groups of loads: written in 32 bits, it becomes
+// 64-bit multiply-adds whose upper addend is an undefined register, and when that register is the destination of
+// a load still in flight the compiler waits for every outstanding load before the multiply-add.
+template <int D>
+__device__ __forceinline__ uint64_t node_offset(int i, int j, int k, const uint32_t* n) {
+    const uint64_t r = (uint64_t)(uint32_t)i * n[1] + (uint32_t)j;
+    return D == 3 ? r * n[2] + (uint32_t)k : r;
+}
 
 
 // ----------------------------------------------------------------------------------------------
@@ -726,7 +734,8 @@ template <int D, int SPL> struct BlockCoordLDS {
             if (t < D * LC) {
                 const int d = t / LC, i = t - d * LC;
                 const int gi = BG<D, SPL>::E * bc[d] - 1 + i;
-                c[d][i] = (gi >= 0 && gi < (int)n[d]) ? coord[d][gi] : 0.0;
+                const double* ca = d == 0 ? coord[0] : (d == 1 ? coord[1] : coord[D == 3 ? 2 : 1]);  // selects, not a load
+                c[d][i] = (gi >= 0 && gi < (int)n[d]) ? ca[gi] : 0.0;
             }
             return;
         }
@@ -2449,7 +2458,7 @@ __device__ __forceinline__ void load_tile(TF* s, TF* const* src, const int* bc, 
             bool in = q < G::NT;
             int gx[3] = {0, 0, 0};
             for (int d = 0; d < D; ++d) { gx[d] = G::E * bc[d] + t[d]; in = in && gx[d] < (int)n[d]; }
-            const uint32_t node = in ? node_index<D>(gx[0], gx[1], gx[2], n) : 0u;
+            const uint64_t node = in ? node_offset<D>(gx[0], gx[1], gx[2], n) : 0ull;
 #pragma unroll
             for (int c = 0; c < D; ++c) val[u][c] = in ? src[c][node] : TF(0);
         }
@@ -2956,7 +2965,10 @@ __device__ __forceinline__ void load_slot_tiles(TF* s_v, BlockCoordLDS<D, SPL>* 
         int bc[3];
         block_coords<D>(blo + (uint32_t)ct, S, bc);
         const int gi = G::E * bc[cd] - 1 + ci;
-        cv = (gi >= 0 && gi < (int)n[cd]) ? coord[cd][gi] : 0.0;
+        // the axis's array chosen by selects: indexing the pointer array with a lane's axis would load the
+        // pointer from memory and wait for every load in flight before the coordinate's own
+        const double* ca = cd == 0 ? coord[0] : (cd == 1 ? coord[1] : coord[D == 3 ? 2 : 1]);
+        cv = (gi >= 0 && gi < (int)n[cd]) ? ca[gi] : 0.0;
     }
     const uint32_t total = nt * (uint32_t)G::NT;
     for (uint32_t k0 = tid; k0 < total; k0 += kTileLoads * blockDim.x) {
@@ -2974,7 +2986,7 @@ __device__ __forceinline__ void load_slot_tiles(TF* s_v, BlockCoordLDS<D, SPL>* 
             bool in = e < total;
             int gx[3] = {0, 0, 0};
             for (int d = 0; d < D; ++d) { gx[d] = G::E * bc[d] + tq[d]; in = in && gx[d] < (int)n[d]; }
-            const uint32_t node = in ? node_index<D>(gx[0], gx[1], gx[2], n) : 0u;
+            const uint64_t node = in ? node_offset<D>(gx[0], gx[1], gx[2], n) : 0ull;
 #pragma unroll
             for (int c = 0; c < D; ++c) val[u][c] = in ? src[c][node] : TF(0);
         }
