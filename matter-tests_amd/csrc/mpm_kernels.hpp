@@ -387,6 +387,15 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, Part
 // accumulating sum is sequential per axis) and later builds that axis's node coordinates; lane 0 then takes
 // dt, gravity and the capacity and finiteness decisions.
 // ----------------------------------------------------------------------------------------------
+// Multiplier and shift for the quotient n / d of n < 2^31 by a block count d >= 1 (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", 1994): with l = ceil(log2 d) and m = ceil(2^(31+l) / d),
+// m d - 2^(31+l) < d <= 2^l, so floor(n / d) = floor(n m / 2^(31+l)) for every n < 2^31; m < 2^32 because
+// d > 2^(l-1). The quotient is taken as mulhi(2n, m) >> l (block_div).
+MPM_HD void block_div_magic(uint32_t d, uint32_t& m, uint32_t& l) {
+    l = 0;
+    while (l < 31 && (1ull << l) < (unsigned long long)d) ++l;
+    m = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
+}
 constexpr int kHeadWords = (int)(offsetof(StepState, plates) / 4);
 template <int D, class TF>
 __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const GridPtrsT<TF>& g) {
@@ -485,7 +494,7 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
                 }
                 if (D == 2) { H.n[2] = 1; H.origin[2] = 0.0; }
                 H.grid_nodes = (uint32_t)nodes;
-                for (int d = 0; d < 3; ++d) H.nb[d] = nb[d];
+                for (int d = 0; d < 3; ++d) { H.nb[d] = nb[d]; block_div_magic(nb[d], H.nb_m[d], H.nb_l[d]); }
                 H.nblocks = (uint32_t)nblocks;
                 // reset the reductions that this step's particle kernels fill
                 H.vmax2_bits = 0ull;
@@ -521,11 +530,18 @@ __device__ __forceinline__ void zero_grid_body(const StepState* S, const SortBuf
 template <int = 0>  // a template, so that every translation unit may instantiate it
 __global__ __launch_bounds__(kBlock) void k_zero_grid(const StepState* S, SortBufs sb) { zero_grid_body(S, sb, blockIdx.x, gridDim.x); }
 
-// Block coordinates of linear block b.
+// Block coordinates of linear block b: the quotients by the block counts as multiplies (block_div), about a
+// tenth of the instructions of a division by a run-time divisor.
+__device__ __forceinline__ uint32_t block_div(uint32_t b, uint32_t m, uint32_t l) { return __umulhi(b + b, m) >> l; }
 template <int D>
 __device__ __forceinline__ void block_coords(uint32_t b, const StepState* S, int* bc) {
-    if (D == 3) { bc[2] = (int)(b % S->nb[2]); const uint32_t r = b / S->nb[2]; bc[1] = (int)(r % S->nb[1]); bc[0] = (int)(r / S->nb[1]); }
-    else { bc[1] = (int)(b % S->nb[1]); bc[0] = (int)(b / S->nb[1]); bc[2] = 0; }
+    if (D == 3) {
+        const uint32_t r = block_div(b, S->nb_m[2], S->nb_l[2]), q = block_div(r, S->nb_m[1], S->nb_l[1]);
+        bc[2] = (int)(b - r * S->nb[2]); bc[1] = (int)(r - q * S->nb[1]); bc[0] = (int)q;
+    } else {
+        const uint32_t q = block_div(b, S->nb_m[1], S->nb_l[1]);
+        bc[1] = (int)(b - q * S->nb[1]); bc[0] = (int)q; bc[2] = 0;
+    }
 }
 
 // Block of a particle from the first node of its non-zero window (P2G's clamped base). A particle whose

@@ -361,18 +361,6 @@ MPM_HD void sym_jacobi_rot(M<D>& A, M<D>& U, bool& rotated, double thr) {
     }
 }
 
-template <int D, int i, int j>
-MPM_HD void eig_order(V<D>& lam, M<D>& U) {
-    const bool sw = lam[j] > lam[i];
-    const double a = lam[i], b = lam[j];
-    lam[i] = sw ? b : a; lam[j] = sw ? a : b;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        const double ui = U(k, i), uj = U(k, j);
-        U(k, i) = sw ? uj : ui; U(k, j) = sw ? ui : uj;
-    }
-}
-
 template <int D> MPM_HD bool left_spectral(const M<D>& F, M<D>& U, V<D>& lam, double tol = kSvdTolEigen) {
     M<D> A;
 #pragma unroll
@@ -398,14 +386,11 @@ template <int D> MPM_HD bool left_spectral(const M<D>& F, M<D>& U, V<D>& lam, do
         }
         if (!rotated) { converged = true; break; }
     }
+    // The eigenpairs are left in the order the rotations leave them: everything the caller forms from them
+    // (the trace of the Hencky strain, the return maps' deviatoric parts, U diag(.) U^T) is symmetric in it.
+    double lmin = A(0, 0), lmax = A(0, 0);
 #pragma unroll
-    for (int k = 0; k < D; ++k) lam[k] = A(k, k);
-    eig_order<D, 0, 1>(lam, U);
-    if constexpr (D == 3) {
-        eig_order<D, 1, 2>(lam, U);
-        eig_order<D, 0, 1>(lam, U);
-    }
-    const double lmin = lam[D - 1], lmax = lam[0];
+    for (int k = 0; k < D; ++k) { lam[k] = A(k, k); lmin = fmin(lmin, lam[k]); lmax = fmax(lmax, lam[k]); }
     return converged && lmax < 1.0e300 && lmin >= 1.0e-8 && lmin >= 1.0e-4 * lmax;
 }
 

@@ -96,6 +96,7 @@ struct StepState {
     uint32_t xsel;          // which particle-state buffer is current (toggles per executed step)
     uint32_t nblocks;       // blocks of the counting sort (BlockEdge^dim nodes each)
     uint32_t nb[3];         // blocks per axis
+    uint32_t nb_m[3], nb_l[3];  // division by nb[d] as a multiply (block_div_magic)
     uint32_t cap_blocks;
     // conservation sums (simulation.cpp:292-333), accumulated after P2G when requested
     double cons_grid_mass, cons_grid_mom[3], cons_part_mom[3];

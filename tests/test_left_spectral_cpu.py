@@ -6,7 +6,8 @@ and the plastic update F' = U diag(exp h) W^T is formed as U diag(exp(h) / sqrt 
 Checked: lam = s^2 (s from dgesvd) to 1e-14 of lam_max, U orthogonal and U diag(lam) U^T = F F^T, F' equal to
 U_svd diag(s exp(shift)) W_svd^T and the Hencky stress of h = ln s + shift equal to the one on U_svd, with an
 isotropic shift (-0.3 times the deviatoric part of ln s, the shape of a Drucker-Prager projection), so that
-equal singular values get equal shifts and the result does not depend on the basis of a repeated one. And the
+equal singular values get equal shifts and the result does not depend on the basis of a repeated one, nor on the
+order in which the decomposition leaves the eigenpairs (it does not sort them). And the
 acceptance rule: singular values within a factor 100 of each other and none below 1e-4 (the two-sided SVD
 takes the rest); every deformation gradient near a rotation (granular flow) is accepted.
 """
@@ -57,9 +58,7 @@ def shift_of(s):
 def run(prog, mats):
     lines = []
     for m in mats:
-        s = np.linalg.svd(m, compute_uv=False)
-        lines.append(f"{m.shape[0]} " + " ".join(format(float(x), ".17g") for x in m.flatten(order="F")) + " "
-                     + " ".join(format(float(x), ".17g") for x in shift_of(np.maximum(s, 1e-300))) + "\n")
+        lines.append(f"{m.shape[0]} " + " ".join(format(float(x), ".17g") for x in m.flatten(order="F")) + "\n")
     res = subprocess.run([prog], input="".join(lines), capture_output=True, text=True, check=True).stdout.split("\n")
     out = []
     for m, line in zip(mats, res):
@@ -95,8 +94,7 @@ def test_left_spectral_against_lapack(prog, dim):
         if not ok:
             continue
         accepted += 1
-        assert np.all(np.diff(lam) <= 0)
-        assert np.max(np.abs(lam - s ** 2)) <= 1e-14 * s[0] ** 2, (idx, lam, s ** 2)
+        assert np.max(np.abs(np.sort(lam)[::-1] - s ** 2)) <= 1e-14 * s[0] ** 2, (idx, lam, s ** 2)
         assert np.max(np.abs(U.T @ U - np.eye(dim))) <= 1e-13
         assert np.max(np.abs(U @ np.diag(lam) @ U.T - m @ m.T)) <= 1e-14 * s[0] ** 2
         sh = shift_of(s)
