@@ -23,7 +23,7 @@ using namespace mpm;
 // Every GPU that loads the library holds its own copy until the process ends. Host code must not pass the
 // array's name to a kernel or a copy: that is a host-side stand-in, not a device address. The device address
 // comes from hipGetSymbolAddress, taken once per device with that device current (scratch_address).
-// Nothing in it has to start at zero: every step clears its block histogram (k_zero_grid) and writes its
+// Nothing in it has to start at zero: every step clears its block histogram (k_setup) and writes its
 // grid, cell offsets and staging tiles before it reads them.
 constexpr size_t kScratchBytes = MPM_NOALLOC_SCRATCH_BYTES;
 static __device__ __align__(256) char scratch[kScratchBytes];

@@ -7,7 +7,7 @@
 //
 // Step pipeline (one HIP stream, no host synchronisation inside a batch of steps):
 //   k_setup        1 wavefront: dt (updateDt), grid geometry (remeshFixedInit/Cont + arange), capacity check
-//   k_zero_grid    clear the block histogram
+//   k_zero_grid    clear the block histogram (batched scenes; k_setup does it for a single scene)
 //   k_bin_count    block of every particle + histogram (wavefront-aggregated atomics)
 //   k_bin_scan     exclusive prefix sum of the histogram (one workgroup)
 //   k_bin_scatter  counting-sort permutation: sorted slot -> particle of the previous layout
@@ -36,9 +36,16 @@
 // many steps and let the device stop at an error, a frame end, the final time or a capacity overflow.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <utility>
 #include "mpm_math.hpp"
 
 namespace mpm {
+
+// f(std::integral_constant<int, i>()) for i = 0 .. N-1, for loops whose index must be a compile-time constant.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) { (f(std::integral_constant<int, I>()), ...); }
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) { static_for_impl(f, std::make_integer_sequence<int, N>()); }
 
 // One layout of the particle state (structure of arrays). TF is the type of the transfer quantities
 // (velocity, affine matrix, Kirchhoff stress): double in the plain form, float in the split form. Position,
@@ -147,23 +154,30 @@ __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
     return v;
 }
 
-// Wavefront-aggregated atomicAdd(ctr + key, 1): lanes with equal keys share one atomic; each lane gets
-// its own slot (old value + rank among the lanes with its key). Lanes with active == false take no slot.
+// Wavefront-aggregated atomicAdd(ctr + key, 1): each run of consecutive active lanes with equal keys shares one
+// atomic, and the runs' atomics go out as one instruction (one memory round trip per wavefront, whatever the number
+// of runs: the particles arrive in the previous step's block order, so a wavefront holds one or two runs, and
+// unsorted input costs no more round trips). Each lane gets its own slot: its run's old value plus its rank in the
+// run. Lanes with active == false take no slot. Two runs of one key in a wavefront get disjoint slot ranges in an
+// order the hardware picks; the slot order inside a block is fixed later (order_block).
 __device__ __forceinline__ uint32_t wave_claim(uint32_t* ctr, uint32_t key, bool active) {
     const int lane = threadIdx.x & 63;
-    unsigned long long pending = __ballot(active);
-    uint32_t slot = 0;
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const uint32_t lk = __shfl(key, leader, 64);
-        const unsigned long long same = __ballot(active && key == lk) & pending;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(ctr + lk, (uint32_t)__popcll(same));
-        base = __shfl(base, leader, 64);
-        if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-        pending &= ~same;
+    const uint32_t kprev = __shfl_up(key, 1, 64);
+    const unsigned long long act = __ballot(active);
+    const bool prev_act = lane > 0 && ((act >> (lane - 1)) & 1ull);
+    const bool start = active && !(prev_act && kprev == key);
+    const unsigned long long starts = __ballot(start);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane (all 64 bits for lane 63)
+    const unsigned long long below = starts & upto;
+    const int leader = below ? 63 - __clzll((long long)below) : lane;
+    uint32_t base = 0;
+    if (start) {
+        const unsigned long long after = (starts | ~act) & ~upto;  // the lanes above where the run cannot go on
+        const int end = after ? __ffsll((long long)after) - 1 : 64;
+        base = atomicAdd(ctr + key, (uint32_t)(end - lane));
     }
-    return slot;
+    base = __shfl(base, leader, 64);
+    return active ? base + (uint32_t)(lane - leader) : 0u;
 }
 
 // Per-thread accumulation of the step reductions (max speed for updateDt, bounds for remesh) over the
@@ -387,21 +401,12 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, Part
 // accumulating sum is sequential per axis) and later builds that axis's node coordinates; lane 0 then takes
 // dt, gravity and the capacity and finiteness decisions.
 // ----------------------------------------------------------------------------------------------
-// Multiplier and shift for the quotient n / d of n < 2^31 by a block count d >= 1 (Granlund and Montgomery,
-// "Division by invariant integers using multiplication", 1994): with l = ceil(log2 d) and m = ceil(2^(31+l) / d),
-// m d - 2^(31+l) < d <= 2^l, so floor(n / d) = floor(n m / 2^(31+l)) for every n < 2^31; m < 2^32 because
-// d > 2^(l-1). The quotient is taken as mulhi(2n, m) >> l (block_div).
-MPM_HD void block_div_magic(uint32_t d, uint32_t& m, uint32_t& l) {
-    l = 0;
-    while (l < 31 && (1ull << l) < (unsigned long long)d) ++l;
-    m = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
-}
 constexpr int kHeadWords = (int)(offsetof(StepState, plates) / 4);
-template <int D, class TF>
-__device__ __forceinline__ void setup_body(const Params& P, StepState* S, const GridPtrsT<TF>& g) {
+template <int D, class TF, bool ZERO = false>
+__device__ __forceinline__ void setup_body(const Params& P, StepState* S, const GridPtrsT<TF>& g, const SortBufs* sb = nullptr) {
     __shared__ __attribute__((aligned(16))) uint32_t head_w[kHeadWords];
     __shared__ double lo_s[3], hi_s[3], stop_s[3], mn_s[3], mx_s[3], mni_s[3], mxi_s[3], loi_s[3], hii_s[3];
-    __shared__ uint32_t cnt_s[3];
+    __shared__ uint32_t cnt_s[3], nb_s[3], nbm_s[3], nbl_s[3];
     __shared__ int go;
     StepState& H = *reinterpret_cast<StepState*>(head_w);  // only head fields are used through H
     const uint32_t* Sw = reinterpret_cast<const uint32_t*>(S);
@@ -441,6 +446,11 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
         lo_s[d] = lo; hi_s[d] = hi; stop_s[d] = stop;
         mni_s[d] = mni; mxi_s[d] = mxi; loi_s[d] = loi; hii_s[d] = hii;
     }
+    if (t < 3) {  // blocks along axis t and the multiplier of their quotient
+        const uint32_t nb = t < D ? (cnt_s[t] + BlockEdge<D>::v - 1) / BlockEdge<D>::v : 1u;
+        nb_s[t] = nb;
+        block_div_magic(nb, nbm_s[t], nbl_s[t]);
+    }
     __syncthreads();
     if (t == 0) {
         go = 0;
@@ -471,8 +481,7 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
             }
             unsigned long long nodes = (unsigned long long)cnt[0] * cnt[1] * cnt[2];
             if (nodes > H.cap_nodes) fits = false;
-            uint32_t nb[3] = {1, 1, 1};
-            for (int d = 0; d < D; ++d) nb[d] = (cnt[d] + BlockEdge<D>::v - 1) / BlockEdge<D>::v;
+            const uint32_t nb[3] = {nb_s[0], nb_s[1], nb_s[2]};
             const unsigned long long nblocks = (unsigned long long)nb[0] * nb[1] * nb[2];
             if (nblocks > H.cap_blocks) fits = false;
             bool finite = state_finite;
@@ -494,7 +503,7 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
                 }
                 if (D == 2) { H.n[2] = 1; H.origin[2] = 0.0; }
                 H.grid_nodes = (uint32_t)nodes;
-                for (int d = 0; d < 3; ++d) { H.nb[d] = nb[d]; block_div_magic(nb[d], H.nb_m[d], H.nb_l[d]); }
+                for (int d = 0; d < 3; ++d) { H.nb[d] = nb[d]; H.nb_m[d] = nbm_s[d]; H.nb_l[d] = nbl_s[d]; }
                 H.nblocks = (uint32_t)nblocks;
                 // reset the reductions that this step's particle kernels fill
                 H.vmax2_bits = 0ull;
@@ -514,9 +523,16 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
         uint32_t i = 0;
         for (double val = lo_s[d]; val < stop; val += dx) g.coord[d][i++] = val;
     }
+    if (ZERO && go) {  // the block histogram of this step (k_zero_grid's work, without a launch of its own)
+        const uint32_t nb = H.nblocks << sb->ghosts, n4 = nb / 4;
+        uint4* c4 = reinterpret_cast<uint4*>(sb->count);  // the histogram starts a 256-byte aligned buffer
+        for (uint32_t i = (uint32_t)t; i < n4; i += 64) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t i = 4 * n4 + (uint32_t)t; i < nb; i += 64) sb->count[i] = 0;
+    }
 }
+// The single-scene form also clears the block histogram.
 template <int D, class TF>
-__global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrsT<TF> g) { setup_body<D, TF>(P, S, g); }
+__global__ __launch_bounds__(64) void k_setup(Params P, StepState* S, GridPtrsT<TF> g, SortBufs sb) { setup_body<D, TF, true>(P, S, g, &sb); }
 
 
 // ----------------------------------------------------------------------------------------------
@@ -532,7 +548,6 @@ __global__ __launch_bounds__(kBlock) void k_zero_grid(const StepState* S, SortBu
 
 // Block coordinates of linear block b: the quotients by the block counts as multiplies (block_div), about a
 // tenth of the instructions of a division by a run-time divisor.
-__device__ __forceinline__ uint32_t block_div(uint32_t b, uint32_t m, uint32_t l) { return __umulhi(b + b, m) >> l; }
 template <int D>
 __device__ __forceinline__ void block_coords(uint32_t b, const StepState* S, int* bc) {
     if (D == 3) {
@@ -549,14 +564,9 @@ __device__ __forceinline__ void block_coords(uint32_t b, const StepState* S, int
 // cell within the block (BG::NC when the window leaves the grid). A window that fits the grid starts at
 // most at node n - 3 < E nb, so its block needs no clamping and the cell is the base modulo the block edge;
 // with these coordinates order_block's own window base (k_p2g) is the same.
-template <int D, int SPL, class TF>
-__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrsT<TF>& g, double h1, uint32_t& cell) {
-    const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
-    const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
+template <int D, int SPL>
+__device__ __forceinline__ uint32_t block_of_base(const int* wb, const StepState* S, uint32_t& cell) {
     constexpr uint32_t E = BlockEdge<D>::v;
-    int wb[3];
-    cell = BG<D, SPL>::NC;
-    if (!window_base<D, SPL>(wb, xp, org, nn, GlobalCoords{g.coord, nn}, h1, true)) return 0u;
     uint32_t c[3] = {0, 0, 0};
     bool in = true;
     for (int d = 0; d < D; ++d) {
@@ -569,6 +579,15 @@ __device__ __forceinline__ uint32_t block_key(const double* xp, const StepState*
         cell = D == 3 ? (l0 * E + l1) * E + l2 : l0 * E + l1;
     }
     return D == 3 ? (c[0] * S->nb[1] + c[1]) * S->nb[2] + c[2] : c[0] * S->nb[1] + c[1];
+}
+template <int D, int SPL, class TF>
+__device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrsT<TF>& g, double h1, uint32_t& cell) {
+    const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
+    const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
+    int wb[3];
+    cell = BG<D, SPL>::NC;
+    if (!window_base<D, SPL>(wb, xp, org, nn, GlobalCoords{g.coord, nn}, h1, true)) return 0u;
+    return block_of_base<D, SPL>(wb, S, cell);
 }
 template <int D, int SPL, class TF>
 __device__ __forceinline__ uint32_t block_key(const double* xp, const StepState* S, const GridPtrsT<TF>& g, double h1) {
@@ -587,112 +606,172 @@ __device__ __forceinline__ bool periodic_copy(double Lx, double band, double x0,
 }
 
 // k_bin_count: block of every particle and the histogram (wavefront-aggregated atomics); with periodic
-// boundaries also the block of each particle's periodic copy (histogram entries nblocks + block).
-template <int D, int SPL, class TF>
+// boundaries also the block of each particle's periodic copy (histogram entries nblocks + block). A thread takes
+// PER particles (PER * blockDim apart, so that every load stays coalesced) and issues their position loads, then
+// their coordinate loads, together: the keys of every lane are computed, inactive lanes' from in-bounds dummy
+// positions, so that no branch separates the groups of loads.
+constexpr int kBinPer = 4;  // particles per thread of the single-scene binning kernels
+template <int D, int SPL, class TF, int PER = 1>
 __device__ __forceinline__ void bin_count_body(const Params& P, const StepState* S, const PartStateT<TF>& src, const SortBufs& sb,
                                                const GridPtrsT<TF>& g, uint32_t n, uint32_t bid) {
     if (S->stop) return;
-    const uint32_t p = bid * blockDim.x + threadIdx.x;
-    const bool active = p < n;
-    uint32_t key = 0;
-    double xp[3] = {0, 0, 0};
-    if (active) {
-        for (int d = 0; d < D; ++d) xp[d] = src.x[d][p];
-        uint32_t cell;
-        key = block_key<D, SPL, TF>(xp, S, g, P.one_over_dx, cell);
-        sb.key[p] = key;
-        sb.cellk[p] = (uint8_t)cell;
+    const uint32_t p0 = bid * blockDim.x * PER + threadIdx.x;
+    double xp[PER][3];
+    bool active[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t p = p0 + (uint32_t)k * blockDim.x;
+        active[k] = p < n;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xp[k][d] = d < D && active[k] ? src.x[d][p] : 0.0;
     }
-    (void)wave_claim(sb.count, key, active);
+    // block_key in three passes over the PER particles: window bases, the first window node's coordinate (an
+    // in-bounds, clamped index for every lane), then the window base's block and cell
+    const double org[3] = {S->origin[0], S->origin[1], S->origin[2]};
+    const uint32_t nn[3] = {S->n[0], S->n[1], S->n[2]};
+    const GlobalCoords gc{g.coord, nn};
+    int wb[PER][3];
+    bool ok[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) ok[k] = stencil_base<D>(wb[k], xp[k], org, nn, P.one_over_dx, true);
+    double c0[PER][3];
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+#pragma unroll
+        for (int d = 0; d < D; ++d) c0[k][d] = SPL == 2 ? gc(d, gc.clamp4(d, wb[k][d])) : 0.0;
+    uint32_t key[PER], cell[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+#pragma unroll
+        for (int d = 0; d < D; ++d)  // the shift of window_base
+            if (SPL == 2) wb[k][d] += fabs((xp[k][d] - c0[k][d]) * P.one_over_dx) < 1.5 ? 0 : 1;
+        cell[k] = BG<D, SPL>::NC;
+        key[k] = ok[k] ? block_of_base<D, SPL>(wb[k], S, cell[k]) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t p = p0 + (uint32_t)k * blockDim.x;
+        if (active[k]) { sb.key[p] = key[k]; sb.cellk[p] = (uint8_t)cell[k]; }
+        (void)wave_claim(sb.count, active[k] ? key[k] : 0u, active[k]);
+    }
     if (sb.ghosts) {
-        double gx = 0.0;
-        const bool has = active && periodic_copy(P.L[0], 4.0 * P.dx, xp[0], gx);
-        uint32_t gkey = 0, gcell = BG<D, SPL>::NC;
-        if (has) {
-            xp[0] = gx;
-            gkey = S->nblocks + block_key<D, SPL, TF>(xp, S, g, P.one_over_dx, gcell);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t p = p0 + (uint32_t)k * blockDim.x;
+            double gx = 0.0;
+            const bool has = active[k] && periodic_copy(P.L[0], 4.0 * P.dx, xp[k][0], gx);
+            uint32_t gkey = 0, gcell = BG<D, SPL>::NC;
+            if (has) {
+                xp[k][0] = gx;
+                gkey = S->nblocks + block_key<D, SPL, TF>(xp[k], S, g, P.one_over_dx, gcell);
+            }
+            if (active[k]) { sb.key[n + p] = has ? gkey : kNoGhost; sb.cellk[n + p] = (uint8_t)gcell; }
+            (void)wave_claim(sb.count, gkey, has);
         }
-        if (active) { sb.key[n + p] = has ? gkey : kNoGhost; sb.cellk[n + p] = (uint8_t)gcell; }
-        (void)wave_claim(sb.count, gkey, has);
     }
 }
-template <int D, int SPL, class TF>
+template <int D, int SPL, class TF, int PER>
 __global__ __launch_bounds__(kBlock) void k_bin_count(Params P, const StepState* S, PartStateT<TF> src, SortBufs sb, GridPtrsT<TF> g, uint32_t n) {
-    bin_count_body<D, SPL, TF>(P, S, src, sb, g, n, blockIdx.x);
+    bin_count_body<D, SPL, TF, PER>(P, S, src, sb, g, n, blockIdx.x);
 }
 
-// k_bin_scan: exclusive prefix sum of the block histogram; one workgroup of 1024 threads, each owning a
-// contiguous run of blocks. The run's counts are read kScanLoads at a time as independent loads (one memory
-// latency per group instead of one per block), the run sums are scanned within each wavefront by shuffles and
+// k_bin_scan: exclusive prefix sum of the block histogram; one workgroup of 1024 threads. The counts pass through
+// LDS in chunks of kScanPer * 1024: they are loaded and the offsets stored with consecutive threads on consecutive
+// entries (whole cache lines per wavefront instruction), while the scan itself runs over runs of kScanPer
+// consecutive entries per thread, read from LDS, with the run sums scanned within each wavefront by shuffles and
 // across the 16 wavefronts by the first one.
-constexpr int kScanLoads = 16;
-__device__ __forceinline__ uint32_t run_sum(const uint32_t* count, uint32_t b0, uint32_t b1) {
-    uint32_t s = 0;
-    for (uint32_t c0 = b0; c0 < b1; c0 += kScanLoads) {
-        uint32_t v[kScanLoads];
-#pragma unroll
-        for (int k = 0; k < kScanLoads; ++k) v[k] = c0 + k < b1 ? count[c0 + k] : 0u;
-#pragma unroll
-        for (int k = 0; k < kScanLoads; ++k) s += v[k];
-    }
-    return s;
-}
+constexpr int kScanPer = 16;
+constexpr uint32_t kScanChunk = 1024u * kScanPer;
 __device__ __forceinline__ void bin_scan_body(StepState* S, const SortBufs& sb) {
-    if (S->stop) return;
+    const uint32_t stop = S->stop, nblocks = S->nblocks;  // read together; nblocks read once (the stores alias S)
+    if (stop) return;
+    __shared__ __attribute__((aligned(16))) uint32_t s_cnt[kScanChunk];
     __shared__ uint32_t wsum[16];
-    const uint32_t nblocks = S->nblocks;  // read once: the stores below could alias S for the compiler
+    __shared__ uint32_t s_ghost_off;
     const uint32_t nb = nblocks << sb.ghosts;
-    const uint32_t per = (nb + 1023) / 1024;
-    const uint32_t b0 = min(nb, threadIdx.x * per), b1 = min(nb, b0 + per);
-    const uint32_t s = run_sum(sb.count, b0, b1);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t incl = s;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    if (wid == 0) {
-        uint32_t v = lane < 16 ? wsum[lane] : 0u;
-        for (int o = 1; o < 16; o <<= 1) {
-            const uint32_t t = __shfl_up(v, o, 64);
-            if (lane >= o) v += t;
+    const uint32_t tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += kScanChunk) {
+        const uint32_t m = min(kScanChunk, nb - c0);
+        uint32_t v[kScanPer];
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) { const uint32_t e = (uint32_t)k * 1024u + tid; v[k] = e < m ? sb.count[c0 + e] : 0u; }
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) s_cnt[(uint32_t)k * 1024u + tid] = v[k];
+        __syncthreads();
+        uint4* run = reinterpret_cast<uint4*>(s_cnt + tid * kScanPer);
+        uint32_t r[kScanPer];
+#pragma unroll
+        for (int k = 0; k < kScanPer / 4; ++k) { const uint4 q = run[k]; r[4 * k] = q.x; r[4 * k + 1] = q.y; r[4 * k + 2] = q.z; r[4 * k + 3] = q.w; }
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) { const uint32_t c = r[k]; r[k] = sum; sum += c; }
+        uint32_t incl = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
         }
-        if (lane < 16) wsum[lane] = v;  // inclusive over the wavefronts
-    }
-    __syncthreads();
-    const uint32_t total = wsum[15];
-    uint32_t run = (wid ? wsum[wid - 1] : 0u) + incl - s;
-    for (uint32_t c0 = b0; c0 < b1; c0 += kScanLoads) {
-        uint32_t v[kScanLoads];
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        if (wid == 0) {
+            uint32_t w = lane < 16 ? wsum[lane] : 0u;
+            for (int o = 1; o < 16; o <<= 1) {
+                const uint32_t t = __shfl_up(w, o, 64);
+                if (lane >= o) w += t;
+            }
+            if (lane < 16) wsum[lane] = w;  // inclusive over the wavefronts
+        }
+        __syncthreads();
+        const uint32_t before = carry + (wid ? wsum[wid - 1] : 0u) + incl - sum;
 #pragma unroll
-        for (int k = 0; k < kScanLoads; ++k) v[k] = c0 + k < b1 ? sb.count[c0 + k] : 0u;
+        for (int k = 0; k < kScanPer / 4; ++k)
+            run[k] = make_uint4(before + r[4 * k], before + r[4 * k + 1], before + r[4 * k + 2], before + r[4 * k + 3]);
+        const uint32_t chunk_total = wsum[15];
+        __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kScanLoads; ++k) {
-            const uint32_t b = c0 + k;
-            if (b < b1) {
-                if (sb.ghosts && b == nblocks) S->n_ghosts = total - run;  // entries from here on are copies
-                sb.offset[b] = run; sb.cursor[b] = run; run += v[k];
+        for (int k = 0; k < kScanPer; ++k) {
+            const uint32_t e = (uint32_t)k * 1024u + tid;
+            if (e < m) {
+                const uint32_t off = s_cnt[e];
+                sb.offset[c0 + e] = off; sb.cursor[c0 + e] = off;
+                if (sb.ghosts && c0 + e == nblocks) s_ghost_off = off;  // entries from here on are copies
             }
         }
+        carry += chunk_total;
+        __syncthreads();  // the next chunk overwrites s_cnt and wsum
     }
+    if (sb.ghosts && tid == 0) S->n_ghosts = carry - s_ghost_off;
 }
 template <int = 0>  // a template, so that every translation unit may instantiate it
 __global__ __launch_bounds__(1024) void k_bin_scan(StepState* S, SortBufs sb) { bin_scan_body(S, sb); }
 
 // k_bin_scatter: counting-sort permutation (the order inside a block is fixed later by k_p2g). With
 // periodic boundaries it runs over 2n entries: entry n + p is the periodic copy of p, if any.
+// A thread takes PER entries, PER * blockDim apart, with their loads and their claims issued together.
+template <int PER = 1>
 __device__ __forceinline__ void bin_scatter_body(const StepState* S, const SortBufs& sb, uint32_t n, uint32_t bid) {
     if (S->stop) return;
-    const uint32_t e = bid * blockDim.x + threadIdx.x;
-    const uint32_t key = e < (n << sb.ghosts) ? sb.key[e] : kNoGhost;
-    const bool active = key != kNoGhost;
-    const uint32_t pos = wave_claim(sb.cursor, active ? key : 0u, active);
-    if (active) { sb.perm[pos] = e < n ? e : (kGhostBit | (e - n)); sb.cells[pos] = sb.cellk[e]; }
+    const uint32_t ne = n << sb.ghosts;
+    const uint32_t e0 = bid * blockDim.x * PER + threadIdx.x;
+    uint32_t key[PER], ck[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t e = e0 + (uint32_t)k * blockDim.x;
+        key[k] = e < ne ? sb.key[e] : kNoGhost;
+        ck[k] = e < ne ? sb.cellk[e] : 0u;
+    }
+    uint32_t pos[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) pos[k] = wave_claim(sb.cursor, key[k] != kNoGhost ? key[k] : 0u, key[k] != kNoGhost);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t e = e0 + (uint32_t)k * blockDim.x;
+        if (key[k] != kNoGhost) { sb.perm[pos[k]] = e < n ? e : (kGhostBit | (e - n)); sb.cells[pos[k]] = (uint8_t)ck[k]; }
+    }
 }
-template <int = 0>  // a template, so that every translation unit may instantiate it
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, SortBufs sb, uint32_t n) { bin_scatter_body(S, sb, n, blockIdx.x); }
+template <int PER>
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(const StepState* S, SortBufs sb, uint32_t n) { bin_scatter_body<PER>(S, sb, n, blockIdx.x); }
 
 // k_binning: cell base index and block of every particle of a layout (diagnostic download).
 template <int D, int SPL, class TF>
@@ -997,7 +1076,9 @@ __device__ __forceinline__ void write_tile(const LT& L, const StagingT<TF>& stg,
 // lanes). A wavefront's LDS instructions execute in program order, so each node still receives its terms
 // in a fixed order. The tile is binary64 in the split form too: LDS binary32 atomic adds issue about ten
 // times slower than binary64 ones on gfx950 (measured: ~80 ns against ~7 ns per wavefront instruction
-// per CU for this access pattern).
+// per CU for this access pattern). A read-add-write per node with a compiler barrier between the nodes (the
+// order kept by the program order of the LDS instructions) ran slower than the atomics: the cubic 8-value
+// P2G 1642 -> 1714 us, the quadratic split P2G and fused G2P about 5 % slower.
 template <int D, int SPL, int NV, class TF, class SS = ScatterShape<D, SPL, NV, TF>, class LT = ScatterLDS<D, SPL, NV, TF>>
 __device__ __forceinline__ void add_partials(LT& L, int cell, int a0, int half, const TF (&acc)[SS::SPP][SS::NSH][NV]) {
     using G = BG<D, SPL>;
@@ -1271,8 +1352,15 @@ template <int D, int SPL, bool MU> struct StagedRec {
     __device__ __forceinline__ static constexpr int t(int r, int d) { return OT + r * D + d; }
 };
 
-// Values a thread loads for one (particle, axis) pair of a staged chunk (issued one chunk ahead).
+// Values of one (particle, axis) pair of a staged chunk, read from the chunk's raw values in LDS.
 template <class TF> struct StagePre { double x, mu; TF b[3], t[3], v[3]; };
+// A staged chunk's particle values travel from global memory to LDS without registers (global_load_lds): the
+// loads of the next chunk are issued before the current chunk's accumulation and land during it. Raw value f of
+// chunk slot s is the 32-bit word raw[f * kStageCH + s]; a binary64 value takes two words (low, high).
+__device__ __forceinline__ double raw_f64(const uint32_t* raw, int f, int s) {
+    return __hiloint2double((int)raw[(f + 1) * kStageCH + s], (int)raw[f * kStageCH + s]);
+}
+__device__ __forceinline__ float raw_f32(const uint32_t* raw, int f, int s) { return __uint_as_float(raw[f * kStageCH + s]); }
 
 // Wavefronts of a staged scatter: one first-axis slice each, and the second axis split in halves when a
 // slice's window nodes would need more than 64 partial sums per lane.
@@ -1487,17 +1575,29 @@ struct P2GContrib {
     // ---- staged scatter (3D split form, cubic spline) ----
     using SRec = StagedRec<D, SPL, MIBF>;
     using Pre = StagePre<TF>;
-    // Loads of axis d of source particle p (position, column d of B and of tau, velocity and muI on axis 0).
+    // Raw values of a source particle p: x (6 words), B (9), tau (6), v (3), muI (2, MIBF).
+    static constexpr int NRAW = 24 + (MIBF ? 2 : 0);
+    template <int f>
+    __device__ __forceinline__ bool raw_used() const { return apic || f < 6 || f >= 15; }
+    template <int f>
+    __device__ __forceinline__ const void* raw_addr(uint32_t p, uint32_t) const {
+        if constexpr (f < 6) return reinterpret_cast<const uint32_t*>(src.x[f / 2] + p) + (f & 1);
+        else if constexpr (f < 15) return src.B[f - 6] + p;
+        else if constexpr (f < 21) return src.tau[f - 15] + p;
+        else if constexpr (f < 24) return src.v[f - 21] + p;
+        else return reinterpret_cast<const uint32_t*>(src.muI + p) + (f - 24);
+    }
+    // Axis d of chunk slot s (position, column d of B and of tau, velocity and muI on axis 0).
     template <int d>
-    __device__ __forceinline__ void load_axis(uint32_t p, uint32_t, Pre& q) const {
-        q.x = src.x[d][p];
+    __device__ __forceinline__ void load_axis(const uint32_t* raw, int s, Pre& q) const {
+        q.x = raw_f64(raw, 2 * d, s);
 #pragma unroll
         for (int r = 0; r < D; ++r) {
-            q.b[r] = apic ? src.B[r + d * D][p] : TF(0);
-            q.t[r] = src.tau[r == d ? r : (r + d == 1 ? 3 : (r + d == 2 ? 4 : 5))][p];  // tau[r][d], (xx, yy, zz, xy, xz, yz)
-            q.v[r] = d == 0 ? src.v[r][p] : TF(0);
+            q.b[r] = apic ? raw_f32(raw, 6 + r + d * D, s) : TF(0);
+            q.t[r] = raw_f32(raw, 15 + (r == d ? r : (r + d == 1 ? 3 : (r + d == 2 ? 4 : 5))), s);  // tau[r][d], (xx, yy, zz, xy, xz, yz)
+            q.v[r] = d == 0 ? raw_f32(raw, 21 + r, s) : TF(0);
         }
-        q.mu = (MIBF && d == 0) ? src.muI[p] : 0.0;
+        q.mu = (MIBF && d == 0) ? raw_f64(raw, 24, s) : 0.0;
     }
     // Phase 1: the axis-d part of the record of chunk slot s.
     template <int d>
@@ -1710,13 +1810,23 @@ struct MuslContrib {
     // ---- staged scatter (3D split form, cubic spline): the momentum sum w2 X(a,b) + w0 w1 Z(c) ----
     using SRec = StagedRec<D, SPL, false>;
     using Pre = StagePre<TF>;
+    // Raw values: x of source particle p (6 words), B (9) and v (3) of destination slot i.
+    static constexpr int NRAW = 18;
+    template <int f>
+    __device__ __forceinline__ bool raw_used() const { return apic || f < 6 || f >= 15; }
+    template <int f>
+    __device__ __forceinline__ const void* raw_addr(uint32_t p, uint32_t i) const {
+        if constexpr (f < 6) return reinterpret_cast<const uint32_t*>(src.x[f / 2] + p) + (f & 1);
+        else if constexpr (f < 15) return dst.B[f - 6] + i;
+        else return dst.v[f - 15] + i;
+    }
     template <int d>
-    __device__ __forceinline__ void load_axis(uint32_t p, uint32_t i, Pre& q) const {
-        q.x = src.x[d][p];
+    __device__ __forceinline__ void load_axis(const uint32_t* raw, int s, Pre& q) const {
+        q.x = raw_f64(raw, 2 * d, s);
 #pragma unroll
         for (int r = 0; r < D; ++r) {
-            q.b[r] = apic ? dst.B[r + d * D][i] : TF(0);
-            q.v[r] = d == 0 ? dst.v[r][i] : TF(0);
+            q.b[r] = apic ? raw_f32(raw, 6 + r + d * D, s) : TF(0);
+            q.v[r] = d == 0 ? raw_f32(raw, 15 + r, s) : TF(0);
             q.t[r] = TF(0);
         }
         q.mu = 0.0;
@@ -1842,60 +1952,88 @@ __device__ __forceinline__ uint32_t stage_chunk_end(uint32_t j, uint32_t kb, uin
     return j1;
 }
 
-// Loads of the (particle, axis) pairs of the chunk of slots kb .. kb + m - 1 that thread tid handles (pairs
-// t = tid, tid + blockDim.x, ... of m * D, axis t / m), at most NP of them.
-template <int D, class Contrib, int NP>
-__device__ __forceinline__ void stage_load(const Contrib& cf, const uint32_t* Lp, uint32_t start, uint32_t kb, uint32_t m,
-                                           typename Contrib::Pre (&pre)[NP]) {
+// Issues the raw-value loads of the chunk of slots kb .. kb + m - 1 straight into LDS: load (f, g) brings raw value
+// f of the 64 slots of group g (G = kStageCH / 64 groups) and goes to wavefront (f * G + g) mod NW. Completion is
+// awaited with wait_vmem and made visible to the other wavefronts by a barrier.
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+template <int NW, class Contrib>
+__device__ __forceinline__ void stage_prefetch(const Contrib& cf, lds_u32* raw, const uint32_t* Lp, uint32_t start, uint32_t kb, uint32_t m) {
+    constexpr int G = kStageCH / 64;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Every lane loads: a slot past the chunk (m >= 1) repeats the chunk's last one, into a word nobody reads; no
+    // per-lane condition stays live across the loads. Every group's slots are read from LDS before the first
+    // load (an LDS read after a load to LDS waits for it).
+    uint32_t p[G], i[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const uint32_t s = min((uint32_t)(g * 64 + lane), m - 1u);
+        p[g] = Lp[kb + s];
+        i[g] = start + kb + s;
+    }
+    // one branch on the wavefront index, then that wavefront's loads with compile-time fields
+    static_for<NW>([&](auto WV) {
+        constexpr int w = decltype(WV)::value;
+        if (wave == w) {
+            static_for<Contrib::NRAW * G>([&](auto QV) {
+                constexpr int q = decltype(QV)::value, f = q / G, g = q % G;
+                if constexpr (q % NW == w) {
+                    if (cf.template raw_used<f>())
+                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)cf.template raw_addr<f>(p[g], i[g]),
+                                                         raw + f * kStageCH + g * 64, 4, 0, 0);
+                }
+            });
+        }
+    });
+}
+// Waits for this wavefront's outstanding vector memory operations (the LDS-direct loads among them).
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0); expcnt, lgkmcnt not waited
+
+// Phase 1 of the chunk of m slots: the records, from the raw values. Pair t = tid, tid + blockDim.x, ... (at
+// most NP of them) is axis t / kStageCH of slot t % kStageCH, so that the axis is the same across a wavefront
+// (kStageCH is a multiple of 64) and each wavefront runs one axis's code.
+template <int D, class Contrib, int NP, class TF>
+__device__ __forceinline__ void stage_write(const Contrib& cf, uint32_t m, const uint32_t* raw, TF* rec) {
+    static_assert(kStageCH % 64 == 0, "a wavefront's pairs must share their axis");
 #pragma unroll
     for (int it = 0; it < NP; ++it) {
         const uint32_t t = threadIdx.x + (uint32_t)it * blockDim.x;
-        if (t >= m * D) break;
-        const uint32_t d = t / m, s = t - d * m, p = Lp[kb + s], i = start + kb + s;
+        const uint32_t d = __builtin_amdgcn_readfirstlane(t / (uint32_t)kStageCH), s = t % (uint32_t)kStageCH;
+        if (d >= (uint32_t)D) break;
+        if (s >= m) continue;
+        typename Contrib::Pre q;
         // the axis is a template argument: indexing the particle arrays by a run-time axis would place the
         // kernel's arrays of pointers in private memory
-        if (d == 0) cf.template load_axis<0>(p, i, pre[it]);
-        else if (d == 1) cf.template load_axis<1>(p, i, pre[it]);
-        else cf.template load_axis<2>(p, i, pre[it]);
-    }
-}
-template <int D, class Contrib, int NP, class TF>
-__device__ __forceinline__ void stage_write(const Contrib& cf, uint32_t m, const typename Contrib::Pre (&pre)[NP], TF* rec) {
-#pragma unroll
-    for (int it = 0; it < NP; ++it) {
-        const uint32_t t = threadIdx.x + (uint32_t)it * blockDim.x;
-        if (t >= m * D) break;
-        const uint32_t d = t / m, s = t - d * m;
-        if (d == 0) cf.template stage_axis<0>(pre[it], rec, (int)s);
-        else if (d == 1) cf.template stage_axis<1>(pre[it], rec, (int)s);
-        else cf.template stage_axis<2>(pre[it], rec, (int)s);
+        if (d == 0) { cf.template load_axis<0>(raw, (int)s, q); cf.template stage_axis<0>(q, rec, (int)s); }
+        else if (d == 1) { cf.template load_axis<1>(raw, (int)s, q); cf.template stage_axis<1>(q, rec, (int)s); }
+        else { cf.template load_axis<2>(raw, (int)s, q); cf.template stage_axis<2>(q, rec, (int)s); }
     }
 }
 
 // Phases 1 and 2 over the block's ordered particles, chunk by chunk. All wavefronts see the same 64 cells, hence
-// the same chunks and barriers.
+// the same chunks and barriers. The raw values of chunk k + 1 load into LDS during chunk k's accumulation.
 template <int D, int SPL, int NV, class TF, class Contrib, class LT>
 __device__ __forceinline__ void staged_accumulate(LT& L, const Contrib& cf, uint32_t start,
                                                   TF (&acc)[1][StagedShape<D, SPL, NV>::NSH][NV]) {
     using SS = StagedShape<D, SPL, NV>;
     constexpr int NP = (kStageCH * D + SS::TB - 1) / SS::TB;  // pairs per thread and chunk
+    __shared__ uint32_t raw[Contrib::NRAW * kStageCH];
+    lds_u32* const lraw = (lds_u32*)raw;  // the same array as an LDS-space pointer (the load-to-LDS destination)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a = wave % SS::NG, half = wave / SS::NG;
     const uint32_t n_c = L.cstart[lane + 1] - L.cstart[lane];
     const uint32_t n_max = wave_max_u32(n_c);
     const unsigned long long below = (1ull << lane) - 1ull;
-    typename Contrib::Pre pre[NP];
     uint32_t j = 0, kb = 0, kb1 = 0;
     uint32_t j1 = stage_chunk_end(0, 0, n_max, n_c, kb1);
+    stage_prefetch<SS::WAVES>(cf, lraw, L.p, start, 0u, kb1);
+    wait_vmem();
+    __syncthreads();  // the first chunk's raw values are in LDS
     while (j < n_max) {
-        // The chunk's loads are issued before the barrier; loading the next chunk during this one's accumulation
-        // held its values in registers through it, which no longer fit once the kernel is bounded to 128 VGPRs.
-        stage_load<D>(cf, L.p, start, kb, kb1 - kb, pre);
-        __syncthreads();  // every wavefront is done with the previous chunk's records
-        stage_write<D>(cf, kb1 - kb, pre, L.u.rec);
-        __syncthreads();
+        stage_write<D, Contrib, NP>(cf, kb1 - kb, raw, L.u.rec);
+        __syncthreads();  // the chunk's records are complete and its raw values read
         uint32_t kb2 = kb1;
         const uint32_t j2 = stage_chunk_end(j1, kb1, n_max, n_c, kb2);
+        if (j1 < n_max) stage_prefetch<SS::WAVES>(cf, lraw, L.p, start, kb1, kb2 - kb1);
         uint32_t k = kb;
         for (; j < j1; ++j) {
             const bool has = j < n_c;
@@ -1905,6 +2043,8 @@ __device__ __forceinline__ void staged_accumulate(LT& L, const Contrib& cf, uint
             if (has && !cf.template accumulate_rec<SS>(L.u.rec, (int)(slot - kb), a, half, acc))
                 atomicOr(&L.flag[slot >> 5], 1u << (slot & 31));
         }
+        wait_vmem();
+        __syncthreads();  // every wavefront is done with the records, and the next chunk's raw values are in LDS
         kb = kb1; kb1 = kb2; j1 = j2;
     }
 }

@@ -58,6 +58,28 @@ MPM_HD double log_d(double x) {
     return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lm));
 }
 
+// Multiplier and shift for the quotient n / d of n < 2^31 by a block count d >= 1 (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", 1994): with l = ceil(log2 d) and m = ceil(2^(31+l) / d),
+// m d - 2^(31+l) < d <= 2^l, so floor(n / d) = floor(n m / 2^(31+l)) for every n < 2^31; m < 2^32 because
+// d > 2^(l-1). The quotient is taken as mulhi(2n, m) >> l (block_div). m starts from the binary64 quotient (within
+// one of the exact one) and is corrected with integer products.
+MPM_HD void block_div_magic(uint32_t d, uint32_t& m, uint32_t& l) {
+    l = 0;
+    while (l < 31 && (1ull << l) < (unsigned long long)d) ++l;
+    const unsigned long long p = 1ull << (31 + l);
+    unsigned long long q = (unsigned long long)((double)p / (double)d);
+    while (q * d < p) ++q;
+    while ((q - 1) * d >= p) --q;
+    m = (uint32_t)q;
+}
+MPM_HD uint32_t block_div(uint32_t n, uint32_t m, uint32_t l) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(n + n, m) >> l;
+#else
+    return (uint32_t)(((unsigned long long)(n + n) * m) >> 32) >> l;
+#endif
+}
+
 // 1/x for the split form's arithmetic: the hardware reciprocal estimate refined by two Newton steps (about 1 ulp,
 // against the correctly rounded IEEE division's 10 instructions; each step at least doubles the correct bits of
 // an estimate that starts near single precision). For positive normal x. The plain form keeps IEEE division, so

@@ -56,13 +56,13 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     const uint32_t n = c->n;
     const uint32_t pblocks = (n + kBlock - 1) / kBlock;
     const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 8192);
-    const uint32_t zblocks = (c->act_blocks + kBlock - 1) / kBlock;
-    { LaunchTimer t(c, KS_SETUP); k_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g); }
-    { LaunchTimer t(c, KS_ZERO); k_zero_grid<><<<zblocks, kBlock, 0, s>>>(c->S, c->sb); }
-    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL, TF><<<pblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, g, n); }
+    { LaunchTimer t(c, KS_SETUP); k_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g, c->sb); }
+    constexpr uint32_t kBinSpan = kBlock * kBinPer;  // particles per workgroup of the binning kernels
+    const uint32_t bblocks = (n + kBinSpan - 1) / kBinSpan;
+    { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL, TF, kBinPer><<<bblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, g, n); }
     { LaunchTimer t(c, KS_BIN_SCAN); k_bin_scan<><<<1, 1024, 0, s>>>(c->S, c->sb); }
-    const uint32_t sblocks = (uint32_t)((((uint64_t)n << c->sb.ghosts) + kBlock - 1) / kBlock);  // + periodic copies
-    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<><<<sblocks, kBlock, 0, s>>>(c->S, c->sb, n); }
+    const uint32_t sblocks = (uint32_t)((((uint64_t)n << c->sb.ghosts) + kBinSpan - 1) / kBinSpan);  // + periodic copies
+    { LaunchTimer t(c, KS_BIN_SCATTER); k_bin_scatter<kBinPer><<<sblocks, kBlock, 0, s>>>(c->S, c->sb, n); }
     const bool sep = P.flip_ratio >= -1 || check_cons;
     c->flip_formed = sep;
     if (sep && P.use_mibf) enqueue_p2g<D, SPL, true, true, TF>(c, src, dst, gblocks);
