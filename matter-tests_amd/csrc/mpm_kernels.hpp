@@ -1069,6 +1069,20 @@ __device__ __forceinline__ void write_tile(const LT& L, const StagingT<TF>& stg,
         for (int v = 0; v < NV; ++v) stg.base[(size_t)v * stg.stride + (size_t)b * G::NT + q] = (TF)L.u.tile[v * G::NT + q];
 }
 
+// DPP quad permutation: lane 4k + i takes the value of lane 4k + Pi (every lane active).
+template <int P0, int P1, int P2, int P3, class T>
+__device__ __forceinline__ T quad_perm(T v) {
+    constexpr int ctrl = P0 | (P1 << 2) | (P2 << 4) | (P3 << 6);
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xFFFFFFFFll), ctrl, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), ctrl, 0xF, 0xF, false);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+}
+
 // Adds one wavefront's partial sums acc (slices a0 .. a0+SPP-1, second-axis half `half`) of dual cell
 // `cell` into the tile. At each instruction every lane adds to a different node, but a later instruction
 // may add to a node that another lane updated earlier, so the adds are LDS atomics: a plain load/add/store
@@ -1092,12 +1106,31 @@ __device__ __forceinline__ void add_partials(LT& L, int cell, int a0, int half, 
 #pragma unroll
         for (int obl = 0; obl < SS::OBN; ++obl) {
             const int ob = half * SS::OBN + obl;
-            if (D == 3) {
+            if constexpr (D == 3) {
+                // The W last-axis partial sums of the row of four cells (c0, c1, 0..3) go to nodes c2 .. c2 + W - 1;
+                // they are summed across the row's lanes first (DPP quad permutations), so that lane c2 adds node c2
+                // ("lo": terms oc <= c2 from lanes c2 - oc) and, for c2 <= W - 2, node c2 + 4 ("hi": terms oc > c2
+                // from lanes c2 + 4 - oc): two LDS atomics per value instead of W. Each node's terms are summed in a
+                // fixed order.
+                static_assert(G::E == 4, "rows of four cells: one DPP quad");
+                const int qlo = tile_index<D, SPL>(c[0] + a, c[1] + ob, c[2]), qhi = tile_index<D, SPL>(c[0] + a, c[1] + ob, c[2] + 4);
+                const bool has_hi = c[2] <= G::W - 2;
 #pragma unroll
-                for (int oc = 0; oc < G::W; ++oc) {
-                    const int q = tile_index<D, SPL>(c[0] + a, c[1] + ob, c[2] + oc);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) atomicAdd(&L.u.tile[v * G::NT + q], (double)acc[s][obl * G::W + oc][v]);
+                for (int v = 0; v < NV; ++v) {
+                    const TF* r = &acc[s][obl * G::W][0];
+                    TF lo = r[v], hi = TF(0);
+                    static_for<G::W - 1>([&](auto K) {
+                        constexpr int oc = decltype(K)::value + 1;
+                        const TF x = r[oc * NV + v];
+                        constexpr int d1 = (1 - oc) < 0 ? 0 : (1 - oc), d2 = (2 - oc) < 0 ? 0 : (2 - oc);
+                        constexpr int u1 = (5 - oc) < 3 ? (5 - oc) : 3, u2 = (6 - oc) < 3 ? (6 - oc) : 3;
+                        const TF dn = quad_perm<0, d1, d2, 3 - oc>(x);  // lane i <- i - oc (lanes i < oc unused)
+                        const TF up = quad_perm<4 - oc, u1, u2, 3>(x);  // lane i <- i + 4 - oc (lanes i >= oc unused)
+                        lo += c[2] >= oc ? dn : TF(0);
+                        hi += c[2] < oc ? up : TF(0);
+                    });
+                    atomicAdd(&L.u.tile[v * G::NT + qlo], (double)lo);
+                    if (has_hi) atomicAdd(&L.u.tile[v * G::NT + qhi], (double)hi);
                 }
             } else {
                 const int q = tile_index<D, SPL>(c[0] + a, c[1] + ob, 0);
