@@ -2746,9 +2746,11 @@ template <int D, class TF> struct GridFetch {
 // FLIP (compile time, so that kernels without flip carry no flip loads) adds the flip sum.
 // Used by the split form: the plain form keeps the reference's node order (gather_nodes), so that its
 // sums round as the reference's do and no particle at a yield threshold takes the other branch.
-template <int D, int SPL, bool PIC, bool GRAD, bool FLIP, class TF, class Fetch>
-__device__ __forceinline__ void gather_rows(const Stencil<D, SPL>& st, const double* xp, double h1, const Fetch& fetch, bool apic,
+// AP: APIC known at compile time (0 or 1) or read from apic (-1).
+template <int D, int SPL, bool PIC, bool GRAD, bool FLIP, class TF, class Fetch, int AP = -1>
+__device__ __forceinline__ void gather_rows(const Stencil<D, SPL>& st, const double* xp, double h1, const Fetch& fetch, bool apic_rt,
                                             TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
+    const bool apic = AP < 0 ? apic_rt : AP == 1;
     static_assert(PIC != GRAD, "one gather per kind");
     constexpr int C = Stencil<D, SPL>::cnt;
     constexpr int C2 = D == 3 ? C : 1;
@@ -2865,8 +2867,16 @@ __device__ __forceinline__ void gather_nodes(const Stencil<D, SPL>& st, const do
 template <int D, int SPL, bool PIC, bool GRAD, bool FLIP, class TF, class Fetch>
 __device__ __forceinline__ void gather(const Stencil<D, SPL>& st, const double* xp, double h1, const Fetch& fetch, bool apic,
                                        bool useflip, TF* pic, TF* fl, TF (&Bp)[3][3], TF (&gv)[3][3]) {
-    if constexpr (sizeof(TF) == 4) gather_rows<D, SPL, PIC, GRAD, FLIP, TF>(st, xp, h1, fetch, apic, pic, fl, Bp, gv);
-    else gather_nodes<D, SPL, PIC, GRAD, TF>(st, xp, h1, fetch, apic, useflip, pic, fl, Bp, gv);
+    if constexpr (sizeof(TF) == 4 && PIC) {
+        // one copy of the gather per APIC setting behind a branch on the (wave-uniform) flag: with the flag read
+        // inside, every affine sum of every row took a select
+        if (apic) gather_rows<D, SPL, PIC, GRAD, FLIP, TF, Fetch, 1>(st, xp, h1, fetch, true, pic, fl, Bp, gv);
+        else gather_rows<D, SPL, PIC, GRAD, FLIP, TF, Fetch, 0>(st, xp, h1, fetch, false, pic, fl, Bp, gv);
+    } else if constexpr (sizeof(TF) == 4) {
+        gather_rows<D, SPL, PIC, GRAD, FLIP, TF>(st, xp, h1, fetch, apic, pic, fl, Bp, gv);
+    } else {
+        gather_nodes<D, SPL, PIC, GRAD, TF>(st, xp, h1, fetch, apic, useflip, pic, fl, Bp, gv);
+    }
 }
 
 // Block prologue shared by the gather kernels: returns false when the block has nothing to do.
