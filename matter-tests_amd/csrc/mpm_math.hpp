@@ -362,11 +362,14 @@ MPM_HD void sym_jacobi_rot(M<D>& A, M<D>& U, bool& rotated, double thr) {
     // t = sgn(al) be / g, c = g / hypot(g, be), s = sgn(al) be / hypot(g, be)
     const double al = A(q, q) - A(p, p), be = 2.0 * apq;
     const double r2 = fma(al, al, be * be);
-    const double g = fabs(al) + r2 * rsqrt_d(r2);
+    const double r = r2 * rsqrt_d(r2);
+    const double g = fabs(al) + r;
     const double ig = rsqrt_d(fma(g, g, be * be));
     const double sbe = al < 0 ? -be : be;
     const double c = g * ig, s = sbe * ig;
-    const double tapq = sbe * apq * rcp_d(g);  // g > 0 (split form only: the plain form takes svd())
+    // t a_pq = sbe a_pq / g, and g^2 + be^2 = 2 r g (r^2 = al^2 + be^2), so 1/g = 2 r ig^2: no reciprocal (g > 0;
+    // the split form only: the plain form takes svd())
+    const double tapq = sbe * apq * ((r + r) * (ig * ig));
     A(p, p) = A(p, p) - tapq;
     A(q, q) = A(q, q) + tapq;
     A(p, q) = 0.0; A(q, p) = 0.0;
