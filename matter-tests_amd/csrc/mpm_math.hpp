@@ -116,6 +116,18 @@ MPM_HD double log_d_split(double x) {
     return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, lm));
 }
 
+// 1/sqrt(x) for the split form's Jacobi rotations (positive normal x): the hardware estimate (measured on gfx950:
+// within 2^-24 relative) and one Newton step, within 2^-47 (4.2e-15, measured over 2^20 arguments on [1e-6, 1e6])
+// against the library's 1 ulp in two steps and special-case handling.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline double rsqrt_d_split(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    return y * fma(-0.5 * x * y, y, 1.5);
+}
+#else
+MPM_HD double rsqrt_d_split(double x) { return 1.0 / sqrt(x); }
+#endif
+
 // 1/sqrt(x): the device library's reciprocal square root (a hardware estimate refined to about 1 ulp).
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ inline double rsqrt_d(double x) { return rsqrt(x); }
@@ -362,9 +374,9 @@ MPM_HD void sym_jacobi_rot(M<D>& A, M<D>& U, bool& rotated, double thr) {
     // t = sgn(al) be / g, c = g / hypot(g, be), s = sgn(al) be / hypot(g, be)
     const double al = A(q, q) - A(p, p), be = 2.0 * apq;
     const double r2 = fma(al, al, be * be);
-    const double r = r2 * rsqrt_d(r2);
+    const double r = r2 * rsqrt_d_split(r2);
     const double g = fabs(al) + r;
-    const double ig = rsqrt_d(fma(g, g, be * be));
+    const double ig = rsqrt_d_split(fma(g, g, be * be));
     const double sbe = al < 0 ? -be : be;
     const double c = g * ig, s = sbe * ig;
     // t a_pq = sbe a_pq / g, and g^2 + be^2 = 2 r g (r^2 = al^2 + be^2), so 1/g = 2 r ig^2: no reciprocal (g > 0;
