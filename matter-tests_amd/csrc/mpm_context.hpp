@@ -131,7 +131,9 @@ inline void collect_timings(mpm_context* c) {
 
 // One step of a context for a spline degree and transfer type (mpm_step.hpp, instantiated per
 // (dimension, degree, type) in its own translation unit so that the engine compiles in parallel).
-template <int D, int SPL, class TF> void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons);
+// head: launch the step's k_setup (else the previous step's k_finish_setup did it); tail: end with k_finish
+// (else with k_finish_setup, which also sets up the next step).
+template <int D, int SPL, class TF> void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons, bool head, bool tail);
 
 // Material index of a run: the plastic model on Hencky elasticity, kMatNeoHookean, or a closed-form
 // Perzyna variant (kMatVMViscLinear, kMatDPViscLinear) when the parameters select that branch.

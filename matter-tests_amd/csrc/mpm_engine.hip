@@ -302,17 +302,20 @@ __global__ void k_control(StepState* S, uint32_t stop_at_frame_end, uint32_t che
 }
 
 template <class TF>
-void enqueue_tf(mpm_context* c, uint32_t xsel, bool check_cons) {
+void enqueue_tf(mpm_context* c, uint32_t xsel, bool check_cons, bool head, bool tail) {
     const int D = c->cfg.dim, SPL = c->cfg.spline_degree;
-    if (D == 3 && SPL == 2) enqueue_step<3, 2, TF>(c, xsel, check_cons);
-    else if (D == 3 && SPL == 3) enqueue_step<3, 3, TF>(c, xsel, check_cons);
-    else if (D == 2 && SPL == 2) enqueue_step<2, 2, TF>(c, xsel, check_cons);
-    else enqueue_step<2, 3, TF>(c, xsel, check_cons);
+    if (D == 3 && SPL == 2) enqueue_step<3, 2, TF>(c, xsel, check_cons, head, tail);
+    else if (D == 3 && SPL == 3) enqueue_step<3, 3, TF>(c, xsel, check_cons, head, tail);
+    else if (D == 2 && SPL == 2) enqueue_step<2, 2, TF>(c, xsel, check_cons, head, tail);
+    else enqueue_step<2, 3, TF>(c, xsel, check_cons, head, tail);
 }
 
-void enqueue(mpm_context* c, uint32_t xsel, bool check_cons) {
-    if (c->split) enqueue_tf<float>(c, xsel, check_cons);
-    else enqueue_tf<double>(c, xsel, check_cons);
+// Step j of a batch of `batch` steps: the first launches its own k_setup, the others are set up by the
+// previous step's last launch, and the last ends with k_finish alone.
+void enqueue(mpm_context* c, uint32_t xsel, bool check_cons, uint32_t j, uint32_t batch) {
+    const bool head = j == 0, tail = j + 1 == batch;
+    if (c->split) enqueue_tf<float>(c, xsel, check_cons, head, tail);
+    else enqueue_tf<double>(c, xsel, check_cons, head, tail);
 }
 
 int sync_state(mpm_context* c) {
@@ -869,7 +872,7 @@ int launch_step_graph(mpm_context* c, uint32_t batch, uint32_t xsel, uint32_t fl
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u, check_cons ? 1u : 0u, c->cap_axis,
                                            (uint32_t)c->cap_nodes, c->act_blocks);
-        for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, check_cons);
+        for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, check_cons, j, batch);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (e != hipSuccess || !graph) { if (graph) (void)hipGraphDestroy(graph); return MPM_ERR_HIP; }
@@ -913,7 +916,7 @@ int step_loop(mpm_context* c, uint32_t nsteps, uint32_t flags, mpm_step_info* in
             k_control<<<1, 64, 0, c->stream>>>(c->S, (flags & MPM_STEP_STOP_AT_FRAME_END) ? 1u : 0u,
                                                (flags & MPM_STEP_CHECK_CONSERVATION) ? 1u : 0u, c->cap_axis, (uint32_t)c->cap_nodes,
                                                c->act_blocks);
-            for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, (flags & MPM_STEP_CHECK_CONSERVATION) != 0);
+            for (uint32_t j = 0; j < batch; ++j) enqueue(c, xsel + j, (flags & MPM_STEP_CHECK_CONSERVATION) != 0, j, batch);
         }
         if (hipGetLastError() != hipSuccess) { status |= MPM_ERR_HIP; break; }
         rc = sync_state(c);

@@ -25,6 +25,7 @@
 //                  grid velocity; MUSL: the re-scattered one), F update, return map, Kirchhoff stress for
 //                  the next step (deformationUpdate + plasticity)
 //   k_finish       1 wavefront: moveObjects, time/frame bookkeeping of simulate()
+//                  (k_finish_setup: the same and the next step's k_setup, between the steps of a batch)
 //
 // No floating-point atomics are used: every sum has a fixed order given the particle state, so a step is
 // deterministic and two runs with equal inputs give equal bits. (A block with more than kMaxBP particles
@@ -3358,12 +3359,17 @@ __global__ __launch_bounds__(kBlock) void k_stress_measures(Params P, PartStateT
 // (simulation.cpp:176-200), conservation checks (simulation.cpp:292-333) when requested.
 // ----------------------------------------------------------------------------------------------
 template <int D>
+__device__ __forceinline__ void finish_lane0(const Params& P, StepState* S, uint32_t n_particles, uint32_t pc);
+template <int D>
 __device__ __forceinline__ void finish_body(const Params& P, StepState* S, uint32_t n_particles) {
     if (S->stop) return;
     uint32_t pc = 0;
     for (int s = threadIdx.x; s < kRedSlots; s += 64) { pc += S->red[s].plastic; S->red[s].plastic = 0; }
     pc = wave_sum_u32(pc);
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x == 0) finish_lane0<D>(P, S, n_particles, pc);
+}
+template <int D>
+__device__ __forceinline__ void finish_lane0(const Params& P, StepState* S, uint32_t n_particles, uint32_t pc) {
     S->plastic_count += pc;
     const double dt = S->dt;
     for (int k = 0; k < P.n_plates; ++k) plate_move<D>(S->plates[k], dt, P.frame_dt, S->time);
@@ -3390,5 +3396,14 @@ __device__ __forceinline__ void finish_body(const Params& P, StepState* S, uint3
 }
 template <int D>
 __global__ __launch_bounds__(64) void k_finish(Params P, StepState* S, uint32_t n_particles) { finish_body<D>(P, S, n_particles); }
+// The end of one step and the set-up of the next in one launch (the steps of a batch but its last). The
+// workgroup is one wavefront on one CU, so the barrier's workgroup-scope fence makes lane 0's stores to
+// StepState visible to the set-up's loads.
+template <int D, class TF>
+__global__ __launch_bounds__(64) void k_finish_setup(Params P, StepState* S, GridPtrsT<TF> g, SortBufs sb, uint32_t n_particles) {
+    finish_body<D>(P, S, n_particles);
+    __syncthreads();
+    setup_body<D, TF, true>(P, S, g, &sb);
+}
 
 }  // namespace mpm

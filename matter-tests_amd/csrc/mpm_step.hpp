@@ -46,7 +46,7 @@ template <class Fn>
 void with_material(const Params& P, Fn&& f) { with_material_id(material_of(P), static_cast<Fn&&>(f)); }
 
 template <int D, int SPL, class TF>
-void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
+void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons, bool head, bool tail) {
     hipStream_t s = c->stream;
     const Params& P = c->P;
     const PartStateT<TF>& src = layouts<TF>(c)[xsel & 1];
@@ -56,7 +56,7 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
     const uint32_t n = c->n;
     const uint32_t pblocks = (n + kBlock - 1) / kBlock;
     const uint32_t gblocks = (uint32_t)std::min<uint64_t>((c->cap_nodes + kBlock - 1) / kBlock, 8192);
-    { LaunchTimer t(c, KS_SETUP); k_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g, c->sb); }
+    if (head) { LaunchTimer t(c, KS_SETUP); k_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g, c->sb); }  // else the previous step's last launch
     constexpr uint32_t kBinSpan = kBlock * kBinPer;  // particles per workgroup of the binning kernels
     const uint32_t bblocks = (n + kBinSpan - 1) / kBinSpan;
     { LaunchTimer t(c, KS_BIN_COUNT); k_bin_count<D, SPL, TF, kBinPer><<<bblocks, kBlock, 0, s>>>(P, c->S, src, c->sb, g, n); }
@@ -92,7 +92,11 @@ void enqueue_step(mpm_context* c, uint32_t xsel, bool check_cons) {
         LaunchTimer t(c, KS_DEFGRAD);
         with_material(P, [&](auto mat) { k_defgrad<D, SPL, TF, decltype(mat)::value><<<pblocks, kBlock, 0, s>>>(P, c->S, src, dst, c->sb, g, n); });
     }
-    { LaunchTimer t(c, KS_FINISH); k_finish<D><<<1, 64, 0, s>>>(P, c->S, n); }
+    {
+        LaunchTimer t(c, KS_FINISH);
+        if (tail) k_finish<D><<<1, 64, 0, s>>>(P, c->S, n);
+        else k_finish_setup<D, TF><<<1, 64, 0, s>>>(P, c->S, g, c->sb, n);  // and the next step's k_setup
+    }
 }
 
 template <int D, int SPL, bool SEP, bool MIBF, class TF>
@@ -130,5 +134,5 @@ void enqueue_batch_step(const BatchClass& bc, hipStream_t s) {
 
 }  // namespace mpm
 
-#define MPM_INSTANTIATE_STEP(D, SPL, TF) template void mpm::enqueue_step<D, SPL, TF>(mpm_context*, uint32_t, bool); \
+#define MPM_INSTANTIATE_STEP(D, SPL, TF) template void mpm::enqueue_step<D, SPL, TF>(mpm_context*, uint32_t, bool, bool, bool); \
     template void mpm::enqueue_batch_step<D, SPL, TF>(const mpm::BatchClass&, hipStream_t);
