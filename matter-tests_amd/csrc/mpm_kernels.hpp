@@ -403,12 +403,16 @@ __global__ __launch_bounds__(kBlock) void k_prepare(Params P, StepState* S, Part
 // dt, gravity and the capacity and finiteness decisions.
 // ----------------------------------------------------------------------------------------------
 constexpr int kHeadWords = (int)(offsetof(StepState, plates) / 4);
+// Node coordinates of each axis kept in LDS by the counting pass of arange (the first kCoordStage of them),
+// so that only a longer axis's tail is walked a second time when the coordinates are stored.
+constexpr uint32_t kCoordStage = 2048;
 template <int D, class TF, bool ZERO = false>
 __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const GridPtrsT<TF>& g, const SortBufs* sb = nullptr) {
     __shared__ __attribute__((aligned(16))) uint32_t head_w[kHeadWords];
     __shared__ double lo_s[3], hi_s[3], stop_s[3], mn_s[3], mx_s[3], mni_s[3], mxi_s[3], loi_s[3], hii_s[3];
     __shared__ uint32_t cnt_s[3], nb_s[3], nbm_s[3], nbl_s[3];
     __shared__ int go;
+    __shared__ double s_coord[D][kCoordStage];
     StepState& H = *reinterpret_cast<StepState*>(head_w);  // only head fields are used through H
     const uint32_t* Sw = reinterpret_cast<const uint32_t*>(S);
     for (int w = threadIdx.x; w < kHeadWords; w += 64) head_w[w] = Sw[w];
@@ -442,7 +446,10 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
         double stop = hi + dx;
         if (P.pbc) { lo = d == 1 ? -dx : -dx * 5.0; stop = P.L[d] + 6.0 * dx; hi = stop - dx; }
         uint32_t c = 0;
-        for (double val = lo; val < stop; val += dx) { if (++c > H.cap_axis) break; }
+        for (double val = lo; val < stop; val += dx) {
+            if (++c > H.cap_axis) break;
+            if (c <= kCoordStage) s_coord[d][c - 1] = val;
+        }
         cnt_s[d] = c;
         lo_s[d] = lo; hi_s[d] = hi; stop_s[d] = stop;
         mni_s[d] = mni; mxi_s[d] = mxi; loi_s[d] = loi; hii_s[d] = hii;
@@ -518,11 +525,17 @@ __device__ __forceinline__ void setup_body(const Params& P, StepState* S, const 
     __syncthreads();
     uint32_t* Swr = reinterpret_cast<uint32_t*>(S);
     for (int w = threadIdx.x; w < kHeadWords; w += 64) Swr[w] = head_w[w];
-    if (go && t < D) {
-        const int d = t;
-        double stop = stop_s[d];
-        uint32_t i = 0;
-        for (double val = lo_s[d]; val < stop; val += dx) g.coord[d][i++] = val;
+    if (go) {  // the accumulating sum of arange (tools.hpp:291-296) as the counting pass formed it
+        for (int d = 0; d < D; ++d) {
+            const uint32_t staged = min(cnt_s[d], kCoordStage);
+            for (uint32_t i = (uint32_t)t; i < staged; i += 64) g.coord[d][i] = s_coord[d][i];
+        }
+        if (t < D && cnt_s[t] > kCoordStage) {  // the tail of a longer axis continues the same sum
+            const int d = t;
+            const double stop = stop_s[d];
+            uint32_t i = kCoordStage;
+            for (double val = s_coord[d][kCoordStage - 1] + dx; val < stop; val += dx) g.coord[d][i++] = val;
+        }
     }
     if (ZERO && go) {  // the block histogram of this step (k_zero_grid's work, without a launch of its own)
         const uint32_t nb = H.nblocks << sb->ghosts, n4 = nb / 4;
